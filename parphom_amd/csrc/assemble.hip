@@ -1,0 +1,235 @@
+// K1 / K9: force-constant symmetrisation, minimum-image tables and batched
+// dynamical-matrix assembly D(q) with analytic or central-difference dD/dq.
+//
+// Semantics follow Phonon_Calculations/SRC/dynamical_matrix.F90:200-262
+// (create_dynamical_ij) and force_read.F90:187-189, 223-252.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pph {
+
+// Phi_sym(I, J) = 0.5 Phi_mat(I, J) + 0.5 Phi_mat(J, I) with
+// Phi_mat(3i+a, 3j+b) = fc[j][i][b][a] (C-order (N, N, 3, 3) as stored in the
+// HDF5 dataset; force_read.F90:223-252 maps (I, J) to Fortran coordinate
+// (a, b, i, j)).  Output column-major n x n.
+__global__ void k_symmetrise_fc(const double* __restrict__ fc, double* __restrict__ phi,
+                                int64_t N) {
+  const int64_t n = 3 * N;
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t J = blockIdx.y;
+  if (I >= n) return;
+  const int64_t i = I / 3, a = I % 3, j = J / 3, b = J % 3;
+  const double pij = fc[((j * N + i) * 3 + b) * 3 + a];   // Phi_mat(I, J)
+  const double pji = fc[((i * N + j) * 3 + a) * 3 + b];   // Phi_mat(J, I)
+  phi[I + J * n] = 0.5 * pij + 0.5 * pji;
+}
+
+// Minimum-image tie sets (dynamical_matrix.F90:223-237): for each pair (i, j),
+// bit l of mask[i + j N] is set when image l of atom j lies within
+// tol32 of the smallest of the 9 in-plane image distances.  Ops are kept
+// unfused so the distances round exactly as the host restatement does.
+__global__ void k_image_table(const double* __restrict__ crys, const double* __restrict__ pos,
+                              const double* __restrict__ lat, uint16_t* __restrict__ mask,
+                              int64_t N) {
+#pragma clang fp contract(off)
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= N) return;
+  // lat is column-major 3x3 (row index = lattice vector)
+  double L[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L[r][c] = lat[r + 3 * c];
+  const double cj0 = crys[j], cj1 = crys[j + N], cj2 = crys[j + 2 * N];
+  const double pi0 = pos[i], pi1 = pos[i + N], pi2 = pos[i + 2 * N];
+  double dist[9];
+  double mn = 1e300;
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const double t1 = (double)(l / 3 - 1), t2 = (double)(l % 3 - 1);
+    const double c0 = cj0 + t1, c1 = cj1 + t2, c2 = cj2;
+    const double x = c0 * L[0][0] + c1 * L[1][0] + c2 * L[2][0];
+    const double y = c0 * L[0][1] + c1 * L[1][1] + c2 * L[2][1];
+    const double z = c0 * L[0][2] + c1 * L[1][2] + c2 * L[2][2];
+    const double dx = pi0 - x, dy = pi1 - y, dz = pi2 - z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    dist[l] = d;
+    mn = fmin(mn, d);
+  }
+  const double thr = mn + (double)1.0000000133514320e-10;   // f32-rounded 1e-10
+  uint16_t m = 0;
+#pragma unroll
+  for (int l = 0; l < 9; ++l)
+    if (dist[l] < thr) m |= (uint16_t)(1u << l);
+  mask[i + j * N] = m;
+}
+
+struct PhaseOut {
+  double p0r, p0i;          // sum exp(i theta)
+  double pdr[3], pdi[3];    // analytic derivative sums
+};
+
+// create_dynamical_ij phase sums for one pair and one q (dynamical_matrix.F90:241-255).
+__device__ __forceinline__ void pair_phase(const double* q, double ci0, double ci1, double ci2,
+                                           double cj0, double cj1, double cj2, uint16_t m,
+                                           const double (&L)[3][3], bool deriv, PhaseOut& o) {
+#pragma clang fp contract(off)
+  const double m2pi = -2.0 * 3.1415927410125732;   // -2 * f32(pi)
+  o.p0r = o.p0i = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) o.pdr[a] = o.pdi[a] = 0.0;
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    if (!(m & (1u << l))) continue;
+    const double t1 = (double)(l / 3 - 1), t2 = (double)(l % 3 - 1);
+    const double d0 = (cj0 + t1) - ci0, d1 = (cj1 + t2) - ci1, d2 = cj2 - ci2;
+    const double pre = q[0] * d0 + q[1] * d1 + q[2] * d2;
+    const double th = m2pi * pre;
+    double s, c;
+    sincos(th, &s, &c);
+    o.p0r += c;
+    o.p0i += s;
+    if (deriv) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double r = d0 * L[0][a] + d1 * L[1][a] + d2 * L[2][a];
+        // -exp(i th) * i * r = r sin(th) - i r cos(th)
+        o.pdr[a] += s * r;
+        o.pdi[a] -= c * r;
+      }
+    }
+  }
+}
+
+// Batched D(q) assembly.  One thread per atom pair (i, j) and all q of the
+// batch; Phi_sym's 3x3 block is read once and reused for every q.
+// Outputs (column-major, leading dimension n, matrix stride n*n):
+//   D64 / D32   : D(q) in complex128 / complex64 (either may be null)
+//   dD64 / dD32 : 3 matrices per q (x, y, z) of dD/dq (null = not computed)
+// mode: 0 analytic derivative, 1 central difference (dynamical_matrix.F90:133-149)
+struct AsmArgs {
+  const double* phi;
+  const double* crys;       // N x 3 column-major
+  const double* lat;        // 3 x 3 column-major
+  const double* rmass;      // N: mass of each atom's type
+  const uint16_t* mask;     // N x N
+  const double* q;          // nb x 3 (row per q), reduced coordinates
+  const double* cd_dc;      // 3 x 3: crystal step for each Cartesian direction (row a)
+  double cd_norm;           // |delta|
+  int64_t N, nb, mode;
+  cplx<double>* D64;
+  cplx<float>* D32;
+  cplx<double>* dD64;
+  cplx<float>* dD32;
+};
+
+__global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
+#pragma clang fp contract(off)
+  const int64_t N = A.N, n = 3 * N;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= N) return;
+  double L[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L[r][c] = A.lat[r + 3 * c];
+  const uint16_t m = A.mask[i + j * N];
+  const double mult = (double)__popc((unsigned)m);
+  const double smm = sqrt(A.rmass[i] * A.rmass[j]);
+  const double ci0 = A.crys[i], ci1 = A.crys[i + N], ci2 = A.crys[i + 2 * N];
+  const double cj0 = A.crys[j], cj1 = A.crys[j + N], cj2 = A.crys[j + 2 * N];
+  double ph[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ph[a][b] = A.phi[(3 * i + a) + (3 * j + b) * n];
+
+  const bool want_d = (A.dD64 != nullptr) || (A.dD32 != nullptr);
+  const size_t mstride = (size_t)n * n;
+  for (int64_t iq = 0; iq < A.nb; ++iq) {
+    const double* q = A.q + 3 * iq;
+    PhaseOut po;
+    pair_phase(q, ci0, ci1, ci2, cj0, cj1, cj2, m, L, want_d && A.mode == 0, po);
+    // analytic: (dr, di) = derivative phase sums; central difference:
+    // (dr, di) = phase sum at q + dc/2 and (er, ei) = phase sum at q - dc/2
+    double dr[3], di[3], er[3], ei[3];
+    if (want_d && A.mode == 1) {
+      for (int a = 0; a < 3; ++a) {
+        double qp[3], qm[3];
+        for (int c = 0; c < 3; ++c) {
+          qp[c] = q[c] + A.cd_dc[3 * a + c] / 2.0;
+          qm[c] = q[c] - A.cd_dc[3 * a + c] / 2.0;
+        }
+        PhaseOut pp, pm;
+        pair_phase(qp, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pp);
+        pair_phase(qm, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pm);
+        dr[a] = pp.p0r; di[a] = pp.p0i;
+        er[a] = pm.p0r; ei[a] = pm.p0i;
+      }
+    } else if (want_d) {
+      for (int a = 0; a < 3; ++a) { dr[a] = po.pdr[a]; di[a] = po.pdi[a]; er[a] = ei[a] = 0.0; }
+    }
+    const size_t base = (size_t)iq * mstride;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const size_t idx = base + (size_t)(3 * i + a) + (size_t)(3 * j + b) * n;
+        const double f = ph[a][b];
+        // dij = Phi * phase / multiplicity / sqrt(m_i m_j)
+        const double vr = ((f * po.p0r) / mult) / smm;
+        const double vi = ((f * po.p0i) / mult) / smm;
+        if (A.D64) A.D64[idx] = cplx<double>{vr, vi};
+        if (A.D32) A.D32[idx] = cplx<float>{(float)vr, (float)vi};
+        if (want_d) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            double wr, wi;
+            if (A.mode == 1) {
+              // (D(q+) - D(q-)) / |delta| elementwise
+              const double pr = ((f * dr[d]) / mult) / smm, pi_ = ((f * di[d]) / mult) / smm;
+              const double mr = ((f * er[d]) / mult) / smm, mi = ((f * ei[d]) / mult) / smm;
+              wr = (pr - mr) / A.cd_norm;
+              wi = (pi_ - mi) / A.cd_norm;
+            } else {
+              wr = ((f * dr[d]) / mult) / smm;
+              wi = ((f * di[d]) / mult) / smm;
+            }
+            const size_t didx = (size_t)(3 * iq + d) * mstride + (idx - base);
+            if (A.dD64) A.dD64[didx] = cplx<double>{wr, wi};
+            if (A.dD32) A.dD32[didx] = cplx<float>{(float)wr, (float)wi};
+          }
+        }
+      }
+    }
+  }
+}
+
+void launch_symmetrise_fc(const double* fc_dev, double* phi, int64_t N, hipStream_t s) {
+  const int64_t n = 3 * N;
+  dim3 grid((unsigned)ceil_div(n, 256), (unsigned)n);
+  hipLaunchKernelGGL(k_symmetrise_fc, grid, dim3(256), 0, s, fc_dev, phi, N);
+  PPH_CHECK_LAUNCH();
+}
+
+void launch_image_table(const double* crys, const double* pos, const double* lat, uint16_t* mask,
+                        int64_t N, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(N, 256), (unsigned)N);
+  hipLaunchKernelGGL(k_image_table, grid, dim3(256), 0, s, crys, pos, lat, mask, N);
+  PPH_CHECK_LAUNCH();
+}
+
+void launch_assemble(const AsmParams& P, hipStream_t s) {
+  AsmArgs A;
+  A.phi = P.phi; A.crys = P.crys; A.lat = P.lat; A.rmass = P.rmass; A.mask = P.mask;
+  A.q = P.q; A.cd_dc = P.cd_dc; A.cd_norm = P.cd_norm;
+  A.N = P.N; A.nb = P.nb; A.mode = P.mode;
+  A.D64 = P.D64; A.D32 = P.D32; A.dD64 = P.dD64; A.dD32 = P.dD32;
+  dim3 grid((unsigned)ceil_div(P.N, 256), (unsigned)P.N);
+  hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, s, A);
+  PPH_CHECK_LAUNCH();
+}
+
+}  // namespace pph

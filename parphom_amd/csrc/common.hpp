@@ -1,0 +1,117 @@
+// Shared device/host helpers for the parphom HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <stdexcept>
+
+namespace pph {
+
+// ---------------------------------------------------------------- complex
+template <typename T> struct cplx { T x, y; };
+template <typename T> struct vec2;
+template <> struct vec2<double> { using type = double2; };
+template <> struct vec2<float> { using type = float2; };
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmk(T r, T i) { return cplx<T>{r, i}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return {a.x, -a.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__host__ __device__ __forceinline__ T cabs2(cplx<T> a) { return a.x * a.x + a.y * a.y; }
+
+// ---------------------------------------------------------------- reductions
+constexpr int WAVE = 64;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// Block-wide sum; `sh` must hold blockDim.x / WAVE entries.  All threads get the result.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  for (int i = 0; i < nw; ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = sh[0];
+  for (int i = 1; i < nw; ++i) r = fmax(r, sh[i]);
+  __syncthreads();
+  return r;
+}
+
+// Device-scope atomics used for the "last block finishes" pattern.
+__device__ __forceinline__ unsigned atomic_inc_acqrel(unsigned* p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void atomic_add_relaxed(T* p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T atomic_load_acq(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void atomic_store_rel(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- host errors
+struct hip_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define PPH_HIP(call)                                                             \
+  do {                                                                            \
+    hipError_t _e = (call);                                                       \
+    if (_e != hipSuccess)                                                         \
+      throw ::pph::hip_error(std::string(#call) + ": " + hipGetErrorString(_e) + \
+                             " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define PPH_CHECK_LAUNCH() PPH_HIP(hipGetLastError())
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace pph

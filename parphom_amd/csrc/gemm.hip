@@ -1,0 +1,192 @@
+// Complex GEMM on CDNA4 matrix cores (MFMA), FP64 and FP32, with a fused
+// "diagonal dot" epilogue used by the group-velocity and Rayleigh-quotient
+// products (sum_rows Re(conj(X) .* (A B))) so the product is never stored.
+//
+// A complex product is four real MFMAs per 16x16x4 step:
+//   Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br.
+// Tile: 64x64 per 256-thread workgroup (2x2 waves, each 32x32 = 2x2 MFMA
+// blocks), K staged through LDS in steps of 16.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pph {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct mfma;
+template <> struct mfma<double> {
+  typedef d4 acc;
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct mfma<float> {
+  typedef f4 acc;
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+template <typename T, bool CONJA, bool CONJB, bool DOT>
+__global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
+  using acc_t = typename mfma<T>::acc;
+  __shared__ cplx<T> As[GBK][GBM];
+  __shared__ cplx<T> Bs[GBK][GBN];
+
+  const int tid = threadIdx.x;
+  const int lane = tid % WAVE, wave = tid / WAVE;
+  const int wr0 = (wave / 2) * 32, wc0 = (wave % 2) * 32;
+  const int64_t b = blockIdx.z;
+  const int64_t row0 = (int64_t)blockIdx.x * GBM, col0 = (int64_t)blockIdx.y * GBN;
+  const cplx<T>* A = p.A + b * p.strideA;
+  const cplx<T>* B = p.B + b * p.strideB;
+  const int64_t M = p.m, N = p.n, K = p.k;
+
+  acc_t accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = acc_t{0, 0, 0, 0};
+      accI[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+  cplx<T> ra[4], rb[4];
+  auto load_tiles = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cplx<T> v{0, 0};
+      if (!CONJA) {
+        const int r = tid % 64, kk = tid / 64 + 4 * i;
+        const int64_t gr = row0 + r, gk = k0 + kk;
+        if (gr < M && gk < K) v = A[gr + gk * p.lda];
+      } else {
+        const int kk = tid % 16, r = tid / 16 + 16 * i;
+        const int64_t gr = row0 + r, gk = k0 + kk;
+        if (gr < M && gk < K) v = cconj(A[gk + gr * p.lda]);
+      }
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cplx<T> v{0, 0};
+      if (!CONJB) {
+        const int kk = tid % 16, c = tid / 16 + 16 * i;
+        const int64_t gk = k0 + kk, gc = col0 + c;
+        if (gk < K && gc < N) v = B[gk + gc * p.ldb];
+      } else {
+        const int c = tid % 64, kk = tid / 64 + 4 * i;
+        const int64_t gk = k0 + kk, gc = col0 + c;
+        if (gk < K && gc < N) v = cconj(B[gc + gk * p.ldb]);
+      }
+      rb[i] = v;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (!CONJA) { As[tid / 64 + 4 * i][tid % 64] = ra[i]; }
+      else { As[tid % 16][tid / 16 + 16 * i] = ra[i]; }
+      if (!CONJB) { Bs[tid % 16][tid / 16 + 16 * i] = rb[i]; }
+      else { Bs[tid / 64 + 4 * i][tid % 64] = rb[i]; }
+    }
+  };
+
+  load_tiles(0);
+  for (int64_t k0 = 0; k0 < K; k0 += GBK) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (k0 + GBK < K) load_tiles(k0 + GBK);
+#pragma unroll
+    for (int ks = 0; ks < GBK / 4; ++ks) {
+      const int kk = ks * 4 + lane / 16;
+      cplx<T> a[2], bb[2];
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi) a[mi] = As[kk][wr0 + mi * 16 + lane % 16];
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) bb[ni] = Bs[kk][wc0 + ni * 16 + lane % 16];
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          accR[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].x, accR[mi][ni]);
+          accR[mi][ni] = mfma<T>::op(-a[mi].y, bb[ni].y, accR[mi][ni]);
+          accI[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].y, accI[mi][ni]);
+          accI[mi][ni] = mfma<T>::op(a[mi].y, bb[ni].x, accI[mi][ni]);
+        }
+    }
+  }
+
+  if (!DOT) {
+    cplx<T>* C = p.C + b * p.strideC;
+    const bool beta0 = (p.beta.x == T(0) && p.beta.y == T(0));
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) {
+        const int64_t col = col0 + wc0 + ni * 16 + lane % 16;
+        if (col >= N) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t row = row0 + wr0 + mi * 16 + 4 * (lane / 16) + i;
+          if (row >= M) continue;
+          cplx<T> v{accR[mi][ni][i], accI[mi][ni][i]};
+          v = cmul(p.alpha, v);
+          cplx<T>* c = C + row + col * p.ldc;
+          if (!beta0) v = cadd(v, cmul(p.beta, *c));
+          *c = v;
+        }
+      }
+  } else {
+    const cplx<T>* X = p.dotX + b * p.strideX;
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) {
+      const int64_t col = col0 + wc0 + ni * 16 + lane % 16;
+      double s = 0.0;
+      if (col < N) {
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int64_t row = row0 + wr0 + mi * 16 + 4 * (lane / 16) + i;
+            if (row >= M) continue;
+            const cplx<T> x = X[row + col * p.lddx];
+            // Re(conj(x) * acc)
+            s += (double)x.x * (double)accR[mi][ni][i] + (double)x.y * (double)accI[mi][ni][i];
+          }
+      }
+      s += __shfl_xor(s, 16, WAVE);
+      s += __shfl_xor(s, 32, WAVE);
+      if (lane < 16 && col < N) atomicAdd(p.dot_out + b * p.dot_stride + col, s);
+    }
+  }
+}
+
+template <typename T>
+void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
+  if (p.m <= 0 || p.n <= 0 || p.batch <= 0) return;
+  dim3 grid((unsigned)ceil_div(p.m, GBM), (unsigned)ceil_div(p.n, GBN), (unsigned)p.batch);
+  const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);
+#define PPH_GEMM_CASE(CA, CB, DT)                                                  \
+  if (ca == CA && cb == CB && dot == DT) {                                         \
+    hipLaunchKernelGGL((k_zgemm<T, CA, CB, DT>), grid, dim3(256), 0, s, p);        \
+    PPH_CHECK_LAUNCH();                                                            \
+    return;                                                                        \
+  }
+  PPH_GEMM_CASE(false, false, false)
+  PPH_GEMM_CASE(true, false, false)
+  PPH_GEMM_CASE(false, true, false)
+  PPH_GEMM_CASE(true, true, false)
+  PPH_GEMM_CASE(false, false, true)
+#undef PPH_GEMM_CASE
+  throw std::runtime_error("launch_gemm: unsupported op combination");
+}
+
+template void launch_gemm<double>(const GemmParams<double>&, hipStream_t);
+template void launch_gemm<float>(const GemmParams<float>&, hipStream_t);
+
+}  // namespace pph

@@ -1,0 +1,117 @@
+// Host-side launch interfaces of the HIP kernels.
+#pragma once
+#include "common.hpp"
+
+namespace pph {
+
+// ---------------------------------------------------------------- assembly (K1, K9)
+struct AsmParams {
+  const double* phi;        // n x n column-major, symmetrised force constants
+  const double* crys;       // N x 3 column-major crystal coordinates
+  const double* lat;        // 3 x 3 column-major lattice (row = vector)
+  const double* rmass;      // N masses
+  const uint16_t* mask;     // N x N minimum-image tie sets
+  const double* q;          // nb x 3, row per q-point (reduced coordinates)
+  const double* cd_dc;      // 3 x 3 central-difference crystal steps (row per direction)
+  double cd_norm;
+  int64_t N, nb, mode;      // mode 0 analytic, 1 central difference
+  cplx<double>* D64;
+  cplx<float>* D32;
+  cplx<double>* dD64;
+  cplx<float>* dD32;
+};
+
+void launch_symmetrise_fc(const double* fc_dev, double* phi, int64_t N, hipStream_t s);
+void launch_image_table(const double* crys, const double* pos, const double* lat, uint16_t* mask,
+                        int64_t N, hipStream_t s);
+void launch_assemble(const AsmParams& P, hipStream_t s);
+
+// ---------------------------------------------------------------- complex GEMM (MFMA)
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], column-major, batched by stride.
+// op: 'N' or 'C' (conjugate transpose).
+template <typename T>
+struct GemmParams {
+  char opA, opB;
+  int64_t m, n, k;
+  cplx<T> alpha, beta;
+  const cplx<T>* A; int64_t lda, strideA;
+  const cplx<T>* B; int64_t ldb, strideB;
+  cplx<T>* C; int64_t ldc, strideC;
+  int64_t batch;
+  // diagonal-dot epilogue: when dot_out != null, C is not written; instead
+  // dot_out[b*dot_stride + col] += sum_rows Re(conj(X[row, col]) * (op(A) op(B))[row, col])
+  // with X = dotX (leading dimension lddx, stride strideX).
+  double* dot_out; int64_t dot_stride;
+  const cplx<T>* dotX; int64_t lddx, strideX;
+};
+
+template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);
+
+// ---------------------------------------------------------------- tridiagonalisation
+template <typename T>
+struct TridiagWork {
+  int64_t n, lda, batch, nb;          // nb = panel width
+  cplx<T>* A;  int64_t strideA;       // matrices, overwritten
+  cplx<T>* V;  int64_t strideV;       // n x n: all Householder vectors (explicit, unit at j+1)
+  cplx<T>* W;  int64_t strideW;       // n x nb panel W
+  cplx<T>* tau; int64_t strideTau;    // n
+  T* d; T* e; int64_t strideDE;       // diagonal / off-diagonal (n each)
+  cplx<T>* ypart; int64_t strideY;    // hemv partials: nchunk x n
+  cplx<T>* dpart; int64_t strideDp;   // correction-dot partials: nrb x 2 nb
+  double* scal; int64_t strideScal;   // per-column scalars (norm acc, alpha2, scale...)
+  unsigned* cnt; int64_t strideCnt;   // last-block counters
+  int nchunk;                          // hemv column chunks
+};
+
+template <typename T> void tridiagonalise(const TridiagWork<T>& w, hipStream_t s);
+
+// ---------------------------------------------------------------- tridiagonal eigensolver (FP64)
+struct TriEigWork {
+  int64_t n, batch;
+  const void* d; const void* e; int64_t strideDE; int de_is_float;
+  double* dd; double* ee; double* e2;     // FP64 copies (n each per matrix)
+  int64_t strideT;
+  char range; double vl, vu; int64_t il, iu; double abstol; double orfac;
+  int64_t* ilo; int64_t* ihi;              // per matrix index range found [ilo, ihi)
+  double* tinfo;                           // per matrix: gl, gu, tnorm, pivmin
+  double* w; int64_t ldw;                  // eigenvalues: w[b*ldw + k]
+  int want_vectors;
+  double* zt; int64_t ldzt; int64_t strideZt;   // row-major n x ldzt eigenvectors of T
+  double* fac; int64_t strideFac;               // LU scratch: 4 n x ldzt
+  int64_t* clus; int64_t strideClus;            // cluster starts
+  int* info;                                    // per matrix pzheevx-style bits
+  int64_t mmax;                                 // max eigenvalues per matrix (threads launched)
+};
+
+void tri_prepare(const TriEigWork& w, hipStream_t s);
+void tri_bisect(const TriEigWork& w, hipStream_t s);
+void tri_vectors(const TriEigWork& w, hipStream_t s);
+
+// ---------------------------------------------------------------- back-transformation
+template <typename T>
+struct BackWork {
+  int64_t n, batch, nb, m;                  // m = eigenvectors to transform
+  const cplx<T>* V; int64_t strideV;        // n x n Householder vectors
+  const cplx<T>* tau; int64_t strideTau;
+  const double* zt; int64_t ldzt; int64_t strideZt;   // row-major T eigenvectors
+  cplx<T>* Z; int64_t ldz; int64_t strideZ;            // output n x m column-major
+  cplx<T>* Tm; cplx<T>* VT; cplx<T>* Wt; int64_t strideWs;   // workspaces
+};
+
+template <typename T> void back_transform(const BackWork<T>& w, hipStream_t s);
+
+// ---------------------------------------------------------------- epilogues
+void launch_eval_to_cm(const double* lam, double* out, int64_t nb, int64_t m, int64_t ld,
+                       hipStream_t s);
+void launch_vel_scale(double* v, const double* om_cm, int64_t nb, int64_t m, int64_t ldw,
+                      int64_t ldv, hipStream_t s);
+template <typename T>
+void launch_colnorm2(const cplx<T>* X, int64_t n, int64_t m, int64_t ldx, int64_t strideX,
+                     int64_t batch, double* out, int64_t ldo, hipStream_t s);
+void launch_rq_finish(const double* num, const double* den, double* lam, int64_t nb, int64_t m,
+                      int64_t ld, hipStream_t s);
+template <typename Ti, typename To>
+void launch_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count, hipStream_t s);
+void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);
+
+}  // namespace pph
