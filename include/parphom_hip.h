@@ -1,0 +1,136 @@
+/*
+ * parphom_hip.h - C ABI of the MI355X phonon library (libparphom_hip.so).
+ *
+ * Drop-in for the per-q hot path of parphom.x: dynamical-matrix assembly,
+ * the pzheevx eigensolve and the analytic group velocity.  A host program
+ * (the Fortran driver through ISO_C_BINDING, Python through ctypes) keeps
+ * reading the input files and writing the HDF5 output; it hands the
+ * geometry, the (N,N,3,3) force constants and the q-points to this library.
+ *
+ * Conventions
+ *   - All integers are 64-bit; arrays are passed as plain pointers.
+ *   - Two-dimensional arrays are column-major (Fortran order), so a Fortran
+ *     caller passes its arrays unchanged.
+ *   - Complex numbers are interleaved (re, im) doubles (Fortran
+ *     double complex, C double _Complex).
+ *   - Return codes: 0 success, < 0 error (text via pph_last_error).  Nothing
+ *     in this library exits the process.
+ *   - A handle is bound to one GPU and is not re-entrant; calls on one
+ *     handle must not overlap.  pph_run* are synchronous at return.
+ */
+#ifndef PARPHOM_HIP_H
+#define PARPHOM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPH_OK 0
+#define PPH_ERR_ARG (-1)
+#define PPH_ERR_HIP (-2)
+#define PPH_ERR_MEMORY (-3)
+#define PPH_ERR_INTERNAL (-4)
+
+/* per-q info bits, with pzheevx's meaning (diagonalize.F90:100-115) */
+#define PPH_INFO_NOCONV 1     /* an eigenvalue failed to converge */
+#define PPH_INFO_ORTH 2       /* eigenvectors of a cluster could not be orthogonalised */
+#define PPH_INFO_MEMORY 4     /* not all eigenvectors fit the caller's evec_cols */
+#define PPH_INFO_NOTCOMP 8    /* some eigenvalues were not computed */
+
+typedef struct pph_handle pph_handle;
+
+/* Geometry: the fields of parphom's `moire` state after read_lammps_data
+ * (mod_variables.F90:83-89, lammps_read.F90:28-243). */
+typedef struct pph_system {
+  int64_t natom;          /* N */
+  int64_t ntypes;
+  const double* lat;      /* 3x3, lat(i,k) = component k of lattice vector i, at lat[i + 3k] */
+  const double* pos;      /* N x 3 Cartesian positions (A), pos(i,k) at pos[i + N k] */
+  const double* mass;     /* ntypes masses (amu) */
+  const int64_t* type;    /* N atom types, 1-based */
+} pph_system;
+
+/* Options: the pzheevx_vars / velocity settings of read_input.F90:368-516. */
+typedef struct pph_opts {
+  char jobz;              /* 'N' eigenvalues only, 'V' with eigenvectors (forced by vel) */
+  char range;             /* 'A' all, 'V' interval, 'I' index range */
+  char vel_method;        /* 'A' analytic dD/dq, 'C' central difference */
+  char precision;         /* 'D' FP64 throughout, 'S' FP32 matrix cores + FP64 refinement */
+  int32_t vel;            /* 1: group velocities */
+  int32_t device;         /* HIP device ordinal */
+  double vl, vu;          /* range 'V' bounds in the input file's units (cm^-1); like
+                             parphom.x they are divided by f32(15.633302*33.35641) and
+                             compared with lambda = omega^2 (diagonalize.F90:73-74) */
+  int64_t il, iu;         /* range 'I', 1-based inclusive */
+  double abstol, orfac;   /* pzheevx ABSTOL / ORFAC */
+  int64_t batch;          /* q-points per device batch, 0 = automatic */
+} pph_opts;
+
+/* Outputs of pph_run (host pointers) or pph_run_device (device pointers).
+ * Per q-point iq (0-based):
+ *   eval_cm[iq*ld_eval + k], k < m[iq]: signed frequencies in cm^-1, ascending
+ *       (write_output.F90:135), the `eigenvalues` dataset;
+ *   evec (optional): complex n x nz[iq] column-major at evec + iq*n*evec_cols,
+ *       the bytes of the HDF5 `evec` dataset (h5py shape (nz, n));
+ *   vel (optional): vel[(iq*evec_cols + k)*3 + a] in m/s, the `vel` dataset
+ *       (h5py shape (nz, 3), write_output.F90:397-446);
+ *   info[iq]: PPH_INFO_* bits. */
+typedef struct pph_out {
+  double* eval_cm;
+  int64_t ld_eval;
+  int64_t* m;
+  int64_t* nz;
+  void* evec;
+  int64_t evec_cols;
+  double* vel;
+  int32_t* info;
+} pph_out;
+
+/* Build the device state: symmetrised force constants (force_read.F90:187-189,
+ * replaces read_force_constants + pdgeadd), crystal coordinates and reciprocal
+ * lattice (lammps_read.F90:198-228), minimum-image tables.
+ * fc: C-order (N,N,3,3) float64, exactly the HDF5 `force_constants` dataset
+ * (a Fortran caller's fc(3,3,N,N) array has the same bytes). */
+int pph_create(const pph_system* sys, const double* fc, const pph_opts* opts, pph_handle** h);
+
+/* The q loop of calculations.F90:49-53 (create_dynamical_matrix,
+ * diagonalize_dynamical_matrix, the velocity block of write_output.F90:410-466)
+ * for nq q-points in reduced coordinates, q(iq,k) at q[iq + ldq k]
+ * (q_file%points). Outputs are host pointers. */
+int pph_run(pph_handle* h, int64_t nq, const double* q, int64_t ldq, pph_out* out);
+
+/* As pph_run, with every output pointer in `out` a device pointer on the
+ * handle's GPU (q stays a host pointer). */
+int pph_run_device(pph_handle* h, int64_t nq, const double* q, int64_t ldq, pph_out* out);
+
+/* Dynamical matrices alone (create_dynamical_matrix, dynamical_matrix.F90:49-191).
+ * what: 0 = D(q); 1, 2, 3 = dD/dq_x,y,z with the handle's vel_method.
+ * out: complex128 n x n column-major per q at out + iq*n*n (host or device
+ * pointer per `on_device`). */
+int pph_assemble(pph_handle* h, int64_t nq, const double* q, int64_t ldq, int32_t what, void* out,
+                 int32_t on_device);
+
+/* The eigensolver alone (pzheevx semantics, diagonalize.F90:73-96) on nmat
+ * Hermitian n x n complex128 matrices (column-major, leading dimension n,
+ * consecutive in memory; destroyed).  Precision, jobz, range, vl, vu
+ * (already in lambda units here), il, iu, abstol and orfac come from opts.
+ * w[imat*n + k] eigenvalues (lambda, ascending), Z optional n x n per matrix,
+ * m[imat] count found, info[imat] bits.  All pointers are host or device
+ * per `on_device`. */
+int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w, void* Z,
+              int64_t* m, int32_t* info, int32_t on_device);
+
+int pph_destroy(pph_handle* h);
+
+/* Last error text for a handle (or the thread's last error when h is NULL). */
+const char* pph_last_error(const pph_handle* h);
+
+/* Library version string. */
+const char* pph_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARPHOM_HIP_H */

@@ -1,0 +1,25 @@
+/*
+ * parphom_hip_debug.h - test hook of libparphom_hip.so (not part of the
+ * parphom.x drop-in surface; no counterpart in the reference).
+ *
+ * pph_debug_zgemm runs the library's matrix-core complex GEMM once on host
+ * data so the GEMM can be checked in isolation:
+ *   mode 0: C = op(A) op(B)
+ *   mode 1: dot[j] = sum_i Re(conj(C_in[i,j]) (op(A) op(B))[i,j]), C unchanged
+ *   mode 2: C = op(A) op(B) through the split-K path (4 pieces)
+ *   mode 3: C = op(A) op(B) through the two-product path (opA = opB = 'N')
+ * prec 'D' complex128, 'S' complex64; column-major; returns 0 or < 0.
+ */
+#ifndef PARPHOM_HIP_DEBUG_H
+#define PARPHOM_HIP_DEBUG_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t k,
+                    const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                    int64_t ldc, int32_t mode, double* dot);
+#ifdef __cplusplus
+}
+#endif
+#endif

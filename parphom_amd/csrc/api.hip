@@ -1,0 +1,699 @@
+// C ABI of libparphom_hip.so: handle life cycle and the batched per-q pipeline
+//   K1 assembly -> tridiagonal reduction -> bisection / inverse iteration ->
+//   back-transformation -> (FP32 path) FP64 Rayleigh-quotient refinement ->
+//   velocity products -> unit conversion.
+#include "../../include/parphom_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace pph {
+
+static thread_local std::string g_last_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw std::bad_alloc();
+    }
+    bytes = b;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct Geometry {
+  // host copies following lammps_read.F90:198-228 (linsolve / matinvtransp3)
+  int64_t N = 0;
+  double lat[9];                 // column-major, lat(i,k) at [i + 3k]
+  std::vector<double> crys, pos; // N x 3 column-major
+  double rec[9];                 // 2 pi inv(lat)^T, rows b_i, column-major
+  std::vector<double> rmass;
+};
+
+static inline double L(const double* a, int i, int k) { return a[i + 3 * k]; }
+
+// inverse of a column-major 3x3 by cofactors, ordered as lammps_read.F90:281-309
+static void inv3(const double* A, double* B /* column-major */, double scale) {
+  const double detinv =
+      1.0 / (L(A, 0, 0) * L(A, 1, 1) * L(A, 2, 2) - L(A, 0, 0) * L(A, 1, 2) * L(A, 2, 1) -
+             L(A, 0, 1) * L(A, 1, 0) * L(A, 2, 2) + L(A, 0, 1) * L(A, 1, 2) * L(A, 2, 0) +
+             L(A, 0, 2) * L(A, 1, 0) * L(A, 2, 1) - L(A, 0, 2) * L(A, 1, 1) * L(A, 2, 0));
+  auto S = [&](int i, int k, double v) { B[i + 3 * k] = v * scale; };
+  S(0, 0, +detinv * (L(A, 1, 1) * L(A, 2, 2) - L(A, 1, 2) * L(A, 2, 1)));
+  S(1, 0, -detinv * (L(A, 1, 0) * L(A, 2, 2) - L(A, 1, 2) * L(A, 2, 0)));
+  S(2, 0, +detinv * (L(A, 1, 0) * L(A, 2, 1) - L(A, 1, 1) * L(A, 2, 0)));
+  S(0, 1, -detinv * (L(A, 0, 1) * L(A, 2, 2) - L(A, 0, 2) * L(A, 2, 1)));
+  S(1, 1, +detinv * (L(A, 0, 0) * L(A, 2, 2) - L(A, 0, 2) * L(A, 2, 0)));
+  S(2, 1, -detinv * (L(A, 0, 0) * L(A, 2, 1) - L(A, 0, 1) * L(A, 2, 0)));
+  S(0, 2, +detinv * (L(A, 0, 1) * L(A, 1, 2) - L(A, 0, 2) * L(A, 1, 1)));
+  S(1, 2, -detinv * (L(A, 0, 0) * L(A, 1, 2) - L(A, 0, 2) * L(A, 1, 0)));
+  S(2, 2, +detinv * (L(A, 0, 0) * L(A, 1, 1) - L(A, 0, 1) * L(A, 1, 0)));
+}
+
+// u = inv(A)^T x  (linsolve)
+static void linsolve(const double* x, const double* A, double* u) {
+  double B[9];
+  inv3(A, B, 1.0);
+  for (int k = 0; k < 3; ++k) u[k] = L(B, 0, k) * x[0] + L(B, 1, k) * x[1] + L(B, 2, k) * x[2];
+}
+
+struct Handle {
+  int device = 0;
+  pph_opts opts{};
+  Geometry g;
+  int64_t N = 0, n = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // persistent device state
+  DevBuf phi, crys, pos, lat, rmass, mask, cd_dc;
+  double cd_norm = 0.0;
+  // batch workspaces
+  DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, cnt;
+  DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
+  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om;
+  int nb_panel = 32;
+  int nchunk = 64;
+  ~Handle() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+
+// ---------------------------------------------------------------- eigen pipeline
+struct EigResult {
+  std::vector<int64_t> M;     // per matrix
+  std::vector<int32_t> info;
+  int64_t mcap = 0;           // leading dimension of w / columns of Z
+  double* lam = nullptr;      // device, B x mcap (lambda)
+  cplx<double>* Z64 = nullptr;  // device, B x n x mcap (when vectors)
+  void* Zt = nullptr;           // device, vectors in the working precision
+};
+
+static int64_t range_mcap(const pph_opts& o, int64_t n) {
+  if (o.range == 'I') return std::max<int64_t>(1, std::min(n, o.iu) - std::max<int64_t>(1, o.il) + 1);
+  return n;
+}
+
+// A: B matrices (n x n, working precision, destroyed); D64: FP64 matrices for
+// the refinement (FP32 path only).  want_vec: caller needs eigenvectors.
+template <typename T>
+static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64, bool want_vec,
+                      double vl_l, double vu_l, EigResult& R) {
+  const int64_t n = h.n;
+  hipStream_t s = h.stream;
+  const pph_opts& o = h.opts;
+  const bool fp32 = std::is_same<T, float>::value;
+  const bool vectors = want_vec || fp32;      // the FP32 path refines with its vectors
+  const int nb = h.nb_panel;
+  const int64_t mcap = range_mcap(o, n);
+  R.mcap = mcap;
+
+  // ---- tridiagonalisation
+  h.V.ensure((size_t)B * n * (n + 64) * sizeof(cplx<T>));
+  h.W.ensure((size_t)B * n * nb * sizeof(cplx<T>));
+  h.tau.ensure((size_t)B * n * sizeof(cplx<T>));
+  h.dgl.ensure((size_t)B * n * sizeof(T));
+  h.egl.ensure((size_t)B * n * sizeof(T));
+  h.ypart.ensure((size_t)B * h.nchunk * n * sizeof(cplx<T>));
+  const int64_t nrb_max = ceil_div(n, 64) + 1;
+  h.dpart.ensure((size_t)B * nrb_max * 2 * nb * sizeof(cplx<T>));
+  h.scal.ensure((size_t)B * 256 * sizeof(double));
+  h.cnt.ensure((size_t)B * 4 * sizeof(unsigned));
+  TridiagWork<T> tw{};
+  tw.n = n; tw.lda = n; tw.batch = B; tw.nb = nb;
+  tw.A = A; tw.strideA = n * n;
+  tw.V = h.V.as<cplx<T>>(); tw.strideV = n * (n + 64);
+  tw.W = h.W.as<cplx<T>>(); tw.strideW = n * nb;
+  tw.tau = h.tau.as<cplx<T>>(); tw.strideTau = n;
+  tw.d = h.dgl.as<T>(); tw.e = h.egl.as<T>(); tw.strideDE = n;
+  tw.ypart = h.ypart.as<cplx<T>>(); tw.strideY = (int64_t)h.nchunk * n;
+  tw.dpart = h.dpart.as<cplx<T>>(); tw.strideDp = nrb_max * 2 * nb;
+  tw.scal = h.scal.as<double>(); tw.strideScal = 256;
+  tw.cnt = h.cnt.as<unsigned>(); tw.strideCnt = 4;
+  tw.nchunk = h.nchunk;
+  tridiagonalise<T>(tw, s);
+
+  // ---- tridiagonal eigenproblem in FP64
+  h.tdd.ensure((size_t)B * n * sizeof(double));
+  h.tee.ensure((size_t)B * n * sizeof(double));
+  h.te2.ensure((size_t)B * n * sizeof(double));
+  h.tilo.ensure((size_t)B * sizeof(int64_t));
+  h.tihi.ensure((size_t)B * sizeof(int64_t));
+  h.tinfo.ensure((size_t)B * 4 * sizeof(double));
+  h.tw.ensure((size_t)B * mcap * sizeof(double));
+  h.tinf.ensure((size_t)B * sizeof(int));
+  TriEigWork te{};
+  te.n = n; te.batch = B;
+  te.d = tw.d; te.e = tw.e; te.strideDE = n; te.de_is_float = fp32 ? 1 : 0;
+  te.dd = h.tdd.as<double>(); te.ee = h.tee.as<double>(); te.e2 = h.te2.as<double>();
+  te.strideT = n;
+  te.range = o.range; te.vl = vl_l; te.vu = vu_l; te.il = o.il; te.iu = o.iu;
+  te.abstol = o.abstol; te.orfac = o.orfac;
+  te.ilo = h.tilo.as<int64_t>(); te.ihi = h.tihi.as<int64_t>();
+  te.tinfo = h.tinfo.as<double>();
+  te.w = h.tw.as<double>(); te.ldw = mcap;
+  te.info = h.tinf.as<int>();
+  te.want_vectors = vectors ? 1 : 0;
+  tri_prepare(te, s);
+  std::vector<int64_t> ilo(B), ihi(B);
+  PPH_HIP(hipMemcpyAsync(ilo.data(), te.ilo, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipMemcpyAsync(ihi.data(), te.ihi, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
+  R.M.assign(B, 0);
+  R.info.assign(B, 0);
+  int64_t mmax = 0;
+  for (int64_t b = 0; b < B; ++b) {
+    R.M[b] = std::min(mcap, ihi[b] - ilo[b]);
+    mmax = std::max(mmax, R.M[b]);
+  }
+  te.mmax = std::max<int64_t>(mmax, 1);
+  tri_bisect(te, s);
+  R.lam = te.w;
+
+  if (vectors && mmax > 0) {
+    const int64_t ldzt = mcap;
+    h.tzt.ensure((size_t)B * n * ldzt * sizeof(double));
+    h.tfac.ensure((size_t)B * 5 * n * ldzt * sizeof(double));
+    te.zt = h.tzt.as<double>(); te.ldzt = ldzt; te.strideZt = n * ldzt;
+    te.fac = h.tfac.as<double>(); te.strideFac = 5 * n * ldzt;
+    tri_vectors(te, s);
+
+    // ---- back-transformation
+    h.Z.ensure((size_t)B * n * mcap * sizeof(cplx<T>));
+    const int64_t strideWs = (n + nb) * nb;
+    h.Tm.ensure((size_t)B * strideWs * sizeof(cplx<T>));
+    h.VT.ensure((size_t)B * std::max<int64_t>(strideWs, n * nb) * sizeof(cplx<T>));
+    h.Wt.ensure((size_t)B * std::max<int64_t>(strideWs, nb * mcap) * sizeof(cplx<T>));
+    BackWork<T> bw{};
+    bw.n = n; bw.batch = B; bw.nb = nb; bw.m = mmax;
+    bw.V = tw.V; bw.strideV = tw.strideV;
+    bw.tau = tw.tau; bw.strideTau = n;
+    bw.zt = te.zt; bw.ldzt = ldzt; bw.strideZt = te.strideZt;
+    bw.Z = h.Z.as<cplx<T>>(); bw.ldz = n; bw.strideZ = n * mcap;
+    bw.Tm = h.Tm.as<cplx<T>>(); bw.VT = h.VT.as<cplx<T>>(); bw.Wt = h.Wt.as<cplx<T>>();
+    bw.strideWs = std::max<int64_t>(strideWs, std::max<int64_t>(n * nb, nb * mcap));
+    // Tm/VT/Wt were sized with the same per-matrix stride
+    h.Tm.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
+    h.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
+    h.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
+    bw.Tm = h.Tm.as<cplx<T>>(); bw.VT = h.VT.as<cplx<T>>(); bw.Wt = h.Wt.as<cplx<T>>();
+    back_transform<T>(bw, s);
+    R.Zt = h.Z.p;
+
+    if (fp32) {
+      // ---- FP64 refinement: lambda_k = Re(z_k^H D z_k) / |z_k|^2 with D in FP64
+      h.Z64.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
+      launch_convert<float, double>(h.Z.as<cplx<float>>(), h.Z64.as<cplx<double>>(), B * n * mcap, s);
+      h.num.ensure((size_t)B * mcap * sizeof(double));
+      h.den.ensure((size_t)B * mcap * sizeof(double));
+      launch_fill_zero(h.num.p, B * mcap * sizeof(double), s);
+      GemmParams<double> g{};
+      g.opA = 'N'; g.opB = 'N';
+      g.m = n; g.n = mmax; g.k = n;
+      g.alpha = cplx<double>{1, 0};
+      g.A = D64; g.lda = n; g.strideA = n * n;
+      g.B = h.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
+      g.batch = B;
+      g.dot_out = h.num.as<double>(); g.dot_stride = mcap;
+      g.dotX = h.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
+      launch_gemm<double>(g, s);
+      launch_colnorm2<double>(h.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, h.den.as<double>(),
+                              mcap, s);
+      launch_rq_finish(h.num.as<double>(), h.den.as<double>(), te.w, B, mmax, mcap, s);
+      R.Z64 = h.Z64.as<cplx<double>>();
+    } else {
+      R.Z64 = reinterpret_cast<cplx<double>*>(h.Z.p);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- run one batch
+struct BatchOut {
+  EigResult R;
+  double* om = nullptr;        // device B x mcap, cm^-1
+  double* vel = nullptr;       // device B x mcap x 3
+};
+
+static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 row-major */,
+                      BatchOut& out) {
+  const int64_t n = h.n, N = h.N;
+  hipStream_t s = h.stream;
+  const pph_opts& o = h.opts;
+  const bool fp32 = (o.precision == 'S');
+  const bool vel = o.vel != 0;
+  const bool want_vec = vel || o.jobz == 'V';
+
+  h.qdev.ensure((size_t)B * 3 * sizeof(double));
+  PPH_HIP(hipMemcpyAsync(h.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+  h.A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
+  if (fp32) h.A32.ensure((size_t)B * n * n * sizeof(cplx<float>));
+  if (vel) h.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
+
+  AsmParams ap{};
+  ap.phi = h.phi.as<double>(); ap.crys = h.crys.as<double>(); ap.lat = h.lat.as<double>();
+  ap.rmass = h.rmass.as<double>(); ap.mask = h.mask.as<uint16_t>(); ap.q = h.qdev.as<double>();
+  ap.cd_dc = h.cd_dc.as<double>(); ap.cd_norm = h.cd_norm;
+  ap.N = N; ap.nb = B; ap.mode = (o.vel_method == 'C') ? 1 : 0;
+  ap.D64 = h.A64.as<cplx<double>>();
+  ap.D32 = fp32 ? h.A32.as<cplx<float>>() : nullptr;
+  if (vel) {
+    if (fp32) ap.dD32 = h.dD.as<cplx<float>>();
+    else ap.dD64 = h.dD.as<cplx<double>>();
+  }
+  launch_assemble(ap, s);
+
+  const double K32 = (double)(float)(15.633302f * 33.35641f);
+  const double vl_l = o.vl / K32, vu_l = o.vu / K32;
+  EigResult& R = out.R;
+  if (fp32) eig_batch<float>(h, B, h.A32.as<cplx<float>>(), h.A64.as<cplx<double>>(), want_vec, vl_l, vu_l, R);
+  else eig_batch<double>(h, B, h.A64.as<cplx<double>>(), nullptr, want_vec, vl_l, vu_l, R);
+
+  const int64_t mcap = R.mcap;
+  int64_t mmax = 0;
+  for (auto m : R.M) mmax = std::max(mmax, m);
+  h.om.ensure((size_t)B * mcap * sizeof(double));
+  launch_eval_to_cm(R.lam, h.om.as<double>(), B, mmax, mcap, s);
+  out.om = h.om.as<double>();
+
+  if (vel && mmax > 0) {
+    h.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
+    launch_fill_zero(h.vacc.p, B * mcap * 3 * sizeof(double), s);
+    if (fp32) {
+      GemmParams<float> g{};
+      g.opA = 'N'; g.opB = 'N';
+      g.m = n; g.n = mmax; g.k = n;
+      g.alpha = cplx<float>{1, 0};
+      g.A = h.dD.as<cplx<float>>(); g.lda = n; g.strideA = 3 * n * n; g.sAin = n * n;
+      g.B = h.Z.as<cplx<float>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
+      g.batch = B; g.binner = 3;
+      g.dot_out = h.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
+      g.dotX = h.Z.as<cplx<float>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
+      launch_gemm<float>(g, s);
+    } else {
+      GemmParams<double> g{};
+      g.opA = 'N'; g.opB = 'N';
+      g.m = n; g.n = mmax; g.k = n;
+      g.alpha = cplx<double>{1, 0};
+      g.A = h.dD.as<cplx<double>>(); g.lda = n; g.strideA = 3 * n * n; g.sAin = n * n;
+      g.B = h.Z.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
+      g.batch = B; g.binner = 3;
+      g.dot_out = h.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
+      g.dotX = h.Z.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
+      launch_gemm<double>(g, s);
+    }
+    launch_vel_scale(h.vacc.as<double>(), h.om.as<double>(), B, mmax, mcap, mcap, s);
+    out.vel = h.vacc.as<double>() + B * 3 * mcap;
+  }
+}
+
+static int64_t auto_batch(const Handle& h, int64_t nq) {
+  if (h.opts.batch > 0) return std::min(nq, h.opts.batch);
+  const int64_t n = h.n;
+  const double per = (double)n * n * (16.0 + (h.opts.precision == 'S' ? 8.0 : 0.0) +
+                                      (h.opts.vel ? 48.0 : 0.0) + 16.0 + 80.0 + 24.0);
+  int64_t b = (int64_t)std::floor(24e9 / std::max(per, 1.0));
+  b = std::max<int64_t>(1, std::min<int64_t>(b, 64));
+  return std::min(nq, b);
+}
+
+}  // namespace pph
+
+using namespace pph;
+
+static int fail(Handle* h, const std::string& msg, int code) {
+  if (h) h->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+#define PPH_API_TRY try {
+#define PPH_API_CATCH(h)                                                     \
+  }                                                                          \
+  catch (const std::bad_alloc&) {                                            \
+    return fail(h, "out of device or host memory", PPH_ERR_MEMORY);          \
+  }                                                                          \
+  catch (const pph::hip_error& e) {                                          \
+    return fail(h, e.what(), PPH_ERR_HIP);                                   \
+  }                                                                          \
+  catch (const std::exception& e) {                                          \
+    return fail(h, e.what(), PPH_ERR_INTERNAL);                              \
+  }
+
+static int check_opts(const pph_opts* o, std::string& why) {
+  if (!o) { why = "opts is NULL"; return -1; }
+  if (o->jobz != 'N' && o->jobz != 'V') { why = "jobz must be 'N' or 'V'"; return -1; }
+  if (o->range != 'A' && o->range != 'V' && o->range != 'I') { why = "range must be 'A', 'V' or 'I'"; return -1; }
+  if (o->vel_method != 'A' && o->vel_method != 'C') { why = "vel_method must be 'A' or 'C'"; return -1; }
+  if (o->precision != 'D' && o->precision != 'S') { why = "precision must be 'D' or 'S'"; return -1; }
+  if (o->range == 'I' && o->il > o->iu) { why = "range I needs il <= iu"; return -1; }
+  if (o->range == 'V' && o->vl > o->vu) { why = "range V needs vl <= vu"; return -1; }
+  if (o->range == 'I' && o->il < 1) { why = "range I needs il >= 1"; return -1; }
+  return 0;
+}
+
+extern "C" {
+
+const char* pph_version(void) { return "parphom_hip 0.1 (gfx950)"; }
+
+const char* pph_last_error(const pph_handle* h) {
+  if (h) return reinterpret_cast<const Handle*>(h)->err.c_str();
+  return g_last_error.c_str();
+}
+
+int pph_create(const pph_system* sys, const double* fc, const pph_opts* opts, pph_handle** out) {
+  if (!out) return fail(nullptr, "pph_create: out is NULL", PPH_ERR_ARG);
+  *out = nullptr;
+  if (!sys || !fc) return fail(nullptr, "pph_create: sys or fc is NULL", PPH_ERR_ARG);
+  if (sys->natom < 1 || sys->ntypes < 1 || !sys->lat || !sys->pos || !sys->mass || !sys->type)
+    return fail(nullptr, "pph_create: incomplete system description", PPH_ERR_ARG);
+  std::string why;
+  if (check_opts(opts, why)) return fail(nullptr, "pph_create: " + why, PPH_ERR_ARG);
+  for (int64_t i = 0; i < sys->natom; ++i)
+    if (sys->type[i] < 1 || sys->type[i] > sys->ntypes)
+      return fail(nullptr, "pph_create: atom type out of range", PPH_ERR_ARG);
+  std::unique_ptr<Handle> h(new Handle());
+  PPH_API_TRY
+  h->opts = *opts;
+  if (h->opts.vel) h->opts.jobz = 'V';
+  h->device = opts->device;
+  PPH_HIP(hipSetDevice(h->device));
+  PPH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  const int64_t N = sys->natom, n = 3 * N;
+  h->N = N; h->n = n;
+  Geometry& g = h->g;
+  g.N = N;
+  std::memcpy(g.lat, sys->lat, sizeof(g.lat));
+  g.pos.assign(sys->pos, sys->pos + 3 * N);
+  g.crys.resize(3 * N);
+  for (int64_t i = 0; i < N; ++i) {
+    const double x[3] = {g.pos[i], g.pos[i + N], g.pos[i + 2 * N]};
+    double u[3];
+    linsolve(x, g.lat, u);
+    g.crys[i] = u[0]; g.crys[i + N] = u[1]; g.crys[i + 2 * N] = u[2];
+  }
+  // rec = 2 pi inv(lat)^T (matinvtransp3)
+  {
+    double Binv[9];
+    inv3(g.lat, Binv, 6.283185307179586);
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) g.rec[i + 3 * k] = Binv[k + 3 * i];
+  }
+  g.rmass.resize(N);
+  for (int64_t i = 0; i < N; ++i) g.rmass[i] = sys->mass[sys->type[i] - 1];
+  // central-difference steps (dynamical_matrix.F90:136-142)
+  const double dinc = (double)1e-6f;
+  const double sdel = std::max(dinc * std::sqrt(std::fabs(L(g.rec, 0, 0) * L(g.rec, 1, 1) -
+                                                          L(g.rec, 0, 1) * L(g.rec, 1, 0))), dinc);
+  double dc[9];
+  for (int a = 0; a < 3; ++a) {
+    double delta[3] = {0, 0, 0};
+    delta[a] = sdel;
+    double u[3];
+    linsolve(delta, g.rec, u);
+    for (int c = 0; c < 3; ++c) dc[3 * a + c] = u[c];
+  }
+  h->cd_norm = sdel;
+
+  hipStream_t s = h->stream;
+  h->phi.ensure((size_t)n * n * sizeof(double));
+  {
+    DevBuf fcd;
+    fcd.ensure((size_t)N * N * 9 * sizeof(double));
+    PPH_HIP(hipMemcpyAsync(fcd.p, fc, (size_t)N * N * 9 * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_symmetrise_fc(fcd.as<double>(), h->phi.as<double>(), N, s);
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  h->crys.ensure(3 * N * sizeof(double));
+  h->pos.ensure(3 * N * sizeof(double));
+  h->lat.ensure(9 * sizeof(double));
+  h->rmass.ensure(N * sizeof(double));
+  h->cd_dc.ensure(9 * sizeof(double));
+  h->mask.ensure((size_t)N * N * sizeof(uint16_t));
+  PPH_HIP(hipMemcpyAsync(h->crys.p, g.crys.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(h->pos.p, g.pos.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(h->lat.p, g.lat, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(h->rmass.p, g.rmass.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(h->cd_dc.p, dc, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_image_table(h->crys.as<double>(), h->pos.as<double>(), h->lat.as<double>(),
+                     h->mask.as<uint16_t>(), N, s);
+  PPH_HIP(hipStreamSynchronize(s));
+  *out = reinterpret_cast<pph_handle*>(h.release());
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
+static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, pph_out* out,
+                      bool device_out) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_run: handle is NULL", PPH_ERR_ARG);
+  if (nq < 0 || (nq > 0 && (!q || ldq < nq))) return fail(h, "pph_run: bad q array", PPH_ERR_ARG);
+  if (!out || !out->eval_cm || !out->m || !out->info)
+    return fail(h, "pph_run: eval_cm, m and info are required", PPH_ERR_ARG);
+  const int64_t n = h->n;
+  const int64_t mcap = range_mcap(h->opts, n);
+  if (out->ld_eval < mcap) return fail(h, "pph_run: ld_eval smaller than the eigenvalue count", PPH_ERR_ARG);
+  if ((out->evec || out->vel) && out->evec_cols <= 0)
+    return fail(h, "pph_run: evec_cols must be > 0 with evec or vel", PPH_ERR_ARG);
+  if (out->vel && !h->opts.vel) return fail(h, "pph_run: vel requested but the handle has vel = 0", PPH_ERR_ARG);
+  if (out->evec && h->opts.jobz != 'V')
+    return fail(h, "pph_run: evec requested but jobz = 'N'", PPH_ERR_ARG);
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(h->device));
+  hipStream_t s = h->stream;
+  const hipMemcpyKind kind = device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  const int64_t bmax = auto_batch(*h, std::max<int64_t>(nq, 1));
+  std::vector<double> qrows;
+  for (int64_t q0 = 0; q0 < nq; q0 += bmax) {
+    const int64_t B = std::min(bmax, nq - q0);
+    qrows.resize(3 * B);
+    for (int64_t b = 0; b < B; ++b)
+      for (int c = 0; c < 3; ++c) qrows[3 * b + c] = q[(q0 + b) + ldq * c];
+    BatchOut bo;
+    run_batch(*h, B, qrows.data(), bo);
+    const EigResult& R = bo.R;
+    for (int64_t b = 0; b < B; ++b) {
+      const int64_t iq = q0 + b;
+      const int64_t m = R.M[b];
+      int32_t info = R.info[b];
+      int64_t nz = 0;
+      if (h->opts.jobz == 'V') {
+        nz = m;
+        if (out->evec || out->vel) {
+          if (nz > out->evec_cols) { info |= PPH_INFO_MEMORY; nz = out->evec_cols; }
+        }
+      }
+      if (m > 0)
+        PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + b * R.mcap,
+                               m * sizeof(double), kind, s));
+      if (out->evec && nz > 0)
+        PPH_HIP(hipMemcpyAsync((char*)out->evec + (size_t)iq * n * out->evec_cols * 16,
+                               R.Z64 + (size_t)b * n * R.mcap, (size_t)nz * n * 16, kind, s));
+      if (out->vel && nz > 0)
+        PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + b * R.mcap * 3,
+                               (size_t)nz * 3 * sizeof(double), kind, s));
+      if (device_out) {
+        PPH_HIP(hipMemcpyAsync(out->m + iq, &R.M[b], sizeof(int64_t), hipMemcpyHostToDevice, s));
+        PPH_HIP(hipMemcpyAsync(out->info + iq, &info, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        if (out->nz) PPH_HIP(hipMemcpyAsync(out->nz + iq, &nz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        PPH_HIP(hipStreamSynchronize(s));
+      } else {
+        out->m[iq] = m;
+        out->info[iq] = info;
+        if (out->nz) out->nz[iq] = nz;
+      }
+    }
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  return PPH_OK;
+  PPH_API_CATCH(h)
+}
+
+int pph_run(pph_handle* h, int64_t nq, const double* q, int64_t ldq, pph_out* out) {
+  return run_common(h, nq, q, ldq, out, false);
+}
+
+int pph_run_device(pph_handle* h, int64_t nq, const double* q, int64_t ldq, pph_out* out) {
+  return run_common(h, nq, q, ldq, out, true);
+}
+
+int pph_assemble(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t what, void* out,
+                 int32_t on_device) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_assemble: handle is NULL", PPH_ERR_ARG);
+  if (what < 0 || what > 3) return fail(h, "pph_assemble: what must be 0..3", PPH_ERR_ARG);
+  if (nq < 0 || (nq > 0 && (!q || !out || ldq < nq))) return fail(h, "pph_assemble: bad arguments", PPH_ERR_ARG);
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(h->device));
+  hipStream_t s = h->stream;
+  const int64_t n = h->n;
+  const int64_t bmax = std::max<int64_t>(1, std::min<int64_t>(nq, 8));
+  std::vector<double> qrows;
+  for (int64_t q0 = 0; q0 < nq; q0 += bmax) {
+    const int64_t B = std::min(bmax, nq - q0);
+    qrows.resize(3 * B);
+    for (int64_t b = 0; b < B; ++b)
+      for (int c = 0; c < 3; ++c) qrows[3 * b + c] = q[(q0 + b) + ldq * c];
+    h->qdev.ensure((size_t)B * 3 * sizeof(double));
+    PPH_HIP(hipMemcpyAsync(h->qdev.p, qrows.data(), B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    AsmParams ap{};
+    ap.phi = h->phi.as<double>(); ap.crys = h->crys.as<double>(); ap.lat = h->lat.as<double>();
+    ap.rmass = h->rmass.as<double>(); ap.mask = h->mask.as<uint16_t>(); ap.q = h->qdev.as<double>();
+    ap.cd_dc = h->cd_dc.as<double>(); ap.cd_norm = h->cd_norm;
+    ap.N = h->N; ap.nb = B; ap.mode = (h->opts.vel_method == 'C') ? 1 : 0;
+    cplx<double>* dst;
+    if (what == 0) {
+      h->A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
+      ap.D64 = h->A64.as<cplx<double>>();
+      dst = ap.D64;
+    } else {
+      h->dD.ensure((size_t)B * 3 * n * n * sizeof(cplx<double>));
+      h->A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
+      ap.D64 = h->A64.as<cplx<double>>();
+      ap.dD64 = h->dD.as<cplx<double>>();
+      dst = ap.dD64;
+    }
+    launch_assemble(ap, s);
+    for (int64_t b = 0; b < B; ++b) {
+      const cplx<double>* src = (what == 0) ? dst + (size_t)b * n * n
+                                            : dst + (size_t)(3 * b + (what - 1)) * n * n;
+      PPH_HIP(hipMemcpyAsync((char*)out + (size_t)(q0 + b) * n * n * 16, src, (size_t)n * n * 16,
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  return PPH_OK;
+  PPH_API_CATCH(h)
+}
+
+int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w, void* Z,
+              int64_t* m, int32_t* info, int32_t on_device) {
+  std::string why;
+  if (check_opts(opts, why)) return fail(nullptr, "pph_heevx: " + why, PPH_ERR_ARG);
+  if (n < 1 || nmat < 0 || (nmat > 0 && (!A || !w || !m || !info)))
+    return fail(nullptr, "pph_heevx: bad arguments", PPH_ERR_ARG);
+  if (Z && opts->jobz != 'V') return fail(nullptr, "pph_heevx: Z given with jobz = 'N'", PPH_ERR_ARG);
+  Handle h;
+  PPH_API_TRY
+  h.opts = *opts;
+  h.device = opts->device;
+  PPH_HIP(hipSetDevice(h.device));
+  PPH_HIP(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  h.n = n; h.N = 0;
+  hipStream_t s = h.stream;
+  const bool fp32 = opts->precision == 'S';
+  const hipMemcpyKind in_kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind out_kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  const int64_t bmax = std::max<int64_t>(1, std::min<int64_t>(nmat, opts->batch > 0 ? opts->batch : 8));
+  for (int64_t m0 = 0; m0 < nmat; m0 += bmax) {
+    const int64_t B = std::min(bmax, nmat - m0);
+    h.A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
+    PPH_HIP(hipMemcpyAsync(h.A64.p, (char*)A + (size_t)m0 * n * n * 16, (size_t)B * n * n * 16, in_kind, s));
+    EigResult R;
+    if (fp32) {
+      h.A32.ensure((size_t)B * n * n * sizeof(cplx<float>));
+      launch_convert<double, float>(h.A64.as<cplx<double>>(), h.A32.as<cplx<float>>(), B * n * n, s);
+      eig_batch<float>(h, B, h.A32.as<cplx<float>>(), h.A64.as<cplx<double>>(), Z != nullptr,
+                       opts->vl, opts->vu, R);
+    } else {
+      eig_batch<double>(h, B, h.A64.as<cplx<double>>(), nullptr, Z != nullptr, opts->vl, opts->vu, R);
+    }
+    for (int64_t b = 0; b < B; ++b) {
+      const int64_t mm = R.M[b];
+      if (mm > 0) {
+        PPH_HIP(hipMemcpyAsync(w + (m0 + b) * n, R.lam + b * R.mcap, mm * sizeof(double), out_kind, s));
+        if (Z && R.Z64)
+          PPH_HIP(hipMemcpyAsync((char*)Z + (size_t)(m0 + b) * n * n * 16, R.Z64 + (size_t)b * n * R.mcap,
+                                 (size_t)mm * n * 16, out_kind, s));
+      }
+      int32_t inf = R.info[b];
+      if (on_device) {
+        PPH_HIP(hipMemcpyAsync(m + m0 + b, &R.M[b], sizeof(int64_t), hipMemcpyHostToDevice, s));
+        PPH_HIP(hipMemcpyAsync(info + m0 + b, &inf, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        PPH_HIP(hipStreamSynchronize(s));
+      } else {
+        m[m0 + b] = mm;
+        info[m0 + b] = inf;
+      }
+    }
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
+int pph_destroy(pph_handle* hh) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return PPH_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+  return PPH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- test hook
+extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t k,
+                               const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                               int64_t ldc, int32_t mode, double* dot) {
+  PPH_API_TRY
+  const size_t es = (prec == 'S') ? 8 : 16;
+  const int64_t ar = (opA == 'N') ? m : k, ac = (opA == 'N') ? k : m;
+  const int64_t br = (opB == 'N') ? k : n, bc = (opB == 'N') ? n : k;
+  (void)ar; (void)br;
+  hipStream_t s;
+  PPH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  DevBuf dA, dB, dC, dX;
+  dA.ensure(es * lda * ac); dB.ensure(es * ldb * bc); dC.ensure(es * ldc * n); dX.ensure(8 * n + 8);
+  PPH_HIP(hipMemcpyAsync(dA.p, A, es * lda * ac, hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(dB.p, B, es * ldb * bc, hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(dC.p, C, es * ldc * n, hipMemcpyHostToDevice, s));
+  launch_fill_zero(dX.p, 8 * n, s);
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    GemmParams<T> g{};
+    g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
+    g.alpha = cplx<T>{1, 0}; g.beta = cplx<T>{0, 0};
+    g.A = dA.as<cplx<T>>(); g.lda = lda; g.B = dB.as<cplx<T>>(); g.ldb = ldb;
+    g.C = dC.as<cplx<T>>(); g.ldc = ldc; g.batch = 1;
+    if (mode == 1) {
+      g.dot_out = dX.as<double>(); g.dotX = dC.as<cplx<T>>(); g.lddx = ldc;
+    } else if (mode == 2) {
+      launch_fill_zero(dC.p, es * ldc * n, s);
+      g.ksplit = 4;
+    } else if (mode == 3) {
+      // C = A(:, 0:k/2) B(0:k/2, :) + A(:, k/2:) B(k/2:, :) through the two-product path (opA = opB = 'N')
+      g.k1 = k / 2; g.A2 = dA.as<cplx<T>>() + (k / 2) * lda; g.B2 = dB.as<cplx<T>>() + k / 2;
+    }
+    launch_gemm<T>(g, s);
+  };
+  if (prec == 'S') run(float{}); else run(double{});
+  PPH_HIP(hipMemcpyAsync(C, dC.p, es * ldc * n, hipMemcpyDeviceToHost, s));
+  if (dot) PPH_HIP(hipMemcpyAsync(dot, dX.p, 8 * n, hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
+  PPH_HIP(hipStreamDestroy(s));
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}

@@ -113,5 +113,6 @@ struct hip_error : std::runtime_error {
 #define PPH_CHECK_LAUNCH() PPH_HIP(hipGetLastError())
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__device__ __forceinline__ int64_t ceil_div_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace pph

@@ -39,11 +39,28 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
   const int tid = threadIdx.x;
   const int lane = tid % WAVE, wave = tid / WAVE;
   const int wr0 = (wave / 2) * 32, wc0 = (wave % 2) * 32;
-  const int64_t b = blockIdx.z;
+  const int64_t ksplit = p.ksplit > 1 ? p.ksplit : 1;
+  const int64_t binner = p.binner > 1 ? p.binner : 1;
+  int64_t z = blockIdx.z;
+  const int64_t ks_idx = z % ksplit;
+  z /= ksplit;
+  const int64_t bi = z % binner, b = z / binner;
   const int64_t row0 = (int64_t)blockIdx.x * GBM, col0 = (int64_t)blockIdx.y * GBN;
-  const cplx<T>* A = p.A + b * p.strideA;
-  const cplx<T>* B = p.B + b * p.strideB;
-  const int64_t M = p.m, N = p.n, K = p.k;
+  const int64_t offA = b * p.strideA + bi * p.sAin, offB = b * p.strideB + bi * p.sBin;
+  const cplx<T>* A1 = p.A + offA;
+  const cplx<T>* B1 = p.B + offB;
+  const bool two = (p.A2 != nullptr);
+  const cplx<T>* A2 = two ? p.A2 + offA : A1;
+  const cplx<T>* B2 = two ? p.B2 + offB : B1;
+  const int64_t M = p.m, N = p.n;
+  // K range of this split
+  int64_t kbeg = 0, K = p.k;
+  if (ksplit > 1) {
+    const int64_t kc = ceil_div_dev(ceil_div_dev(p.k, ksplit), GBK) * GBK;
+    kbeg = ks_idx * kc;
+    K = min(p.k, kbeg + kc);
+  }
+  const int64_t K1 = two ? p.k1 : K;
 
   acc_t accR[2][2], accI[2][2];
 #pragma unroll
@@ -55,18 +72,23 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
     }
 
   cplx<T> ra[4], rb[4];
-  auto load_tiles = [&](int64_t k0) {
+  auto load_tiles = [&](int64_t kg) {
+    const bool second = two && (kg >= K1);
+    const cplx<T>* A = second ? A2 : A1;
+    const cplx<T>* B = second ? B2 : B1;
+    const int64_t k0 = second ? kg - K1 : kg;
+    const int64_t Kseg = second ? K - K1 : K1;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       cplx<T> v{0, 0};
       if (!CONJA) {
         const int r = tid % 64, kk = tid / 64 + 4 * i;
         const int64_t gr = row0 + r, gk = k0 + kk;
-        if (gr < M && gk < K) v = A[gr + gk * p.lda];
+        if (gr < M && gk < Kseg) v = A[gr + gk * p.lda];
       } else {
         const int kk = tid % 16, r = tid / 16 + 16 * i;
         const int64_t gr = row0 + r, gk = k0 + kk;
-        if (gr < M && gk < K) v = cconj(A[gk + gr * p.lda]);
+        if (gr < M && gk < Kseg) v = cconj(A[gk + gr * p.lda]);
       }
       ra[i] = v;
     }
@@ -76,11 +98,11 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
       if (!CONJB) {
         const int kk = tid % 16, c = tid / 16 + 16 * i;
         const int64_t gk = k0 + kk, gc = col0 + c;
-        if (gk < K && gc < N) v = B[gk + gc * p.ldb];
+        if (gk < Kseg && gc < N) v = B[gk + gc * p.ldb];
       } else {
         const int c = tid % 64, kk = tid / 64 + 4 * i;
         const int64_t gk = k0 + kk, gc = col0 + c;
-        if (gk < K && gc < N) v = cconj(B[gc + gk * p.ldb]);
+        if (gk < Kseg && gc < N) v = cconj(B[gc + gk * p.ldb]);
       }
       rb[i] = v;
     }
@@ -95,8 +117,8 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
     }
   };
 
-  load_tiles(0);
-  for (int64_t k0 = 0; k0 < K; k0 += GBK) {
+  load_tiles(kbeg);
+  for (int64_t k0 = kbeg; k0 < K; k0 += GBK) {
     __syncthreads();
     store_tiles();
     __syncthreads();
@@ -122,7 +144,7 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
   }
 
   if (!DOT) {
-    cplx<T>* C = p.C + b * p.strideC;
+    cplx<T>* C = p.C + b * p.strideC + bi * p.sCin;
     const bool beta0 = (p.beta.x == T(0) && p.beta.y == T(0));
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
@@ -137,12 +159,18 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
           cplx<T> v{accR[mi][ni][i], accI[mi][ni][i]};
           v = cmul(p.alpha, v);
           cplx<T>* c = C + row + col * p.ldc;
-          if (!beta0) v = cadd(v, cmul(p.beta, *c));
-          *c = v;
+          if (ksplit > 1) {
+            atomicAdd(&c->x, v.x);
+            atomicAdd(&c->y, v.y);
+          } else {
+            if (!beta0) v = cadd(v, cmul(p.beta, *c));
+            *c = v;
+          }
         }
       }
   } else {
-    const cplx<T>* X = p.dotX + b * p.strideX;
+    const cplx<T>* X = p.dotX + b * p.strideX + bi * p.sXin;
+    double* dout = p.dot_out + b * p.dot_stride + bi * p.sDotin;
 #pragma unroll
     for (int ni = 0; ni < 2; ++ni) {
       const int64_t col = col0 + wc0 + ni * 16 + lane % 16;
@@ -161,7 +189,7 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
       }
       s += __shfl_xor(s, 16, WAVE);
       s += __shfl_xor(s, 32, WAVE);
-      if (lane < 16 && col < N) atomicAdd(p.dot_out + b * p.dot_stride + col, s);
+      if (lane < 16 && col < N) atomicAdd(dout + col, s);
     }
   }
 }
@@ -169,7 +197,10 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
 template <typename T>
 void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0 || p.batch <= 0) return;
-  dim3 grid((unsigned)ceil_div(p.m, GBM), (unsigned)ceil_div(p.n, GBN), (unsigned)p.batch);
+  const int64_t ks = p.ksplit > 1 ? p.ksplit : 1, bin = p.binner > 1 ? p.binner : 1;
+  if (ks > 1 && p.A2 != nullptr) throw std::runtime_error("launch_gemm: split-K with two products");
+  dim3 grid((unsigned)ceil_div(p.m, GBM), (unsigned)ceil_div(p.n, GBN),
+            (unsigned)(p.batch * bin * ks));
   const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);
 #define PPH_GEMM_CASE(CA, CB, DT)                                                  \
   if (ca == CA && cb == CB && dot == DT) {                                         \

@@ -36,8 +36,17 @@ struct GemmParams {
   cplx<T> alpha, beta;
   const cplx<T>* A; int64_t lda, strideA;
   const cplx<T>* B; int64_t ldb, strideB;
+  // optional second product: K range [k1, k) uses (A2, B2) with the same
+  // leading dimensions, strides and ops (k1 must be a multiple of 16)
+  const cplx<T>* A2; const cplx<T>* B2; int64_t k1;
   cplx<T>* C; int64_t ldc, strideC;
   int64_t batch;
+  // inner batch: blockIdx.z = (outer * binner + inner) * ksplit + split; operand
+  // offsets add inner * sXin to the outer strides (binner 0 or 1 = no inner batch)
+  int64_t binner; int64_t sAin, sBin, sCin, sXin, sDotin;
+  // split-K: the K range is cut into ksplit pieces whose products are added
+  // into C atomically (requires beta = 0 and C cleared beforehand)
+  int64_t ksplit;
   // diagonal-dot epilogue: when dot_out != null, C is not written; instead
   // dot_out[b*dot_stride + col] += sum_rows Re(conj(X[row, col]) * (op(A) op(B))[row, col])
   // with X = dotX (leading dimension lddx, stride strideX).

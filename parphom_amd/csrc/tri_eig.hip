@@ -1,0 +1,314 @@
+// Real symmetric tridiagonal eigensolver in FP64, batched over matrices:
+// Sturm-count bisection for the selected eigenvalues (RANGE A / I / V with
+// abstol, as pzheevx's PDSTEBZ stage does) and inverse iteration for the
+// eigenvectors (the PZSTEIN stage), with Gram-Schmidt inside clusters of
+// close eigenvalues.  Eigenvectors are written interleaved, element i of
+// vector k at zt[i * ldzt + k], so neighbouring lanes touch neighbouring
+// addresses; that array is the row-major n x M matrix Z_T.
+#include "common.hpp"
+#include "kernels.hpp"
+#include <cfloat>
+
+namespace pph {
+
+constexpr double ULP = DBL_EPSILON;          // 2^-52
+constexpr double SAFMIN = DBL_MIN;
+
+__device__ __forceinline__ int64_t sturm_count(const double* __restrict__ d,
+                                               const double* __restrict__ e2, int64_t n,
+                                               double x, double pivmin) {
+  // number of eigenvalues < x (pivots of T - xI below zero)
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int64_t c = (q < 0.0);
+  for (int64_t i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += (q < 0.0);
+  }
+  return c;
+}
+
+// one workgroup per matrix
+__global__ void __launch_bounds__(256) k_tri_prepare(TriEigWork w) {
+  __shared__ double sh[8];
+  const int64_t b = blockIdx.x;
+  const int64_t n = w.n;
+  double* d = w.dd + b * w.strideT;
+  double* e = w.ee + b * w.strideT;
+  double* e2 = w.e2 + b * w.strideT;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double di, ei;
+    if (w.de_is_float) {
+      di = (double)((const float*)w.d)[b * w.strideDE + i];
+      ei = (i < n - 1) ? (double)((const float*)w.e)[b * w.strideDE + i] : 0.0;
+    } else {
+      di = ((const double*)w.d)[b * w.strideDE + i];
+      ei = (i < n - 1) ? ((const double*)w.e)[b * w.strideDE + i] : 0.0;
+    }
+    d[i] = di;
+    e[i] = ei;
+  }
+  __syncthreads();
+  double emax = 0.0, gl = 1e300, gu = -1e300;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double ei = e[i];
+    const double em = (i > 0) ? e[i - 1] : 0.0;
+    const double rad = fabs(em) + fabs(ei);
+    gl = fmin(gl, d[i] - rad);
+    gu = fmax(gu, d[i] + rad);
+    emax = fmax(emax, ei * ei);
+  }
+  gl = -block_max(-gl, sh);
+  gu = block_max(gu, sh);
+  emax = block_max(emax, sh);
+  const double pivmin = SAFMIN * fmax(1.0, emax);
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  // splitting: negligible off-diagonals become exact zeros (PDSTEBZ criterion)
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double ei = e[i];
+    if (i < n - 1) {
+      if (fabs(d[i] * d[i + 1]) * ULP * ULP + SAFMIN > ei * ei) ei = 0.0;
+    } else {
+      ei = 0.0;
+    }
+    e[i] = ei;
+    e2[i] = ei * ei;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double fudge = 2.1;
+    gl = gl - fudge * tnorm * ULP * (double)n - fudge * 2.0 * pivmin;
+    gu = gu + fudge * tnorm * ULP * (double)n + fudge * pivmin;
+    int64_t lo = 0, hi = n;
+    if (w.range == 'I') {
+      lo = max((int64_t)0, min(n, w.il - 1));
+      hi = max(lo, min(n, w.iu));
+    } else if (w.range == 'V') {
+      lo = (w.vl <= gl) ? 0 : (w.vl >= gu ? n : sturm_count(d, e2, n, w.vl, pivmin));
+      hi = (w.vu <= gl) ? 0 : (w.vu >= gu ? n : sturm_count(d, e2, n, w.vu, pivmin));
+      // (vl, vu]: eigenvalues equal to vu are counted by the strict count up to ties
+      hi = max(lo, hi);
+    }
+    w.ilo[b] = lo;
+    w.ihi[b] = hi;
+    double* ti = w.tinfo + b * 4;
+    ti[0] = gl; ti[1] = gu; ti[2] = tnorm; ti[3] = pivmin;
+    w.info[b] = 0;
+  }
+}
+
+// one thread per (matrix, selected eigenvalue)
+__global__ void __launch_bounds__(256) k_tri_bisect(TriEigWork w) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = blockIdx.y;
+  const int64_t lo_i = w.ilo[b], hi_i = w.ihi[b];
+  if (k >= hi_i - lo_i) return;
+  const int64_t kk = lo_i + k;
+  const double* d = w.dd + b * w.strideT;
+  const double* e2 = w.e2 + b * w.strideT;
+  const double* ti = w.tinfo + b * 4;
+  const double pivmin = ti[3];
+  double lo = ti[0], hi = ti[1];
+  const double atol = fmax(w.abstol, pivmin);
+  for (int it = 0; it < 200; ++it) {
+    const double tol = fmax(atol, 2.0 * ULP * fmax(fabs(lo), fabs(hi)));
+    if (hi - lo <= tol) break;
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(d, e2, w.n, mid, pivmin) > kk) hi = mid;
+    else lo = mid;
+  }
+  w.w[b * w.ldw + k] = 0.5 * (lo + hi);
+}
+
+__device__ __forceinline__ double rand_pm1(uint64_t s) {
+  // splitmix64 -> uniform in (-1, 1)
+  s += 0x9E3779B97F4A7C15ull;
+  s = (s ^ (s >> 30)) * 0xBF58476D1CE4E5B9ull;
+  s = (s ^ (s >> 27)) * 0x94D049BB133111EBull;
+  s ^= s >> 31;
+  return 2.0 * ((double)(s >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+
+// LU with partial pivoting of T - sigma I (the DLAGTF scheme); factors stored
+// interleaved: u0, u1, u2, l, piv at fac[(f * n + i) * ld + k].
+__device__ void tri_factor(const double* __restrict__ d, const double* __restrict__ e, int64_t n,
+                           double sigma, double tol, double* __restrict__ fac, int64_t ld,
+                           int64_t k) {
+  double alpha = d[0] - sigma, beta = (n > 1) ? e[0] : 0.0;
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const double c = e[i];                  // subdiagonal (row i+1, col i)
+    const double dn = d[i + 1] - sigma;
+    const double en = (i + 2 < n) ? e[i + 1] : 0.0;
+    double u0, u1, u2, l, piv;
+    if (fabs(alpha) >= fabs(c)) {
+      u0 = alpha; u1 = beta; u2 = 0.0; piv = 0.0;
+      if (fabs(u0) < tol) u0 = (u0 >= 0.0) ? tol : -tol;
+      l = c / u0;
+      alpha = dn - l * u1;
+      beta = en;
+    } else {
+      u0 = c; u1 = dn; u2 = en; piv = 1.0;
+      l = alpha / c;
+      alpha = beta - l * dn;
+      beta = -l * en;
+    }
+    fac[(0 * n + i) * ld + k] = u0;
+    fac[(1 * n + i) * ld + k] = u1;
+    fac[(2 * n + i) * ld + k] = u2;
+    fac[(3 * n + i) * ld + k] = l;
+    fac[(4 * n + i) * ld + k] = piv;
+  }
+  if (fabs(alpha) < tol) alpha = (alpha >= 0.0) ? tol : -tol;
+  fac[(0 * n + (n - 1)) * ld + k] = alpha;
+  fac[(1 * n + (n - 1)) * ld + k] = 0.0;
+  fac[(2 * n + (n - 1)) * ld + k] = 0.0;
+}
+
+// y <- (T - sigma I)^{-1} y in place (y interleaved at y[i * ld + k]); returns max|y|
+__device__ double tri_solve(const double* __restrict__ fac, int64_t n, int64_t ld, int64_t k,
+                            double* __restrict__ y) {
+  // forward: apply row interchanges and multipliers
+  double cur = y[k];
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const double nxt = y[(i + 1) * ld + k];
+    const double l = fac[(3 * n + i) * ld + k];
+    const double piv = fac[(4 * n + i) * ld + k];
+    double bi, bn;
+    if (piv == 0.0) { bi = cur; bn = nxt - l * cur; }
+    else { bi = nxt; bn = cur - l * nxt; }
+    y[i * ld + k] = bi;
+    cur = bn;
+  }
+  y[(n - 1) * ld + k] = cur;
+  // backward substitution with the upper factor (bandwidth 3)
+  double y1 = 0.0, y2 = 0.0, mx = 0.0;
+  for (int64_t i = n - 1; i >= 0; --i) {
+    const double u0 = fac[(0 * n + i) * ld + k];
+    const double u1 = fac[(1 * n + i) * ld + k];
+    const double u2 = fac[(2 * n + i) * ld + k];
+    const double v = (y[i * ld + k] - u1 * y1 - u2 * y2) / u0;
+    y[i * ld + k] = v;
+    y2 = y1;
+    y1 = v;
+    mx = fmax(mx, fabs(v));
+  }
+  return mx;
+}
+
+__device__ double perturbed_shift(const double* __restrict__ wv, int64_t k, double tnorm) {
+  // shifts of (nearly) equal eigenvalues are separated by pertol, as DSTEIN does
+  const int64_t s = (k > 64) ? k - 64 : 0;
+  double x = wv[s];
+  for (int64_t i = s + 1; i <= k; ++i) {
+    const double pertol = 10.0 * ULP * fmax(fabs(wv[i]), ULP * tnorm);
+    x = fmax(wv[i], x + pertol);
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(256) k_tri_inviter(TriEigWork w) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = blockIdx.y;
+  const int64_t M = w.ihi[b] - w.ilo[b];
+  if (k >= M) return;
+  const int64_t n = w.n, ld = w.ldzt;
+  const double* d = w.dd + b * w.strideT;
+  const double* e = w.ee + b * w.strideT;
+  const double tnorm = w.tinfo[b * 4 + 2];
+  const double* wv = w.w + b * w.ldw;
+  double* y = w.zt + b * w.strideZt;
+  double* fac = w.fac + b * w.strideFac;
+  const double sigma = perturbed_shift(wv, k, tnorm);
+  const double tol = fmax(ULP * tnorm, SAFMIN);
+  tri_factor(d, e, n, sigma, tol, fac, ld, k);
+  const uint64_t seed = (uint64_t)(w.ilo[b] + k) * 0x100000001B3ull;
+  for (int64_t i = 0; i < n; ++i) y[i * ld + k] = rand_pm1(seed + (uint64_t)i);
+  for (int it = 0; it < 3; ++it) {
+    const double mx = tri_solve(fac, n, ld, k, y);
+    const double sc = (mx > 0.0) ? 1.0 / mx : 1.0;
+    for (int64_t i = 0; i < n; ++i) y[i * ld + k] *= sc;
+  }
+  double nrm = 0.0, big = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double v = y[i * ld + k];
+    nrm += v * v;
+    if (fabs(v) > fabs(big)) big = v;
+  }
+  double sc = 1.0 / sqrt(nrm);
+  if (big < 0.0) sc = -sc;
+  for (int64_t i = 0; i < n; ++i) y[i * ld + k] *= sc;
+}
+
+// one workgroup per cluster of close eigenvalues: redo inverse iteration for
+// each member with the earlier members projected out before every solve.
+__global__ void __launch_bounds__(256) k_tri_cluster(TriEigWork w, double ctol_rel) {
+  __shared__ double sh[8];
+  __shared__ double smx;
+  const int64_t k0 = blockIdx.x;
+  const int64_t b = blockIdx.y;
+  const int64_t M = w.ihi[b] - w.ilo[b];
+  if (k0 >= M) return;
+  const double tnorm = w.tinfo[b * 4 + 2];
+  const double ctol = ctol_rel * tnorm;
+  const double* wv = w.w + b * w.ldw;
+  if (k0 > 0 && wv[k0] - wv[k0 - 1] <= ctol) return;          // not a cluster start
+  if (k0 + 1 >= M || wv[k0 + 1] - wv[k0] > ctol) return;       // singleton
+  int64_t k1 = k0 + 1;
+  while (k1 + 1 < M && wv[k1 + 1] - wv[k1] <= ctol) ++k1;      // members k0..k1
+  const int64_t n = w.n, ld = w.ldzt;
+  double* y = w.zt + b * w.strideZt;
+  const double* fac = w.fac + b * w.strideFac;
+  for (int64_t k = k0; k <= k1; ++k) {
+    const uint64_t seed = (uint64_t)(w.ilo[b] + k) * 0x100000001B3ull + 0x5bd1e995ull;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i * ld + k] = rand_pm1(seed + (uint64_t)i);
+    for (int it = 0; it <= 3; ++it) {
+      // project out earlier cluster members (modified Gram-Schmidt), twice
+      for (int pass = 0; pass < 2; ++pass)
+        for (int64_t kp = k0; kp < k; ++kp) {
+          double dot = 0.0;
+          for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dot += y[i * ld + kp] * y[i * ld + k];
+          dot = block_sum(dot, sh);
+          for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i * ld + k] -= dot * y[i * ld + kp];
+          __syncthreads();
+        }
+      if (it == 3) break;
+      __syncthreads();
+      if (threadIdx.x == 0) smx = tri_solve(fac, n, ld, k, y);
+      __syncthreads();
+      const double sc = (smx > 0.0) ? 1.0 / smx : 1.0;
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i * ld + k] *= sc;
+      __syncthreads();
+    }
+    double nrm = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) nrm += y[i * ld + k] * y[i * ld + k];
+    nrm = block_sum(nrm, sh);
+    const double sc = 1.0 / sqrt(nrm);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i * ld + k] *= sc;
+    __syncthreads();
+  }
+}
+
+void tri_prepare(const TriEigWork& w, hipStream_t s) {
+  hipLaunchKernelGGL(k_tri_prepare, dim3((unsigned)w.batch), dim3(256), 0, s, w);
+  PPH_CHECK_LAUNCH();
+}
+
+void tri_bisect(const TriEigWork& w, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(w.mmax, 64), (unsigned)w.batch);
+  hipLaunchKernelGGL(k_tri_bisect, grid, dim3(64), 0, s, w);
+  PPH_CHECK_LAUNCH();
+}
+
+void tri_vectors(const TriEigWork& w, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(w.mmax, 64), (unsigned)w.batch);
+  hipLaunchKernelGGL(k_tri_inviter, grid, dim3(64), 0, s, w);
+  PPH_CHECK_LAUNCH();
+  const double ctol_rel = fmax(w.orfac, 1e-9);
+  hipLaunchKernelGGL(k_tri_cluster, dim3((unsigned)w.mmax, (unsigned)w.batch), dim3(256), 0, s, w,
+                     ctol_rel);
+  PPH_CHECK_LAUNCH();
+}
+
+}  // namespace pph

@@ -1,0 +1,239 @@
+"""ctypes binding of libparphom_hip.so (the C ABI in include/parphom_hip.h).
+
+The library is built in-tree by `parphom_amd/csrc/Makefile`; importing this
+module loads it and raises if it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparphom_hip.so")
+
+PPH_INFO_NOCONV, PPH_INFO_ORTH, PPH_INFO_MEMORY, PPH_INFO_NOTCOMP = 1, 2, 4, 8
+
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+
+
+class PphSystem(C.Structure):
+    _fields_ = [("natom", C.c_int64), ("ntypes", C.c_int64), ("lat", _dp), ("pos", _dp),
+                ("mass", _dp), ("type", _i64p)]
+
+
+class PphOpts(C.Structure):
+    _fields_ = [("jobz", C.c_char), ("range", C.c_char), ("vel_method", C.c_char),
+                ("precision", C.c_char), ("vel", C.c_int32), ("device", C.c_int32),
+                ("vl", C.c_double), ("vu", C.c_double), ("il", C.c_int64), ("iu", C.c_int64),
+                ("abstol", C.c_double), ("orfac", C.c_double), ("batch", C.c_int64)]
+
+
+class PphOut(C.Structure):
+    _fields_ = [("eval_cm", _dp), ("ld_eval", C.c_int64), ("m", _i64p), ("nz", _i64p),
+                ("evec", C.c_void_p), ("evec_cols", C.c_int64), ("vel", _dp), ("info", _i32p)]
+
+
+EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
+            "pph_destroy", "pph_last_error", "pph_version", "pph_debug_zgemm")
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (once).  Raises OSError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is not built; run `make -C parphom_amd/csrc` "
+                      "(or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    lib.pph_create.argtypes = [C.POINTER(PphSystem), _dp, C.POINTER(PphOpts), C.POINTER(C.c_void_p)]
+    lib.pph_run.argtypes = [C.c_void_p, C.c_int64, _dp, C.c_int64, C.POINTER(PphOut)]
+    lib.pph_run_device.argtypes = [C.c_void_p, C.c_int64, _dp, C.c_int64, C.POINTER(PphOut)]
+    lib.pph_assemble.argtypes = [C.c_void_p, C.c_int64, _dp, C.c_int64, C.c_int32, C.c_void_p,
+                                 C.c_int32]
+    lib.pph_heevx.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.POINTER(PphOpts), _dp,
+                              C.c_void_p, _i64p, _i32p, C.c_int32]
+    lib.pph_destroy.argtypes = [C.c_void_p]
+    lib.pph_last_error.argtypes = [C.c_void_p]
+    lib.pph_last_error.restype = C.c_char_p
+    lib.pph_version.restype = C.c_char_p
+    lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
+                                    C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_int64, C.c_int32, _dp]
+    for name in ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
+                 "pph_destroy", "pph_debug_zgemm"):
+        getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class PphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"parphom_hip error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc, handle=None):
+    if rc != 0:
+        msg = load().pph_last_error(handle)
+        raise PphError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_opts(jobz="N", rng="A", vel=False, vel_method="A", precision="D", vl=0.0, vu=0.0,
+              il=1, iu=1, abstol=float(np.float32(1e-15)), orfac=float(np.float32(1e-15)),
+              batch=0, device=0) -> PphOpts:
+    return PphOpts(jobz.encode(), rng.encode(), vel_method.encode(), precision.encode(),
+                   1 if vel else 0, device, vl, vu, il, iu, abstol, orfac, batch)
+
+
+class Handle:
+    """Device state for one structure (pph_create / pph_destroy)."""
+
+    def __init__(self, lat, pos, types, masses, fc, **opt):
+        lib = load()
+        self.lat = np.asfortranarray(np.asarray(lat, np.float64))
+        self.pos = np.asfortranarray(np.asarray(pos, np.float64))
+        self.types = np.ascontiguousarray(np.asarray(types, np.int64))
+        self.masses = np.ascontiguousarray(np.asarray(masses, np.float64))
+        fc = np.ascontiguousarray(np.asarray(fc, np.float64))
+        N = self.pos.shape[0]
+        if fc.shape != (N, N, 3, 3):
+            raise ValueError(f"fc must be (N,N,3,3) with N={N}, got {fc.shape}")
+        self.N, self.n = N, 3 * N
+        self.opts = make_opts(**opt)
+        self._sys = PphSystem(N, len(self.masses), _ptr(self.lat), _ptr(self.pos),
+                              _ptr(self.masses), _ptr(self.types, C.c_int64))
+        h = C.c_void_p()
+        _check(lib.pph_create(C.byref(self._sys), _ptr(fc), C.byref(self.opts), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().pph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def mcap(self):
+        o = self.opts
+        if o.range == b"I":
+            return max(1, min(self.n, o.iu) - max(1, o.il) + 1)
+        return self.n
+
+    def run(self, q, evec=None, vel=None):
+        """Host-buffer pipeline.  Returns dict(eval_cm[list of arrays], m, nz,
+        info, evec[list], vel[list])."""
+        lib = load()
+        q = np.asfortranarray(np.atleast_2d(np.asarray(q, np.float64)))
+        nq = q.shape[0]
+        n, mcap = self.n, self.mcap
+        want_vec = self.opts.jobz == b"V" if evec is None else evec
+        want_vel = bool(self.opts.vel) if vel is None else vel
+        ev = np.zeros((nq, mcap))
+        m = np.zeros(nq, np.int64)
+        nz = np.zeros(nq, np.int64)
+        info = np.zeros(nq, np.int32)
+        Z = np.zeros((nq, mcap, n), np.complex128) if want_vec else None
+        V = np.zeros((nq, mcap, 3)) if want_vel else None
+        out = PphOut(_ptr(ev), mcap, _ptr(m, C.c_int64), _ptr(nz, C.c_int64),
+                     Z.ctypes.data if Z is not None else None, mcap,
+                     _ptr(V) if V is not None else None, _ptr(info, C.c_int32))
+        _check(lib.pph_run(self._h, nq, _ptr(q), nq, C.byref(out)), self._h)
+        res = {"eval_cm": [ev[i, :m[i]] for i in range(nq)], "m": m, "nz": nz, "info": info}
+        res["evec"] = [Z[i, :nz[i]].T for i in range(nq)] if Z is not None else None
+        res["vel"] = [V[i, :nz[i]] for i in range(nq)] if V is not None else None
+        return res
+
+    def run_device(self, q, out_tensors):
+        """Device-buffer pipeline; out_tensors holds torch tensors on the GPU:
+        eval_cm (nq, ld), m (nq,) int64, info (nq,) int32, optional nz, evec
+        (nq, cols, n) complex128, vel (nq, cols, 3)."""
+        lib = load()
+        q = np.asfortranarray(np.atleast_2d(np.asarray(q, np.float64)))
+        nq = q.shape[0]
+        t = out_tensors
+
+        def dp(x, ct=C.c_double):
+            return C.cast(C.c_void_p(x.data_ptr()), C.POINTER(ct)) if x is not None else None
+
+        cols = t["evec"].shape[1] if t.get("evec") is not None else (
+            t["vel"].shape[1] if t.get("vel") is not None else 0)
+        out = PphOut(dp(t["eval_cm"]), t["eval_cm"].shape[1], dp(t["m"], C.c_int64),
+                     dp(t.get("nz"), C.c_int64),
+                     C.c_void_p(t["evec"].data_ptr()) if t.get("evec") is not None else None,
+                     cols, dp(t.get("vel")), dp(t["info"], C.c_int32))
+        _check(lib.pph_run_device(self._h, nq, _ptr(q), nq, C.byref(out)), self._h)
+
+    def assemble(self, q, what=0):
+        """D(q) (what=0) or dD/dq_{x,y,z} (what=1..3) as (nq, n, n) complex128."""
+        lib = load()
+        q = np.asfortranarray(np.atleast_2d(np.asarray(q, np.float64)))
+        nq = q.shape[0]
+        out = np.zeros((nq, self.n, self.n), np.complex128)   # each matrix column-major
+        _check(lib.pph_assemble(self._h, nq, _ptr(q), nq, what, out.ctypes.data, 0), self._h)
+        return out.transpose(0, 2, 1)
+
+
+def heevx(A, jobz="V", rng="A", vl=0.0, vu=0.0, il=1, iu=1, precision="D",
+          abstol=float(np.float32(1e-15)), orfac=float(np.float32(1e-15)), batch=0, device=0):
+    """pzheevx-semantics eigensolve of Hermitian matrices A (nmat, n, n) or (n, n).
+    vl/vu are in lambda units here.  Returns (w list, Z list or None, info)."""
+    lib = load()
+    A = np.asarray(A, np.complex128)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    nmat, n, _ = A.shape
+    Af = np.ascontiguousarray(A.transpose(0, 2, 1))          # column-major per matrix
+    o = make_opts(jobz=jobz, rng=rng, vl=vl, vu=vu, il=il, iu=iu, precision=precision,
+                  abstol=abstol, orfac=orfac, batch=batch, device=device)
+    w = np.zeros((nmat, n))
+    Z = np.zeros((nmat, n, n), np.complex128) if jobz == "V" else None
+    m = np.zeros(nmat, np.int64)
+    info = np.zeros(nmat, np.int32)
+    _check(lib.pph_heevx(n, nmat, Af.ctypes.data, C.byref(o), _ptr(w),
+                         Z.ctypes.data if Z is not None else None, _ptr(m, C.c_int64),
+                         _ptr(info, C.c_int32), 0))
+    ws = [w[i, :m[i]] for i in range(nmat)]
+    Zs = [Z[i, :m[i]].T for i in range(nmat)] if Z is not None else None
+    if single:
+        return ws[0], (Zs[0] if Zs else None), int(info[0])
+    return ws, Zs, info
+
+
+def debug_zgemm(A, B, opA="N", opB="N", prec="D", mode=0, Cin=None):
+    """Run the library's MFMA complex GEMM on host arrays (test hook)."""
+    lib = load()
+    dt = np.complex64 if prec == "S" else np.complex128
+    A = np.asfortranarray(A.astype(dt))
+    B = np.asfortranarray(B.astype(dt))
+    m = A.shape[0] if opA == "N" else A.shape[1]
+    k = A.shape[1] if opA == "N" else A.shape[0]
+    n = B.shape[1] if opB == "N" else B.shape[0]
+    Cm = np.asfortranarray(np.zeros((m, n), dt) if Cin is None else Cin.astype(dt))
+    dot = np.zeros(n)
+    _check(lib.pph_debug_zgemm(prec.encode(), opA.encode(), opB.encode(), m, n, k, A.ctypes.data,
+                               A.shape[0], B.ctypes.data, B.shape[0], Cm.ctypes.data, m, mode,
+                               _ptr(dot)))
+    return Cm, dot

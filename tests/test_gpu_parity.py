@@ -1,0 +1,140 @@
+"""GPU parity: the HIP library against the FP64 oracle (marked gpu)."""
+import numpy as np
+import pytest
+
+from tests import _util as U
+from oracle import phonon as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    from parphom_amd import lib
+    lib.load()
+    return lib
+
+
+@pytest.mark.parametrize("prec,tol", [("D", 1e-12), ("S", 2e-5)])
+@pytest.mark.parametrize("ops", ["NN", "CN", "NC", "CC"])
+def test_gemm_ops(gpu_available, prec, tol, ops):
+    lib = _lib()
+    rng = np.random.default_rng(1)
+    m, n, k = 77, 130, 45
+    opA, opB = ops
+    A = rng.standard_normal((m, k) if opA == "N" else (k, m)) + 1j * rng.standard_normal((m, k) if opA == "N" else (k, m))
+    B = rng.standard_normal((k, n) if opB == "N" else (n, k)) + 1j * rng.standard_normal((k, n) if opB == "N" else (n, k))
+    Aop = A if opA == "N" else A.conj().T
+    Bop = B if opB == "N" else B.conj().T
+    ref = Aop @ Bop
+    C, _ = lib.debug_zgemm(A, B, opA, opB, prec=prec, mode=0)
+    assert np.abs(C - ref).max() <= tol * np.abs(ref).max() * 10
+
+
+@pytest.mark.parametrize("prec,tol", [("D", 1e-12), ("S", 2e-5)])
+def test_gemm_dot_splitk_twoproduct(gpu_available, prec, tol):
+    lib = _lib()
+    rng = np.random.default_rng(2)
+    m, n, k = 150, 70, 64
+    A = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+    B = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+    X = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    ref = A @ B
+    _, dot = lib.debug_zgemm(A, B, prec=prec, mode=1, Cin=X)
+    dref = np.einsum("ij,ij->j", X.conj(), ref).real
+    assert np.abs(dot - dref).max() <= tol * np.abs(dref).max() * 10
+    C2, _ = lib.debug_zgemm(A, B, prec=prec, mode=2)
+    assert np.abs(C2 - ref).max() <= tol * np.abs(ref).max() * 10
+    C3, _ = lib.debug_zgemm(A, B, prec=prec, mode=3)
+    assert np.abs(C3 - ref).max() <= tol * np.abs(ref).max() * 10
+
+
+@pytest.mark.parametrize("mn,seed", [((1, 2), 11), ((3, 4), 1003)])
+@pytest.mark.parametrize("method", ["A", "C"])
+def test_assembly_matches_oracle(gpu_available, mn, seed, method):
+    lib = _lib()
+    st, fc = U.structure(*mn, seed)
+    S = U.oracle_system(*mn, seed)
+    qs = np.array([[0.0, 0.0, 0.0], [2 / 3, 1 / 3, 0.0], [0.123, 0.456, 0.0], [-0.31, 0.77, 0.2]])
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, vel_method=method) as h:
+        D = h.assemble(qs, 0)
+        for iq, q in enumerate(qs):
+            ref = S.D(q)
+            assert np.abs(D[iq] - ref).max() <= 1e-13 * np.abs(ref).max()
+        for a in (1, 2, 3):
+            dD = h.assemble(qs, a)
+            for iq, q in enumerate(qs):
+                ref = S.D(q, a, method)
+                scale = max(np.abs(ref).max(), 1e-30)
+                tol = 1e-12 if method == "A" else 1e-6
+                assert np.abs(dD[iq] - ref).max() <= tol * scale, (a, iq)
+
+
+@pytest.mark.parametrize("prec", ["D", "S"])
+def test_heevx_random_hermitian(gpu_available, prec):
+    lib = _lib()
+    rng = np.random.default_rng(5)
+    n = 200
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = X + X.conj().T
+    # planted exact degeneracies
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lam = np.sort(rng.uniform(-3, 3, n))
+    lam[10:13] = lam[10]
+    lam[50:52] = lam[50]
+    A2 = (Q * lam) @ Q.conj().T
+    A2 = 0.5 * (A2 + A2.conj().T)
+    for M in (A, A2):
+        w, Z, info = lib.heevx(M, jobz="V", precision=prec)
+        wref = np.linalg.eigvalsh(M)
+        nrm = np.abs(wref).max()
+        tol = 1e-12 if prec == "D" else 1e-5
+        assert info == 0
+        assert np.abs(w - wref).max() <= tol * nrm * (1 if prec == "D" else 1)
+        R = M @ Z - Z * w
+        assert np.linalg.norm(R, axis=0).max() <= (1e-11 if prec == "D" else 1e-5) * nrm
+        if prec == "D":
+            assert np.abs(Z.conj().T @ Z - np.eye(n)).max() <= 1e-10
+
+
+def test_heevx_ranges(gpu_available):
+    lib = _lib()
+    rng = np.random.default_rng(6)
+    n = 150
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = X + X.conj().T
+    wref = np.linalg.eigvalsh(A)
+    w, Z, info = lib.heevx(A, jobz="V", rng="I", il=5, iu=20)
+    assert len(w) == 16 and np.abs(w - wref[4:20]).max() < 1e-11
+    assert np.linalg.norm(A @ Z - Z * w, axis=0).max() < 1e-10
+    vl, vu = -3.0, 2.5
+    w, Z, info = lib.heevx(A, jobz="N", rng="V", vl=vl, vu=vu)
+    sel = wref[(wref > vl) & (wref <= vu)]
+    assert len(w) == len(sel) and np.abs(w - sel).max() < 1e-11
+
+
+@pytest.mark.parametrize("prec", ["D", "S"])
+def test_run_tbg_3_4_path(gpu_available, prec):
+    """Config-1 structure: eigenvalues (<= 1e-3 cm^-1) and velocities."""
+    lib = _lib()
+    st, fc = U.structure(3, 4, 1003)
+    S = U.oracle_system(3, 4, 1003)
+    from parphom_amd import synth
+    qs = synth.q_path_gkmg()[[0, 5, 26, 30, 39, 50]]
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                    precision=prec) as h:
+        res = h.run(qs)
+    for iq, q in enumerate(qs):
+        ref = S.solve(q, vel=True)
+        om = res["eval_cm"][iq]
+        assert res["info"][iq] == 0
+        assert np.abs(om - ref["eval_cm"]).max() <= 1e-3, np.abs(om - ref["eval_cm"]).max()
+        if prec == "D":
+            groups = U.clusters(ref["lam"], 1e-6)
+            dD = [S.D(q, a) for a in (1, 2, 3)]
+            v = res["vel"][iq]
+            vs_ref = U.vel_cluster_sums(ref["vel"], groups)
+            vs = U.vel_cluster_sums(v, groups)
+            assert np.abs(vs - vs_ref).max() <= 1e-6 * np.abs(vs_ref).max() + 1e-9
+            vp = U.vel_projected(res["evec"][iq], dD, ref["eval_cm"], groups)
+            vp_ref = U.vel_projected(ref["evec"], dD, ref["eval_cm"], groups)
+            assert np.abs(vp - vp_ref).max() <= 1e-6 * np.abs(vp_ref).max() + 1e-9
