@@ -1,0 +1,169 @@
+"""Throughput of the phonon hot path on one MI355X.
+
+Workload (default): commensurate twisted bilayer graphene (11,12), N = 1588
+atoms, n = 3N = 4764 complex Hermitian D(q) from the seeded spring-model
+force constants, 128 random q-points; every step is one device batch of q
+through assembly -> eigensolve (all eigenpairs) -> group velocities, with
+the FP32 matrix-core eigensolver and FP64 refinement (precision 'S').  The
+same batch on the FP64 path (precision 'D') is timed alongside, and the
+frequencies of both are compared.
+
+Prints one JSON line (see the keys below).  Synthetic data; inputs (force
+constants, geometry, tables) are resident on the GPU before timing starts,
+outputs stay in device buffers.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12            # B/s
+FP32_PEAK = 157.3e12         # FLOP/s (dense, matrix and vector)
+FP64_PEAK = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def device_outputs(torch, B, n):
+    dev = torch.device("cuda:0")
+    return {
+        "eval_cm": torch.zeros((B, n), dtype=torch.float64, device=dev),
+        "m": torch.zeros(B, dtype=torch.int64, device=dev),
+        "nz": torch.zeros(B, dtype=torch.int64, device=dev),
+        "info": torch.zeros(B, dtype=torch.int32, device=dev),
+        "evec": torch.zeros((B, n, n), dtype=torch.complex128, device=dev),
+        "vel": torch.zeros((B, n, 3), dtype=torch.float64, device=dev),
+    }
+
+
+def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup):
+    n = 3 * st.natom
+    h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision=prec,
+                   batch=B)
+    out = device_outputs(torch, B, n)
+    nq = len(qs)
+
+    def batch(i):
+        idx = [(i * B + k) % nq for k in range(B)]
+        return qs[idx]
+
+    for i in range(warmup):
+        h.run_device(batch(i), out)
+        log(f"[bench] {prec} warmup {i + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    h.profile(enable=True, reset=True)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        h.run_device(batch(warmup + i), out)
+        log(f"[bench] {prec} step {i + 1}/{steps} done")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = h.profile(enable=False)
+    # accuracy check batch (untimed): the first B q-points
+    h.run_device(qs[:B], out)
+    ev = out["eval_cm"].cpu().numpy().copy()
+    h.close()
+    del out
+    torch.cuda.empty_cache()
+    return dt, prof, ev
+
+
+def cpu_baseline(st, fc, q, seconds_cap=60.0):
+    """The FP64 oracle (numpy assembly, LAPACK zheevx, numpy GEMMs) on one
+    q-point of the same workload, on this host's cores."""
+    from oracle import phonon as O
+    S = O.System(st.lat, st.pos, st.types, st.masses, fc)
+    t0 = time.perf_counter()
+    r = S.solve(q, vel=True)
+    dt = time.perf_counter() - t0
+    return dt, r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--workload", default="tbg_11_12")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: the phonon path has no multi-GPU mode "
+              f"(q-points shard with no exchange); got --gpus {a.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from parphom_amd import lib, synth
+    lib.load()
+    t0 = time.perf_counter()
+    st, fc, qs = synth.workload(a.workload)
+    n = 3 * st.natom
+    log(f"[bench] workload {a.workload}: N={st.natom} n={n} nq={len(qs)} "
+        f"(inputs built in {time.perf_counter() - t0:.1f}s)")
+
+    B = a.batch
+    dt32, prof32, ev32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup)
+    qps32 = a.steps * B / dt32
+    res = {"metric": "q-pts/s at 3N=4764 eigpairs+v_g: FP32+refine vs FP64, max|dw|<=1e-3 cm^-1",
+           "value": qps32, "unit": "q-pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * dt32 / a.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c64 MFMA (FP32 acc) + f64 refinement",
+           "data": "synthetic (seeded spring-model force constants, random q)",
+           "config": {"workload": a.workload, "natom": st.natom, "n": n, "q_per_step": B,
+                      "range": "A", "eigenvectors": True, "group_velocity": True,
+                      "precision": "S"}}
+    phases = {k: v[0] / max(1, v[1]) for k, v in prof32.items()}
+    res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
+
+    # roofline of the assembly kernel (one launch per batch, HBM-bound):
+    # algorithmic bytes = Phi (8 n^2) + B * (D in c128 and c64, dD in c64)
+    asm_bytes = 8 * n * n + B * (16 + 8 + 3 * 8) * n * n
+    asm_s = phases["assemble"] / 1e3
+    # roofline of the velocity products (one launch per batch, 3 directions):
+    # algorithmic flops = B * 24 n^3 (complex n x n x n per direction)
+    vel_flops = B * 24.0 * n ** 3
+    vel_s = phases["velocity"] / 1e3
+    rl = {"assemble": {"bound": "hbm", "achieved": asm_bytes / asm_s / 1e9, "peak": HBM_PEAK / 1e9,
+                       "unit": "GB/s", "frac": asm_bytes / asm_s / HBM_PEAK, "traffic": None},
+          "velocity": {"bound": "mfma", "achieved": vel_flops / vel_s / 1e12,
+                       "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                       "frac": vel_flops / vel_s / FP32_PEAK, "traffic": None}}
+    dominant = max(("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity"),
+                   key=lambda k: phases[k])
+    res["dominant_phase"] = dominant
+    res["roofline"] = dict(rl["velocity"], kernel="velocity products (k_zgemm dot epilogue, FP32)")
+    res["roofline_assembly"] = dict(rl["assemble"], kernel="k_assemble")
+
+    if not a.no_fp64:
+        dt64, prof64, ev64 = time_path(lib, torch, st, fc, qs, "D", B, max(1, a.steps - 1), 0
+                                          if a.warmup == 0 else 1)
+        qps64 = max(1, a.steps - 1) * B / dt64
+        dw = float(np.abs(ev32 - ev64).max())
+        res["fp64_path"] = {"value": qps64, "unit": "q-pts/s",
+                            "phase_ms_per_batch": {k: round(v[0] / max(1, v[1]), 3)
+                                                   for k, v in prof64.items()}}
+        res["speedup_vs_fp64"] = qps32 / qps64
+        res["max_abs_domega_vs_fp64_cm"] = dw
+
+    if not a.no_cpu:
+        cdt, r = cpu_baseline(st, fc, qs[0])
+        res["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "q-pts/s",
+                               "cores": os.cpu_count(), "kind": "port",
+                               "sample": f"1 q-point of {a.workload} (assembly + zheevx all "
+                                         f"vectors + 3 velocity GEMMs, numpy/OpenBLAS)"}
+        res["max_abs_domega_vs_oracle_cm"] = float(np.abs(ev32[0] - r["eval_cm"]).max())
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

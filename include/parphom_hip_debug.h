@@ -19,6 +19,14 @@ extern "C" {
 int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t k,
                     const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                     int64_t ldc, int32_t mode, double* dot);
+/* Phase timers measured with HIP events on the handle's stream around each
+ * phase of pph_run / pph_run_device.  Slots: 0 assembly, 1 tridiagonal
+ * reduction, 2 tridiagonal eigensolver, 3 back-transformation, 4 FP64
+ * refinement, 5 velocity products, 6 whole batch.  ms/counts (7 entries
+ * each, may be NULL) receive the accumulated milliseconds and batch counts;
+ * reset != 0 clears them; enable turns recording on or off. */
+int pph_profile(void* h, int32_t enable, int32_t reset, double* ms, int64_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

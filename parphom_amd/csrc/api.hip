@@ -79,8 +79,49 @@ static void linsolve(const double* x, const double* A, double* u) {
   for (int k = 0; k < 3; ++k) u[k] = L(B, 0, k) * x[0] + L(B, 1, k) * x[1] + L(B, 2, k) * x[2];
 }
 
+// phase timers recorded on the handle's stream (HIP events)
+enum Phase { PH_ASM = 0, PH_TRID, PH_TRIEIG, PH_BACK, PH_REFINE, PH_VEL, PH_TOTAL, PH_N };
+
+struct Timers {
+  bool on = false;
+  hipEvent_t ev[2 * PH_N] = {};
+  bool used[PH_N] = {};
+  double ms[PH_N] = {};
+  int64_t cnt[PH_N] = {};
+  void init() {
+    if (ev[0]) return;
+    for (auto& e : ev) PPH_HIP(hipEventCreate(&e));
+  }
+  void begin(int p, hipStream_t s) {
+    if (!on) return;
+    PPH_HIP(hipEventRecord(ev[2 * p], s));
+    used[p] = true;
+  }
+  void end(int p, hipStream_t s) {
+    if (!on) return;
+    PPH_HIP(hipEventRecord(ev[2 * p + 1], s));
+  }
+  void collect() {
+    if (!on) return;
+    for (int p = 0; p < PH_N; ++p) {
+      if (!used[p]) continue;
+      PPH_HIP(hipEventSynchronize(ev[2 * p + 1]));
+      float t = 0.f;
+      PPH_HIP(hipEventElapsedTime(&t, ev[2 * p], ev[2 * p + 1]));
+      ms[p] += t;
+      cnt[p] += 1;
+      used[p] = false;
+    }
+  }
+  ~Timers() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
 struct Handle {
   int device = 0;
+  Timers tm;
   pph_opts opts{};
   Geometry g;
   int64_t N = 0, n = 0;
@@ -153,7 +194,9 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   tw.scal = h.scal.as<double>(); tw.strideScal = 256;
   tw.cnt = h.cnt.as<unsigned>(); tw.strideCnt = 4;
   tw.nchunk = h.nchunk;
+  h.tm.begin(PH_TRID, s);
   tridiagonalise<T>(tw, s);
+  h.tm.end(PH_TRID, s);
 
   // ---- tridiagonal eigenproblem in FP64
   h.tdd.ensure((size_t)B * n * sizeof(double));
@@ -176,6 +219,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   te.w = h.tw.as<double>(); te.ldw = mcap;
   te.info = h.tinf.as<int>();
   te.want_vectors = vectors ? 1 : 0;
+  h.tm.begin(PH_TRIEIG, s);
   tri_prepare(te, s);
   std::vector<int64_t> ilo(B), ihi(B);
   PPH_HIP(hipMemcpyAsync(ilo.data(), te.ilo, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -191,6 +235,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   te.mmax = std::max<int64_t>(mmax, 1);
   tri_bisect(te, s);
   R.lam = te.w;
+  if (!(vectors && mmax > 0)) h.tm.end(PH_TRIEIG, s);
 
   if (vectors && mmax > 0) {
     const int64_t ldzt = mcap;
@@ -199,6 +244,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
     te.zt = h.tzt.as<double>(); te.ldzt = ldzt; te.strideZt = n * ldzt;
     te.fac = h.tfac.as<double>(); te.strideFac = 5 * n * ldzt;
     tri_vectors(te, s);
+    h.tm.end(PH_TRIEIG, s);
 
     // ---- back-transformation
     h.Z.ensure((size_t)B * n * mcap * sizeof(cplx<T>));
@@ -219,12 +265,15 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
     h.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     h.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     bw.Tm = h.Tm.as<cplx<T>>(); bw.VT = h.VT.as<cplx<T>>(); bw.Wt = h.Wt.as<cplx<T>>();
+    h.tm.begin(PH_BACK, s);
     back_transform<T>(bw, s);
+    h.tm.end(PH_BACK, s);
     R.Zt = h.Z.p;
 
     if (fp32) {
       // ---- FP64 refinement: lambda_k = Re(z_k^H D z_k) / |z_k|^2 with D in FP64
       h.Z64.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
+      h.tm.begin(PH_REFINE, s);
       launch_convert<float, double>(h.Z.as<cplx<float>>(), h.Z64.as<cplx<double>>(), B * n * mcap, s);
       h.num.ensure((size_t)B * mcap * sizeof(double));
       h.den.ensure((size_t)B * mcap * sizeof(double));
@@ -242,6 +291,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
       launch_colnorm2<double>(h.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, h.den.as<double>(),
                               mcap, s);
       launch_rq_finish(h.num.as<double>(), h.den.as<double>(), te.w, B, mmax, mcap, s);
+      h.tm.end(PH_REFINE, s);
       R.Z64 = h.Z64.as<cplx<double>>();
     } else {
       R.Z64 = reinterpret_cast<cplx<double>*>(h.Z.p);
@@ -282,7 +332,10 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
     if (fp32) ap.dD32 = h.dD.as<cplx<float>>();
     else ap.dD64 = h.dD.as<cplx<double>>();
   }
+  h.tm.begin(PH_TOTAL, s);
+  h.tm.begin(PH_ASM, s);
   launch_assemble(ap, s);
+  h.tm.end(PH_ASM, s);
 
   const double K32 = (double)(float)(15.633302f * 33.35641f);
   const double vl_l = o.vl / K32, vu_l = o.vu / K32;
@@ -300,6 +353,7 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
   if (vel && mmax > 0) {
     h.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
     launch_fill_zero(h.vacc.p, B * mcap * 3 * sizeof(double), s);
+    h.tm.begin(PH_VEL, s);
     if (fp32) {
       GemmParams<float> g{};
       g.opA = 'N'; g.opB = 'N';
@@ -323,9 +377,11 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
       g.dotX = h.Z.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
       launch_gemm<double>(g, s);
     }
+    h.tm.end(PH_VEL, s);
     launch_vel_scale(h.vacc.as<double>(), h.om.as<double>(), B, mmax, mcap, mcap, s);
     out.vel = h.vacc.as<double>() + B * 3 * mcap;
   }
+  h.tm.end(PH_TOTAL, s);
 }
 
 static int64_t auto_batch(const Handle& h, int64_t nq) {
@@ -492,6 +548,7 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
       for (int c = 0; c < 3; ++c) qrows[3 * b + c] = q[(q0 + b) + ldq * c];
     BatchOut bo;
     run_batch(*h, B, qrows.data(), bo);
+    h->tm.collect();
     const EigResult& R = bo.R;
     for (int64_t b = 0; b < B; ++b) {
       const int64_t iq = q0 + b;
@@ -654,6 +711,25 @@ int pph_destroy(pph_handle* hh) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------------- measurement hook
+extern "C" int pph_profile(pph_handle* hh, int32_t enable, int32_t reset, double* ms,
+                           int64_t* counts) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_profile: handle is NULL", PPH_ERR_ARG);
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(h->device));
+  h->tm.init();
+  if (ms)
+    for (int p = 0; p < PH_N; ++p) ms[p] = h->tm.ms[p];
+  if (counts)
+    for (int p = 0; p < PH_N; ++p) counts[p] = h->tm.cnt[p];
+  if (reset)
+    for (int p = 0; p < PH_N; ++p) { h->tm.ms[p] = 0; h->tm.cnt[p] = 0; }
+  h->tm.on = enable != 0;
+  return PPH_OK;
+  PPH_API_CATCH(h)
+}
+
 // ---------------------------------------------------------------- test hook
 extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t k,
                                const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
@@ -686,6 +762,7 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
     } else if (mode == 3) {
       // C = A(:, 0:k/2) B(0:k/2, :) + A(:, k/2:) B(k/2:, :) through the two-product path (opA = opB = 'N')
       g.k1 = k / 2; g.A2 = dA.as<cplx<T>>() + (k / 2) * lda; g.B2 = dB.as<cplx<T>>() + k / 2;
+      g.strideA2 = 0; g.strideB2 = 0;
     }
     launch_gemm<T>(g, s);
   };

@@ -15,17 +15,22 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct mfma;
+// Operand layout of both 16x16x4 instructions: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16].  Result: lane l holds column l % 16; accumulator
+// register r holds row 4 * (l / 16) + r for f32 but row 4 * r + l / 16 for f64.
 template <> struct mfma<double> {
   typedef d4 acc;
   static __device__ __forceinline__ acc op(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + lane / 16; }
 };
 template <> struct mfma<float> {
   typedef f4 acc;
   static __device__ __forceinline__ acc op(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
 constexpr int GBM = 64, GBN = 64, GBK = 16;
@@ -50,8 +55,8 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
   const cplx<T>* A1 = p.A + offA;
   const cplx<T>* B1 = p.B + offB;
   const bool two = (p.A2 != nullptr);
-  const cplx<T>* A2 = two ? p.A2 + offA : A1;
-  const cplx<T>* B2 = two ? p.B2 + offB : B1;
+  const cplx<T>* A2 = two ? p.A2 + b * p.strideA2 : A1;
+  const cplx<T>* B2 = two ? p.B2 + b * p.strideB2 : B1;
   const int64_t M = p.m, N = p.n;
   // K range of this split
   int64_t kbeg = 0, K = p.k;
@@ -154,7 +159,7 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
         if (col >= N) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int64_t row = row0 + wr0 + mi * 16 + 4 * (lane / 16) + i;
+          const int64_t row = row0 + wr0 + mi * 16 + mfma<T>::row(lane, i);
           if (row >= M) continue;
           cplx<T> v{accR[mi][ni][i], accI[mi][ni][i]};
           v = cmul(p.alpha, v);
@@ -180,7 +185,7 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int64_t row = row0 + wr0 + mi * 16 + 4 * (lane / 16) + i;
+            const int64_t row = row0 + wr0 + mi * 16 + mfma<T>::row(lane, i);
             if (row >= M) continue;
             const cplx<T> x = X[row + col * p.lddx];
             // Re(conj(x) * acc)

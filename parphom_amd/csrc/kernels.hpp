@@ -37,8 +37,9 @@ struct GemmParams {
   const cplx<T>* A; int64_t lda, strideA;
   const cplx<T>* B; int64_t ldb, strideB;
   // optional second product: K range [k1, k) uses (A2, B2) with the same
-  // leading dimensions, strides and ops (k1 must be a multiple of 16)
-  const cplx<T>* A2; const cplx<T>* B2; int64_t k1;
+  // leading dimensions and ops, batch strides strideA2 / strideB2 (k1 must be a
+  // multiple of 16)
+  const cplx<T>* A2; const cplx<T>* B2; int64_t k1, strideA2, strideB2;
   cplx<T>* C; int64_t ldc, strideC;
   int64_t batch;
   // inner batch: blockIdx.z = (outer * binner + inner) * ksplit + split; operand

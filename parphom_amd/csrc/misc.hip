@@ -7,7 +7,7 @@
 namespace pph {
 
 // f32-rounded literals as the Fortran source promotes them to f64
-constexpr double C1 = 15.633302688598633;       // f32(15.633302)
+constexpr double C1 = 15.633301734924316;       // f32(15.633302)
 constexpr double C2 = 33.356410980224609;       // f32(33.35641)
 constexpr double C1SQ = 244.40011596679688;     // f32(f32(15.633302)**2)
 constexpr double PI32 = 3.1415927410125732;     // f32(pi)

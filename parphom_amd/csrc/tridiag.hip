@@ -306,7 +306,8 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
     g.alpha = cplx<T>{-1, 0}; g.beta = cplx<T>{1, 0};
     g.A = w.V + r0 + p * n; g.lda = n; g.strideA = w.strideV;
     g.B = w.W + r0; g.ldb = n; g.strideB = w.strideW;
-    g.A2 = w.W + r0; g.B2 = w.V + r0 + p * n;
+    g.A2 = w.W + r0; g.strideA2 = w.strideW;
+    g.B2 = w.V + r0 + p * n; g.strideB2 = w.strideV;
     g.C = w.A + r0 + r0 * w.lda; g.ldc = w.lda; g.strideC = w.strideA;
     g.batch = w.batch;
     launch_gemm<T>(g, s);

@@ -38,7 +38,9 @@ class PphOut(C.Structure):
 
 
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
-            "pph_destroy", "pph_last_error", "pph_version", "pph_debug_zgemm")
+            "pph_destroy", "pph_last_error", "pph_version", "pph_debug_zgemm", "pph_profile")
+
+PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total")
 
 _lib = None
 
@@ -63,6 +65,8 @@ def load(path: str = LIB_PATH):
     lib.pph_last_error.argtypes = [C.c_void_p]
     lib.pph_last_error.restype = C.c_char_p
     lib.pph_version.restype = C.c_char_p
+    lib.pph_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _i64p]
+    lib.pph_profile.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -184,6 +188,14 @@ class Handle:
                      C.c_void_p(t["evec"].data_ptr()) if t.get("evec") is not None else None,
                      cols, dp(t.get("vel")), dp(t["info"], C.c_int32))
         _check(lib.pph_run_device(self._h, nq, _ptr(q), nq, C.byref(out)), self._h)
+
+    def profile(self, enable=True, reset=False):
+        """Accumulated phase times (ms) and batch counts, then set recording."""
+        ms = np.zeros(len(PHASES))
+        cnt = np.zeros(len(PHASES), np.int64)
+        _check(load().pph_profile(self._h, 1 if enable else 0, 1 if reset else 0, _ptr(ms),
+                                  _ptr(cnt, C.c_int64)), self._h)
+        return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(PHASES)}
 
     def assemble(self, q, what=0):
         """D(q) (what=0) or dD/dq_{x,y,z} (what=1..3) as (nq, n, n) complex128."""

@@ -51,3 +51,22 @@ def vel_projected(Z, dD_list, om_cm, groups):
 
 def vel_cluster_sums(v, groups):
     return np.array([v[g].sum(axis=0) for g in groups])
+
+
+def vel_noise(dD_list, om_cm, n, kappa=20.0):
+    """Absolute noise level of the analytic velocity of each mode in FP64:
+    eigenvectors carry errors ~ n eps |D|, so <e|dD|e> is known to about
+    n eps max|dD|, then scaled by the velocity factor / max(2|omega|, 1e-3)."""
+    scale = max(np.abs(d).max() for d in dD_list)
+    den = np.maximum(2.0 * np.abs(om_cm), 1e-3)
+    const = O.C2 * O.C1SQ32 * 100.0 * 2.0 * O.PI32
+    return kappa * n * np.finfo(float).eps * scale * const / den
+
+
+def assert_vel_close(v, v_ref, groups, noise, rel=1e-6, floor=1e-9):
+    """Per-cluster comparison: |a - b| <= rel |b| + floor + noise."""
+    for g in groups:
+        a = v[g].sum(axis=0)
+        b = v_ref[g].sum(axis=0)
+        tol = rel * np.abs(v_ref[g]).sum(axis=0) + floor + noise[g].sum()
+        assert np.all(np.abs(a - b) <= tol), (g, a, b, tol)
