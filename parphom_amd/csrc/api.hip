@@ -7,6 +7,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -133,7 +134,7 @@ struct Handle {
   // batch workspaces
   DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, cnt;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
-  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om;
+  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
   int nb_panel = 32;
   int nchunk = 64;
   ~Handle() {
@@ -141,6 +142,106 @@ struct Handle {
   }
 };
 
+
+// ---------------------------------------------------------------- small dense FP64 (host)
+// Eigenvalues of the Hermitian pencil (H, G), G positive definite, k <= 64:
+// Cholesky G = L L^H, A = L^-1 H L^-H, cyclic complex Jacobi on A.
+// Returns eigenvalues ascending and the vectors U (k x k, column-major) with
+// U^H G U = I.
+static void small_gen_eig(int k, std::vector<std::complex<double>> H, std::vector<std::complex<double>> G,
+                          std::vector<double>& w, std::vector<std::complex<double>>& U) {
+  using cd = std::complex<double>;
+  auto at = [k](std::vector<cd>& M, int i, int j) -> cd& { return M[i + (size_t)j * k]; };
+  // Cholesky (lower) in place in G
+  for (int j = 0; j < k; ++j) {
+    double d = at(G, j, j).real();
+    for (int l = 0; l < j; ++l) d -= std::norm(at(G, j, l));
+    d = std::sqrt(std::max(d, 1e-300));
+    at(G, j, j) = d;
+    for (int i = j + 1; i < k; ++i) {
+      cd s = at(G, i, j);
+      for (int l = 0; l < j; ++l) s -= at(G, i, l) * std::conj(at(G, j, l));
+      at(G, i, j) = s / d;
+    }
+    for (int i = 0; i < j; ++i) at(G, i, j) = 0.0;
+  }
+  // A = L^-1 H L^-H : solve L X = H, then A^H = L^-1 X^H
+  std::vector<cd> X(H);
+  for (int c = 0; c < k; ++c)
+    for (int i = 0; i < k; ++i) {
+      cd s = at(X, i, c);
+      for (int l = 0; l < i; ++l) s -= at(G, i, l) * at(X, l, c);
+      at(X, i, c) = s / at(G, i, i);
+    }
+  std::vector<cd> A((size_t)k * k);
+  for (int r = 0; r < k; ++r)          // row r of X -> solve L a = conj(X(r,:))^T
+    for (int i = 0; i < k; ++i) {
+      cd s = std::conj(at(X, r, i));
+      for (int l = 0; l < i; ++l) s -= at(G, i, l) * at(A, l, r);
+      at(A, i, r) = s / at(G, i, i);
+    }
+  for (int i = 0; i < k; ++i) {          // Hermitise
+    at(A, i, i) = at(A, i, i).real();
+    for (int j = i + 1; j < k; ++j) {
+      const cd m = 0.5 * (at(A, i, j) + std::conj(at(A, j, i)));
+      at(A, i, j) = m;
+      at(A, j, i) = std::conj(m);
+    }
+  }
+  std::vector<cd> Q((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) at(Q, i, i) = 1.0;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) (i == j ? tot : off) += std::norm(at(A, i, j));
+    if (off <= 1e-32 * std::max(tot, 1e-300)) break;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const cd apq = at(A, p, q);
+        const double ab = std::abs(apq);
+        if (ab == 0.0) continue;
+        const double app = at(A, p, p).real(), aqq = at(A, q, q).real();
+        const cd em = std::conj(apq / ab);            // e^{-i phi}, a_pq = |a_pq| e^{i phi}
+        const double tau = (aqq - app) / (2.0 * ab);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+        // J = diag(1, e^{-i phi}) [c s; -s c] on the (p, q) plane; A <- J^H A J, Q <- Q J
+        for (int r = 0; r < k; ++r) {
+          const cd arp = at(A, r, p), arq = at(A, r, q);
+          at(A, r, p) = c * arp - s * em * arq;
+          at(A, r, q) = s * arp + c * em * arq;
+        }
+        for (int r = 0; r < k; ++r) {
+          const cd apr = at(A, p, r), aqr = at(A, q, r);
+          at(A, p, r) = c * apr - s * std::conj(em) * aqr;
+          at(A, q, r) = s * apr + c * std::conj(em) * aqr;
+        }
+        for (int r = 0; r < k; ++r) {
+          const cd qrp = at(Q, r, p), qrq = at(Q, r, q);
+          at(Q, r, p) = c * qrp - s * em * qrq;
+          at(Q, r, q) = s * qrp + c * em * qrq;
+        }
+      }
+  }
+  // U = L^-H Q
+  U.assign((size_t)k * k, 0.0);
+  for (int c = 0; c < k; ++c)
+    for (int i = k - 1; i >= 0; --i) {
+      cd s = at(Q, i, c);
+      for (int l = i + 1; l < k; ++l) s -= std::conj(at(G, l, i)) * at(U, l, c);
+      at(U, i, c) = s / at(G, i, i);
+    }
+  std::vector<int> ord(k);
+  for (int i = 0; i < k; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return at(A, x, x).real() < at(A, y, y).real(); });
+  w.resize(k);
+  std::vector<cd> Us(U);
+  for (int c = 0; c < k; ++c) {
+    w[c] = at(A, ord[c], ord[c]).real();
+    for (int i = 0; i < k; ++i) Us[i + (size_t)c * k] = at(U, i, ord[c]);
+  }
+  U.swap(Us);
+}
 
 // ---------------------------------------------------------------- eigen pipeline
 struct EigResult {
@@ -155,6 +256,119 @@ struct EigResult {
 static int64_t range_mcap(const pph_opts& o, int64_t n) {
   if (o.range == 'I') return std::max<int64_t>(1, std::min(n, o.iu) - std::max<int64_t>(1, o.il) + 1);
   return n;
+}
+
+// Low-frequency modes after the Rayleigh quotients: their frequencies
+// omega = K sqrt(lambda) magnify eigenvalue errors, so the lowest modes
+// (omega < 50 cm^-1, at most 16 per matrix) get a Rayleigh-Ritz step inside
+// each cluster of close eigenvalues and a second-order correction
+// -sum_j |x_j^H r|^2 / ((rho_j - theta) |x_j|^2) from every other computed
+// mode j, all in FP64: W = D X_T, C1 = X^H W, C2 = X^H X_T on the GPU, the
+// k x k algebra on the host.
+static void refine_low_modes(Handle& h, int64_t B, const cplx<double>* D64, const cplx<double>* X,
+                             double* lam_dev, const double* nrm2_dev, const std::vector<int64_t>& Mv,
+                             int64_t mcap, hipStream_t s) {
+  const int64_t n = h.n;
+  const double K32 = (double)(float)(15.633302f * 33.35641f);
+  const double lam_low = (50.0 / K32) * (50.0 / K32);
+  const int64_t KMAX = 16;
+  std::vector<double> lam((size_t)B * mcap), nrm((size_t)B * mcap);
+  PPH_HIP(hipMemcpyAsync(lam.data(), lam_dev, lam.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipMemcpyAsync(nrm.data(), nrm2_dev, nrm.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
+  std::vector<int64_t> kb(B, 0);
+  int64_t K = 0;
+  for (int64_t b = 0; b < B; ++b) {
+    int64_t k = 0;
+    while (k < std::min(KMAX, Mv[b]) && lam[b * mcap + k] < lam_low) ++k;
+    kb[b] = k;
+    K = std::max(K, k);
+  }
+  if (K == 0) return;
+  const int64_t Kp = 16;
+  h.WT.ensure((size_t)B * n * Kp * sizeof(cplx<double>));
+  h.C12.ensure((size_t)B * 2 * mcap * Kp * sizeof(cplx<double>));
+  cplx<double>* WT = h.WT.as<cplx<double>>();
+  cplx<double>* C1 = h.C12.as<cplx<double>>();
+  cplx<double>* C2 = C1 + (size_t)B * mcap * Kp;
+  int64_t mmax = 0;
+  for (auto m : Mv) mmax = std::max(mmax, m);
+  {
+    GemmParams<double> g{};                 // W_T = D X(:, 0:K)
+    g.opA = 'N'; g.opB = 'N'; g.m = n; g.n = K; g.k = n;
+    g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
+    g.A = D64; g.lda = n; g.strideA = n * n;
+    g.B = X; g.ldb = n; g.strideB = n * mcap;
+    g.C = WT; g.ldc = n; g.strideC = n * Kp;
+    g.batch = B;
+    launch_gemm<double>(g, s);
+  }
+  for (int which = 0; which < 2; ++which) {   // C1 = X^H W_T, C2 = X^H X_T
+    GemmParams<double> g{};
+    g.opA = 'C'; g.opB = 'N'; g.m = mmax; g.n = K; g.k = n;
+    g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
+    g.A = X; g.lda = n; g.strideA = n * mcap;
+    g.B = which == 0 ? WT : X; g.ldb = n; g.strideB = which == 0 ? n * Kp : n * mcap;
+    g.C = which == 0 ? C1 : C2; g.ldc = mcap; g.strideC = mcap * Kp;
+    g.batch = B;
+    launch_gemm<double>(g, s);
+  }
+  using cd = std::complex<double>;
+  std::vector<cd> c1((size_t)B * mcap * Kp), c2((size_t)B * mcap * Kp);
+  PPH_HIP(hipMemcpyAsync(c1.data(), C1, c1.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipMemcpyAsync(c2.data(), C2, c2.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
+  const double tnorm_proxy = [&] {
+    double m = 0;
+    for (int64_t b = 0; b < B; ++b)
+      for (int64_t k = 0; k < Mv[b]; ++k) m = std::max(m, std::fabs(lam[b * mcap + k]));
+    return m;
+  }();
+  const double ctol = 1e-7 * std::max(tnorm_proxy, 1e-300);
+  for (int64_t b = 0; b < B; ++b) {
+    const int64_t kt = kb[b], M = Mv[b];
+    if (kt == 0) continue;
+    const cd* C1b = c1.data() + (size_t)b * mcap * Kp;
+    const cd* C2b = c2.data() + (size_t)b * mcap * Kp;
+    double* lb = lam.data() + b * mcap;
+    const double* nb = nrm.data() + b * mcap;
+    std::vector<double> newl(lb, lb + kt);
+    int64_t c0 = 0;
+    while (c0 < kt) {
+      int64_t c1e = c0 + 1;
+      while (c1e < kt && lb[c1e] - lb[c1e - 1] <= ctol) ++c1e;
+      // extend over the full cluster in the whole spectrum (members beyond the targets
+      // are not refined, but they are excluded from the coupling sums)
+      int64_t cend = c1e;
+      while (cend < M && lb[cend] - lb[cend - 1] <= ctol) ++cend;
+      const int kc = (int)(c1e - c0);
+      std::vector<cd> Hm((size_t)kc * kc), Gm((size_t)kc * kc);
+      for (int i = 0; i < kc; ++i)
+        for (int j = 0; j < kc; ++j) {
+          Hm[i + (size_t)j * kc] = C1b[(c0 + i) + (c0 + j) * mcap];
+          Gm[i + (size_t)j * kc] = C2b[(c0 + i) + (c0 + j) * mcap];
+        }
+      std::vector<double> th;
+      std::vector<cd> U;
+      small_gen_eig(kc, Hm, Gm, th, U);
+      for (int m = 0; m < kc; ++m) {
+        double corr = 0.0;
+        for (int64_t j = 0; j < M; ++j) {
+          if (j >= c0 && j < cend) continue;
+          cd sjm = 0.0;
+          for (int c = 0; c < kc; ++c)
+            sjm += (C1b[j + (c0 + c) * mcap] - th[m] * C2b[j + (c0 + c) * mcap]) * U[c + (size_t)m * kc];
+          const double dl = lb[j] - th[m];
+          if (std::fabs(dl) > 0.0) corr -= std::norm(sjm) / (dl * nb[j]);
+        }
+        newl[c0 + m] = th[m] + corr;
+      }
+      c0 = c1e;
+    }
+    std::sort(newl.begin(), newl.end());
+    PPH_HIP(hipMemcpyAsync(lam_dev + b * mcap, newl.data(), kt * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  PPH_HIP(hipStreamSynchronize(s));
 }
 
 // A: B matrices (n x n, working precision, destroyed); D64: FP64 matrices for
@@ -291,6 +505,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
       launch_colnorm2<double>(h.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, h.den.as<double>(),
                               mcap, s);
       launch_rq_finish(h.num.as<double>(), h.den.as<double>(), te.w, B, mmax, mcap, s);
+      refine_low_modes(h, B, D64, h.Z64.as<cplx<double>>(), te.w, h.den.as<double>(), R.M, mcap, s);
       h.tm.end(PH_REFINE, s);
       R.Z64 = h.Z64.as<cplx<double>>();
     } else {

@@ -53,14 +53,29 @@ def vel_cluster_sums(v, groups):
     return np.array([v[g].sum(axis=0) for g in groups])
 
 
-def vel_noise(dD_list, om_cm, n, kappa=20.0):
-    """Absolute noise level of the analytic velocity of each mode in FP64:
-    eigenvectors carry errors ~ n eps |D|, so <e|dD|e> is known to about
-    n eps max|dD|, then scaled by the velocity factor / max(2|omega|, 1e-3)."""
-    scale = max(np.abs(d).max() for d in dD_list)
+def vel_noise(dD_list, om_cm, lam, groups, kappa=4.0):
+    """Conditioning of the analytic velocity of each mode in FP64.  A backward
+    stable eigensolver returns e with errors ~ eps |D| / gap along the other
+    eigenvectors (gap = distance to the nearest eigenvalue outside the mode's
+    cluster), so <e|dD|e> is determined to ~ 2 eps |D| |dD| / gap; scaled by
+    the velocity factor / max(2|omega|, 1e-3)."""
+    lam = np.asarray(lam)
+    n = len(lam)
+    dnorm = max(np.linalg.norm(d, 2) for d in dD_list)
+    Dnorm = np.abs(lam).max()
+    gap = np.full(n, np.inf)
+    for g in groups:
+        lo, hi = g[0], g[-1]
+        d = np.inf
+        if lo > 0:
+            d = min(d, lam[lo] - lam[lo - 1])
+        if hi < n - 1:
+            d = min(d, lam[hi + 1] - lam[hi])
+        gap[g] = d
+    gap = np.maximum(gap, np.finfo(float).eps * Dnorm)
     den = np.maximum(2.0 * np.abs(om_cm), 1e-3)
     const = O.C2 * O.C1SQ32 * 100.0 * 2.0 * O.PI32
-    return kappa * n * np.finfo(float).eps * scale * const / den
+    return kappa * 2.0 * n * np.finfo(float).eps * Dnorm * dnorm / gap * const / den
 
 
 def assert_vel_close(v, v_ref, groups, noise, rel=1e-6, floor=1e-9):

@@ -131,7 +131,7 @@ def test_run_tbg_3_4_path(gpu_available, prec):
         if prec == "D":
             groups = U.clusters(ref["lam"], 1e-6)
             dD = [S.D(q, a) for a in (1, 2, 3)]
-            noise = U.vel_noise(dD, ref["eval_cm"], S.n)
+            noise = U.vel_noise(dD, ref["eval_cm"], ref["lam"], groups)
             U.assert_vel_close(res["vel"][iq], ref["vel"], groups, noise)
             vp = U.vel_projected(res["evec"][iq], dD, ref["eval_cm"], groups)
             vp_ref = U.vel_projected(ref["evec"], dD, ref["eval_cm"], groups)
