@@ -135,7 +135,8 @@ struct Handle {
   DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, cnt;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
-  int nb_panel = 32;
+  int nb_panel = 32;       // reflectors per reduction panel
+  int nb_back = 128;       // reflectors per back-transformation block
   int nchunk = 64;
   ~Handle() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -244,6 +245,10 @@ static void small_gen_eig(int k, std::vector<std::complex<double>> H, std::vecto
 }
 
 // ---------------------------------------------------------------- eigen pipeline
+// Householder vectors are stored explicitly with kVpad zero columns after the
+// last one, so reflector blocks of any width up to kVpad read in bounds.
+constexpr int64_t kVpad = 128;
+
 struct EigResult {
   std::vector<int64_t> M;     // per matrix
   std::vector<int32_t> info;
@@ -386,25 +391,24 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   R.mcap = mcap;
 
   // ---- tridiagonalisation
-  h.V.ensure((size_t)B * n * (n + 64) * sizeof(cplx<T>));
+  h.V.ensure((size_t)B * n * (n + kVpad) * sizeof(cplx<T>));
   h.W.ensure((size_t)B * n * nb * sizeof(cplx<T>));
   h.tau.ensure((size_t)B * n * sizeof(cplx<T>));
   h.dgl.ensure((size_t)B * n * sizeof(T));
   h.egl.ensure((size_t)B * n * sizeof(T));
-  h.ypart.ensure((size_t)B * h.nchunk * n * sizeof(cplx<T>));
-  const int64_t nrb_max = ceil_div(n, 64) + 1;
-  h.dpart.ensure((size_t)B * nrb_max * 2 * nb * sizeof(cplx<T>));
+  h.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
+  h.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
   h.scal.ensure((size_t)B * 256 * sizeof(double));
   h.cnt.ensure((size_t)B * 4 * sizeof(unsigned));
   TridiagWork<T> tw{};
   tw.n = n; tw.lda = n; tw.batch = B; tw.nb = nb;
   tw.A = A; tw.strideA = n * n;
-  tw.V = h.V.as<cplx<T>>(); tw.strideV = n * (n + 64);
+  tw.V = h.V.as<cplx<T>>(); tw.strideV = n * (n + kVpad);
   tw.W = h.W.as<cplx<T>>(); tw.strideW = n * nb;
   tw.tau = h.tau.as<cplx<T>>(); tw.strideTau = n;
   tw.d = h.dgl.as<T>(); tw.e = h.egl.as<T>(); tw.strideDE = n;
-  tw.ypart = h.ypart.as<cplx<T>>(); tw.strideY = (int64_t)h.nchunk * n;
-  tw.dpart = h.dpart.as<cplx<T>>(); tw.strideDp = nrb_max * 2 * nb;
+  tw.ypart = h.ypart.as<cplx<T>>(); tw.strideY = n;
+  tw.dpart = h.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
   tw.scal = h.scal.as<double>(); tw.strideScal = 256;
   tw.cnt = h.cnt.as<unsigned>(); tw.strideCnt = 4;
   tw.nchunk = h.nchunk;
@@ -460,20 +464,19 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
     tri_vectors(te, s);
     h.tm.end(PH_TRIEIG, s);
 
-    // ---- back-transformation
+    // ---- back-transformation, in blocks of nbt reflectors
+    const int64_t nbt = h.nb_back;
     h.Z.ensure((size_t)B * n * mcap * sizeof(cplx<T>));
-    const int64_t strideWs = (n + nb) * nb;
+    const int64_t strideWs = (n + nbt) * nbt;
     h.Tm.ensure((size_t)B * strideWs * sizeof(cplx<T>));
-    h.VT.ensure((size_t)B * std::max<int64_t>(strideWs, n * nb) * sizeof(cplx<T>));
-    h.Wt.ensure((size_t)B * std::max<int64_t>(strideWs, nb * mcap) * sizeof(cplx<T>));
     BackWork<T> bw{};
-    bw.n = n; bw.batch = B; bw.nb = nb; bw.m = mmax;
+    bw.n = n; bw.batch = B; bw.nb = nbt; bw.m = mmax;
     bw.V = tw.V; bw.strideV = tw.strideV;
     bw.tau = tw.tau; bw.strideTau = n;
     bw.zt = te.zt; bw.ldzt = ldzt; bw.strideZt = te.strideZt;
     bw.Z = h.Z.as<cplx<T>>(); bw.ldz = n; bw.strideZ = n * mcap;
     bw.Tm = h.Tm.as<cplx<T>>(); bw.VT = h.VT.as<cplx<T>>(); bw.Wt = h.Wt.as<cplx<T>>();
-    bw.strideWs = std::max<int64_t>(strideWs, std::max<int64_t>(n * nb, nb * mcap));
+    bw.strideWs = std::max<int64_t>(strideWs, std::max<int64_t>(n * nbt, nbt * mcap));
     // Tm/VT/Wt were sized with the same per-matrix stride
     h.Tm.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     h.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));

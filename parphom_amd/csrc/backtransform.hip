@@ -32,9 +32,9 @@ __global__ void __launch_bounds__(256) k_zt_to_z(const double* __restrict__ zt, 
 }
 
 // T_p from G_p = V_p^H V_p: T(k,k) = tau_k, T(0:k, k) = -tau_k T(0:k, 0:k) G(0:k, k).
-// One workgroup (64 threads) per (panel, matrix); entries beyond the panel width are 0.
+// One workgroup (nb threads) per (block, matrix); entries beyond the block width are 0.
 template <typename T>
-__global__ void __launch_bounds__(64) k_form_t(const cplx<T>* __restrict__ G, cplx<T>* __restrict__ Tm,
+__global__ void __launch_bounds__(256) k_form_t(const cplx<T>* __restrict__ G, cplx<T>* __restrict__ Tm,
                                                const cplx<T>* __restrict__ tau, int64_t strideTau,
                                                int64_t n, int64_t nb, int64_t npanel,
                                                int64_t strideWs) {
@@ -88,7 +88,8 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
     g.batch = B; g.binner = npanel;
     launch_gemm<T>(g, s);
   }
-  hipLaunchKernelGGL(k_form_t<T>, dim3((unsigned)npanel, (unsigned)B), dim3(64), 0, s, G, w.Tm,
+  if (nb > 256) throw std::runtime_error("back_transform: block width must be <= 256");
+  hipLaunchKernelGGL(k_form_t<T>, dim3((unsigned)npanel, (unsigned)B), dim3((unsigned)nb), 0, s, G, w.Tm,
                      w.tau, w.strideTau, n, nb, npanel, w.strideWs);
   PPH_CHECK_LAUNCH();
   for (int64_t p = npanel - 1; p >= 0; --p) {

@@ -9,7 +9,7 @@
 namespace pph {
 
 // ---------------------------------------------------------------- complex
-template <typename T> struct cplx { T x, y; };
+template <typename T> struct alignas(2 * sizeof(T)) cplx { T x, y; };
 template <typename T> struct vec2;
 template <> struct vec2<double> { using type = double2; };
 template <> struct vec2<float> { using type = float2; };

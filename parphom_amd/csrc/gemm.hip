@@ -77,6 +77,13 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
     }
 
   cplx<T> ra[4], rb[4];
+  // Operand tiles are loaded unconditionally from a clamped address and then
+  // masked, so all eight loads of a stage are in flight together.
+  auto ld = [](const cplx<T>* p, int64_t idx, bool ok, bool conj) {
+    const cplx<T> t = p[ok ? idx : 0];
+    const T zr = T(0);
+    return cplx<T>{ok ? t.x : zr, ok ? (conj ? -t.y : t.y) : zr};
+  };
   auto load_tiles = [&](int64_t kg) {
     const bool second = two && (kg >= K1);
     const cplx<T>* A = second ? A2 : A1;
@@ -85,31 +92,27 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
     const int64_t Kseg = second ? K - K1 : K1;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      cplx<T> v{0, 0};
       if (!CONJA) {
         const int r = tid % 64, kk = tid / 64 + 4 * i;
         const int64_t gr = row0 + r, gk = k0 + kk;
-        if (gr < M && gk < Kseg) v = A[gr + gk * p.lda];
+        ra[i] = ld(A, gr + gk * p.lda, gr < M && gk < Kseg, false);
       } else {
         const int kk = tid % 16, r = tid / 16 + 16 * i;
         const int64_t gr = row0 + r, gk = k0 + kk;
-        if (gr < M && gk < Kseg) v = cconj(A[gk + gr * p.lda]);
+        ra[i] = ld(A, gk + gr * p.lda, gr < M && gk < Kseg, true);
       }
-      ra[i] = v;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      cplx<T> v{0, 0};
       if (!CONJB) {
         const int kk = tid % 16, c = tid / 16 + 16 * i;
         const int64_t gk = k0 + kk, gc = col0 + c;
-        if (gk < Kseg && gc < N) v = B[gk + gc * p.ldb];
+        rb[i] = ld(B, gk + gc * p.ldb, gk < Kseg && gc < N, false);
       } else {
         const int c = tid % 64, kk = tid / 64 + 4 * i;
         const int64_t gk = k0 + kk, gc = col0 + c;
-        if (gk < Kseg && gc < N) v = cconj(B[gc + gk * p.ldb]);
+        rb[i] = ld(B, gc + gk * p.ldb, gk < Kseg && gc < N, true);
       }
-      rb[i] = v;
     }
   };
   auto store_tiles = [&]() {

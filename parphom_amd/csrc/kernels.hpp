@@ -66,8 +66,8 @@ struct TridiagWork {
   cplx<T>* W;  int64_t strideW;       // n x nb panel W
   cplx<T>* tau; int64_t strideTau;    // n
   T* d; T* e; int64_t strideDE;       // diagonal / off-diagonal (n each)
-  cplx<T>* ypart; int64_t strideY;    // hemv partials: nchunk x n
-  cplx<T>* dpart; int64_t strideDp;   // correction-dot partials: nrb x 2 nb
+  cplx<T>* ypart; int64_t strideY;    // y = A v accumulator: n
+  cplx<T>* dpart; int64_t strideDp;   // correction dots W^H v, V^H v: 2 nb
   double* scal; int64_t strideScal;   // per-column scalars (norm acc, alpha2, scale...)
   unsigned* cnt; int64_t strideCnt;   // last-block counters
   int nchunk;                          // hemv column chunks

@@ -7,10 +7,11 @@
 // A -= V W^H + W V^H on the matrix cores.  Per column three kernels run:
 //   k_col   : correct column j with the panel so far, |x|^2 partial sums; the
 //             last workgroup forms the reflector (beta, tau, 1/(alpha-beta)).
-//   k_hemv  : y partials = A(j+1:, j+1:) v over a 2-D grid of row blocks x
-//             column chunks, plus partial dots W^H v, V^H v; writes v into V.
-//   k_fin   : y = sum of partials - V (W^H v) - W (V^H v), w = tau y, and the
-//             last workgroup forms alpha2 = -tau/2 (w^H v).
+//   k_hemv  : y = A(j+1:, j+1:) v over a 2-D grid of row blocks x column
+//             chunks, accumulated with device atomics, plus the dots W^H v,
+//             V^H v; writes v into V.
+//   k_fin   : y - V (W^H v) - W (V^H v), w = tau y, and the last workgroup
+//             forms alpha2 = -tau/2 (w^H v).
 // The final w = tau y + alpha2 v is never written separately: every consumer
 // reads W_eff(:, k) = W(:, k) + alpha2_k V(:, k), which avoids a fourth pass.
 // Scalars accumulate in FP64 for both FP32 and FP64 matrices.
@@ -35,7 +36,7 @@ __device__ __forceinline__ cplx<T> weff(const TridiagWork<T>& w, int64_t b, int6
 
 // ---------------------------------------------------------------- k_col
 template <typename T>
-__global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
+__global__ void __launch_bounds__(64) k_col(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
   __shared__ double sh[8];
   __shared__ cplx<T> vj[64], wj[64];     // row j of V and W_eff for k < jj
   const int64_t b = blockIdx.z;
@@ -52,9 +53,14 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   __syncthreads();
 
   const int64_t r = j + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // clear the accumulators k_hemv adds into (y over rows j+1.., dots 2 nb)
+  if (r < n) w.ypart[b * w.strideY + r] = cplx<T>{0, 0};
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < 2 * (int)w.nb; k += blockDim.x) w.dpart[b * w.strideDp + k] = cplx<T>{0, 0};
   double part = 0.0;
   if (r < n) {
     cplx<T> a = A[r + j * w.lda];
+#pragma unroll 4
     for (int k = 0; k < jj; ++k) {
       const cplx<T> vr = V[r + (p + k) * n];
       const cplx<T> wr = weff(w, b, r, k, p, scal);
@@ -146,7 +152,9 @@ __global__ void __launch_bounds__(256) k_hemv(TridiagWork<T> w, int64_t j, int j
     s = cadd(s, red[1][tr]);
     s = cadd(s, red[2][tr]);
     s = cadd(s, red[3][tr]);
-    w.ypart[b * w.strideY + blockIdx.y * n + rr] = s;
+    cplx<T>* y = w.ypart + b * w.strideY + (j + 1 + rr);
+    atomicAdd(&y->x, s.x);
+    atomicAdd(&y->y, s.y);
   }
   if (blockIdx.y == 0) {
     // v for this row block; store into V and form correction-dot partials
@@ -165,15 +173,18 @@ __global__ void __launch_bounds__(256) k_hemv(TridiagWork<T> w, int64_t j, int j
       }
       t.x = wave_sum(t.x);
       t.y = wave_sum(t.y);
-      if (tr == 0) w.dpart[b * w.strideDp + blockIdx.x * (2 * w.nb) + k] = t;
+      if (tr == 0) {
+        cplx<T>* d = w.dpart + b * w.strideDp + k;
+        atomicAdd(&d->x, t.x);
+        atomicAdd(&d->y, t.y);
+      }
     }
   }
 }
 
 // ---------------------------------------------------------------- k_fin
 template <typename T>
-__global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj, int64_t p,
-                                             int nchunk, int nrb) {
+__global__ void __launch_bounds__(64) k_fin(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
   __shared__ cplx<T> sW[64], sV[64];
   __shared__ double sh[8];
   const int64_t b = blockIdx.z;
@@ -182,10 +193,8 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   const cplx<T>* V = w.V + b * w.strideV;
   const cplx<T> tau{(T)scal[S_TAU], (T)scal[S_TAU + 1]};
 
-  if ((int)threadIdx.x < 2 * jj) {
-    const int k = threadIdx.x;
-    cplx<T> s{0, 0};
-    for (int rb = 0; rb < nrb; ++rb) s = cadd(s, w.dpart[b * w.strideDp + rb * (2 * w.nb) + k]);
+  for (int k = threadIdx.x; k < 2 * jj; k += blockDim.x) {
+    const cplx<T> s = w.dpart[b * w.strideDp + k];
     if (k < jj) sW[k] = s; else sV[k - jj] = s;
   }
   __syncthreads();
@@ -194,8 +203,8 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   double pr = 0, pi = 0;
   if (rr < m) {
     const int64_t r = j + 1 + rr;
-    cplx<T> y{0, 0};
-    for (int c = 0; c < nchunk; ++c) y = cadd(y, w.ypart[b * w.strideY + c * n + rr]);
+    cplx<T> y = w.ypart[b * w.strideY + r];
+#pragma unroll 4
     for (int k = 0; k < jj; ++k) {
       y = csub(y, cmul(V[r + (p + k) * n], sW[k]));
       y = csub(y, cmul(weff(w, b, r, k, p, scal), sV[k]));
@@ -274,7 +283,7 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
     for (int jj = 0; jj < pw; ++jj) {
       const int64_t j = p + jj;
       const int64_t m = n - j - 1;
-      hipLaunchKernelGGL(k_col<T>, dim3((unsigned)ceil_div(n - j, 256), 1, B), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_col<T>, dim3((unsigned)ceil_div(n - j, 64), 1, B), dim3(64), 0, s,
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
       const int64_t nrb = ceil_div(m, 64);
@@ -288,8 +297,8 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
       hipLaunchKernelGGL(k_hemv<T>, dim3((unsigned)nrb, (unsigned)nch, B), dim3(256), shm, s,
                          w, j, jj, p, chunk);
       PPH_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 256), 1, B), dim3(256), 0, s,
-                         w, j, jj, p, (int)nch, (int)nrb);
+      hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 64), 1, B), dim3(64), 0, s,
+                         w, j, jj, p);
       PPH_CHECK_LAUNCH();
     }
     const int64_t r0 = p + pw;
