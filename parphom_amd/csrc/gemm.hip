@@ -51,6 +51,7 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
   z /= ksplit;
   const int64_t bi = z % binner, b = z / binner;
   const int64_t row0 = (int64_t)blockIdx.x * GBM, col0 = (int64_t)blockIdx.y * GBN;
+  if (p.lower && col0 > row0 + GBM - 1) return;
   const int64_t offA = b * p.strideA + bi * p.sAin, offB = b * p.strideB + bi * p.sBin;
   const cplx<T>* A1 = p.A + offA;
   const cplx<T>* B1 = p.B + offB;

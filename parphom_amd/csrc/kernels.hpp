@@ -48,6 +48,8 @@ struct GemmParams {
   // split-K: the K range is cut into ksplit pieces whose products are added
   // into C atomically (requires beta = 0 and C cleared beforehand)
   int64_t ksplit;
+  // lower != 0: tiles lying entirely above the diagonal of C are skipped
+  int lower;
   // diagonal-dot epilogue: when dot_out != null, C is not written; instead
   // dot_out[b*dot_stride + col] += sum_rows Re(conj(X[row, col]) * (op(A) op(B))[row, col])
   // with X = dotX (leading dimension lddx, stride strideX).

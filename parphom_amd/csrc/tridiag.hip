@@ -12,6 +12,8 @@
 //             V^H v; writes v into V.
 //   k_fin   : y - V (W^H v) - W (V^H v), w = tau y, and the last workgroup
 //             forms alpha2 = -tau/2 (w^H v).
+// The trailing matrix is kept correct in its lower triangle only (the hemv
+// reads the lower triangle and uses it twice).
 // The final w = tau y + alpha2 v is never written separately: every consumer
 // reads W_eff(:, k) = W(:, k) + alpha2_k V(:, k), which avoids a fourth pass.
 // Scalars accumulate in FP64 for both FP32 and FP64 matrices.
@@ -108,72 +110,110 @@ __global__ void __launch_bounds__(64) k_col(TridiagWork<T> w, int64_t j, int jj,
   }
 }
 
-// ---------------------------------------------------------------- k_hemv
-// grid: x = row blocks of 64 rows over [j+1, n), y = column chunks, z = batch.
+// ---------------------------------------------------------------- k_hemv_sym
+// y = S v for the Hermitian trailing matrix S = A(j+1:, j+1:), reading only
+// its lower triangle: each 64 x 64 tile S_IJ (J <= I) is staged in LDS once
+// and used twice, y_I += S_IJ v_J and y_J += S_IJ^H v_I.  grid: x = tile row
+// I, y = segment of `seg` tiles along the row, z = batch.  Results go to the
+// accumulator with device atomics.  Blocks of segment 0 also form the
+// correction dots W^H v, V^H v for their rows and write v into V.
 template <typename T>
-__global__ void __launch_bounds__(256) k_hemv(TridiagWork<T> w, int64_t j, int jj, int64_t p,
-                                              int64_t chunk) {
-  extern __shared__ char smem_raw[];
-  cplx<T>* vs = reinterpret_cast<cplx<T>*>(smem_raw);          // chunk entries
+__global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
+                                                  int seg) {
+  __shared__ cplx<T> tile[64][65];        // tile[c][r] = S(I*64 + r, J*64 + c)
+  __shared__ cplx<T> vI[64], vJ[64];
   __shared__ cplx<T> red[4][64];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
+  const int I = blockIdx.x;
+  const int J0 = blockIdx.y * seg;
+  if (J0 > I) return;
+  const int J1 = min(I + 1, J0 + seg);
   const double* scal = w.scal + b * w.strideScal;
   const cplx<T>* A = w.A + b * w.strideA;
+  const cplx<T>* S = A + (j + 1) + (j + 1) * w.lda;
   cplx<T>* V = w.V + b * w.strideV;
+  cplx<T>* y = w.ypart + b * w.strideY + (j + 1);
   const cplx<T> sc{(T)scal[S_SCALE], (T)scal[S_SCALE + 1]};
-
-  const int64_t c0 = blockIdx.y * chunk;                         // offsets within [0, m)
-  const int64_t c1 = min(m, c0 + chunk);
-  for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-    const int64_t r = j + 1 + c;
-    vs[c - c0] = (c == 0) ? cplx<T>{1, 0} : cmul(A[r + j * w.lda], sc);
-  }
-  __syncthreads();
-
-  const int tr = threadIdx.x % 64, cg = threadIdx.x / 64;
-  const int64_t rr = blockIdx.x * 64 + tr;                        // offset within [0, m)
-  cplx<T> acc{0, 0};
-  if (rr < m) {
-    const cplx<T>* Ar = A + (j + 1 + rr) + (j + 1) * w.lda;
-    int64_t c = c0 + cg;
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  auto vval = [&](int64_t o) -> cplx<T> {
+    if (o >= m) return cplx<T>{0, 0};
+    return (o == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + o) + j * w.lda], sc);
+  };
+  if (tid < 64) vI[tid] = vval((int64_t)I * 64 + tid);
+  const int64_t rI = (int64_t)I * 64 + lane;          // this lane's row within S
+  cplx<T> rowacc{0, 0};
+  // register double buffer: the next tile's 16 elements per thread are in
+  // flight while the current tile is used from LDS
+  cplx<T> nxt[16];
+  auto load_tile = [&](int J) {
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+      const int64_t col = (int64_t)J * 64 + wv * 16 + cc;
+      const bool ok = (rI < m) && (col < m);
+      const cplx<T> a = S[(ok ? rI : 0) + (ok ? col : 0) * w.lda];
+      nxt[cc] = ok ? a : cplx<T>{0, 0};
+    }
+  };
+  load_tile(J0);
+  for (int J = J0; J < J1; ++J) {
+    __syncthreads();
+    if (tid < 64) vJ[tid] = vval((int64_t)J * 64 + tid);
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) tile[wv * 16 + cc][lane] = nxt[cc];
+    __syncthreads();
+    if (J + 1 < J1) load_tile(J + 1);
+    const bool diag = (J == I);
+    // row sums: row = lane, columns of this wave
 #pragma unroll 4
-    for (; c < c1; c += 4) {
-      const cplx<T> a = Ar[c * w.lda];
-      const cplx<T> v = vs[c - c0];
-      acc.x += a.x * v.x - a.y * v.y;
-      acc.y += a.x * v.y + a.y * v.x;
+    for (int cc = 0; cc < 16; ++cc) {
+      const int c = wv * 16 + cc;
+      if (diag && c > lane) continue;
+      cplx<T> a = tile[c][lane];
+      if (diag && c == lane) a.y = 0;
+      rowacc = cadd(rowacc, cmul(a, vJ[c]));
+    }
+    // column sums: column = lane, rows of this wave (strictly below the diagonal)
+    cplx<T> colacc{0, 0};
+#pragma unroll 4
+    for (int rr = 0; rr < 16; ++rr) {
+      const int r = wv * 16 + rr;
+      if (diag && r <= lane) continue;
+      colacc = cadd(colacc, cmulc(tile[lane][r], vI[r]));
+    }
+    red[wv][lane] = colacc;
+    __syncthreads();
+    if (wv == 0) {
+      cplx<T> s = cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane]));
+      const int64_t cJ = (int64_t)J * 64 + lane;
+      if (cJ < m) {
+        atomicAdd(&y[cJ].x, s.x);
+        atomicAdd(&y[cJ].y, s.y);
+      }
     }
   }
-  red[cg][tr] = acc;
   __syncthreads();
-  if (cg == 0 && rr < m) {
-    cplx<T> s = red[0][tr];
-    s = cadd(s, red[1][tr]);
-    s = cadd(s, red[2][tr]);
-    s = cadd(s, red[3][tr]);
-    cplx<T>* y = w.ypart + b * w.strideY + (j + 1 + rr);
-    atomicAdd(&y->x, s.x);
-    atomicAdd(&y->y, s.y);
+  red[wv][lane] = rowacc;
+  __syncthreads();
+  if (wv == 0 && rI < m) {
+    const cplx<T> s = cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane]));
+    atomicAdd(&y[rI].x, s.x);
+    atomicAdd(&y[rI].y, s.y);
   }
   if (blockIdx.y == 0) {
-    // v for this row block; store into V and form correction-dot partials
-    cplx<T> vr{0, 0};
-    if (rr < m) {
-      vr = (rr == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + rr) + j * w.lda], sc);
-      if (cg == 0) V[(j + 1 + rr) + j * n] = vr;
-    }
-    // 2*jj dots over 64 rows; wave cg handles k = cg, cg + 4, ...
-    for (int k = cg; k < 2 * jj; k += 4) {
+    // v for this tile row: store into V and form the correction dots
+    const cplx<T> vr = (rI < m) ? vI[lane] : cplx<T>{0, 0};
+    if (wv == 0 && rI < m) V[(j + 1 + rI) + j * n] = vr;
+    for (int k = wv; k < 2 * jj; k += 4) {
       cplx<T> t{0, 0};
-      if (rr < m) {
-        const cplx<T> x = (k < jj) ? weff(w, b, j + 1 + rr, k, p, scal)
-                                   : V[(j + 1 + rr) + (p + k - jj) * n];
+      if (rI < m) {
+        const cplx<T> x = (k < jj) ? weff(w, b, j + 1 + rI, k, p, scal)
+                                   : V[(j + 1 + rI) + (p + k - jj) * n];
         t = cmulc(x, vr);
       }
       t.x = wave_sum(t.x);
       t.y = wave_sum(t.y);
-      if (tr == 0) {
+      if (lane == 0) {
         cplx<T>* d = w.dpart + b * w.strideDp + k;
         atomicAdd(&d->x, t.x);
         atomicAdd(&d->y, t.y);
@@ -286,16 +326,10 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
       hipLaunchKernelGGL(k_col<T>, dim3((unsigned)ceil_div(n - j, 64), 1, B), dim3(64), 0, s,
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
-      const int64_t nrb = ceil_div(m, 64);
-      int64_t nch = std::max<int64_t>(1, std::min<int64_t>((int64_t)w.nchunk,
-                                                           ceil_div(2048, nrb * (int64_t)B)));
-      nch = std::min<int64_t>(nch, ceil_div(m, 64));
-      nch = std::max<int64_t>(nch, ceil_div(m, 2048));        // chunk of v in LDS <= 2048
-      const int64_t chunk = ceil_div(m, nch);
-      nch = ceil_div(m, chunk);
-      const size_t shm = (size_t)chunk * sizeof(cplx<T>);
-      hipLaunchKernelGGL(k_hemv<T>, dim3((unsigned)nrb, (unsigned)nch, B), dim3(256), shm, s,
-                         w, j, jj, p, chunk);
+      const int64_t nt = ceil_div(m, 64);
+      const int seg = 12;
+      hipLaunchKernelGGL(k_hemv_sym<T>, dim3((unsigned)nt, (unsigned)ceil_div(nt, seg), B), dim3(256),
+                         0, s, w, j, jj, p, seg);
       PPH_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 64), 1, B), dim3(64), 0, s,
                          w, j, jj, p);
@@ -319,6 +353,7 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
     g.B2 = w.V + r0 + p * n; g.strideB2 = w.strideV;
     g.C = w.A + r0 + r0 * w.lda; g.ldc = w.lda; g.strideC = w.strideA;
     g.batch = w.batch;
+    g.lower = 1;              // only the lower triangle of the trailing matrix is read later
     launch_gemm<T>(g, s);
   }
   hipLaunchKernelGGL(k_last_diag<T>, dim3((unsigned)ceil_div(w.batch, 64)), dim3(64), 0, s, w);
