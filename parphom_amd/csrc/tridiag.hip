@@ -37,15 +37,18 @@ __device__ __forceinline__ cplx<T> weff(const TridiagWork<T>& w, int64_t b, int6
 }
 
 // ---------------------------------------------------------------- k_col
+// 256 threads = 64 rows x 4 waves; wave s handles the panel terms k = s, s+4, ...
 template <typename T>
-__global__ void __launch_bounds__(64) k_col(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
+__global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
   __shared__ double sh[8];
   __shared__ cplx<T> vj[64], wj[64];     // row j of V and W_eff for k < jj
+  __shared__ cplx<T> red[4][64];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n;
   double* scal = w.scal + b * w.strideScal;
   cplx<T>* A = w.A + b * w.strideA;
   const cplx<T>* V = w.V + b * w.strideV;
+  const int lane = threadIdx.x % 64, sub = threadIdx.x / 64;
 
   if ((int)threadIdx.x < jj) {
     const int k = threadIdx.x;
@@ -54,22 +57,25 @@ __global__ void __launch_bounds__(64) k_col(TridiagWork<T> w, int64_t j, int jj,
   }
   __syncthreads();
 
-  const int64_t r = j + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  // clear the accumulators k_hemv adds into (y over rows j+1.., dots 2 nb)
-  if (r < n) w.ypart[b * w.strideY + r] = cplx<T>{0, 0};
+  const int64_t r = j + blockIdx.x * (int64_t)64 + lane;
+  // clear the accumulators k_hemv_sym adds into (y over rows j+1.., dots 2 nb)
+  if (sub == 0 && r < n) w.ypart[b * w.strideY + r] = cplx<T>{0, 0};
   if (blockIdx.x == 0)
     for (int k = threadIdx.x; k < 2 * (int)w.nb; k += blockDim.x) w.dpart[b * w.strideDp + k] = cplx<T>{0, 0};
-  double part = 0.0;
+  cplx<T> acc{0, 0};
   if (r < n) {
-    cplx<T> a = A[r + j * w.lda];
-#pragma unroll 4
-    for (int k = 0; k < jj; ++k) {
-      const cplx<T> vr = V[r + (p + k) * n];
-      const cplx<T> wr = weff(w, b, r, k, p, scal);
-      // a -= V(r,k) conj(W(j,k)) + W(r,k) conj(V(j,k))
-      a = csub(a, cmul(vr, cconj(wj[k])));
-      a = csub(a, cmul(wr, cconj(vj[k])));
+    for (int k = sub; k < jj; k += 4) {
+      // V(r,k) conj(W(j,k)) + W(r,k) conj(V(j,k))
+      acc = cadd(acc, cmul(V[r + (p + k) * n], cconj(wj[k])));
+      acc = cadd(acc, cmul(weff(w, b, r, k, p, scal), cconj(vj[k])));
     }
+  }
+  red[sub][lane] = acc;
+  __syncthreads();
+  double part = 0.0;
+  if (sub == 0 && r < n) {
+    cplx<T> a = A[r + j * w.lda];
+    a = csub(a, cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane])));
     if (r == j) a.y = 0;
     A[r + j * w.lda] = a;
     if (r >= j + 2) part = (double)cabs2(a);
@@ -110,18 +116,13 @@ __global__ void __launch_bounds__(64) k_col(TridiagWork<T> w, int64_t j, int jj,
   }
 }
 
-// ---------------------------------------------------------------- k_hemv_sym
-// y = S v for the Hermitian trailing matrix S = A(j+1:, j+1:), reading only
-// its lower triangle: each 64 x 64 tile S_IJ (J <= I) is staged in LDS once
-// and used twice, y_I += S_IJ v_J and y_J += S_IJ^H v_I.  grid: x = tile row
-// I, y = segment of `seg` tiles along the row, z = batch.  Results go to the
-// accumulator with device atomics.  Blocks of segment 0 also form the
-// correction dots W^H v, V^H v for their rows and write v into V.
+constexpr int kHemvSeg = 12;      // tiles per workgroup along a tile row
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
                                                   int seg) {
   __shared__ cplx<T> tile[64][65];        // tile[c][r] = S(I*64 + r, J*64 + c)
-  __shared__ cplx<T> vI[64], vJ[64];
+  __shared__ cplx<T> vI[64], vseg[kHemvSeg * 64];
   __shared__ cplx<T> red[4][64];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
@@ -141,6 +142,7 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     return (o == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + o) + j * w.lda], sc);
   };
   if (tid < 64) vI[tid] = vval((int64_t)I * 64 + tid);
+  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = vval((int64_t)J0 * 64 + o);
   const int64_t rI = (int64_t)I * 64 + lane;          // this lane's row within S
   cplx<T> rowacc{0, 0};
   // register double buffer: the next tile's 16 elements per thread are in
@@ -158,7 +160,7 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
   load_tile(J0);
   for (int J = J0; J < J1; ++J) {
     __syncthreads();
-    if (tid < 64) vJ[tid] = vval((int64_t)J * 64 + tid);
+    const cplx<T>* vJ = vseg + (J - J0) * 64;
 #pragma unroll
     for (int cc = 0; cc < 16; ++cc) tile[wv * 16 + cc][lane] = nxt[cc];
     __syncthreads();
@@ -223,15 +225,18 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
 }
 
 // ---------------------------------------------------------------- k_fin
+// 256 threads = 64 rows x 4 waves; wave s handles the panel terms k = s, s+4, ...
 template <typename T>
-__global__ void __launch_bounds__(64) k_fin(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
+__global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
   __shared__ cplx<T> sW[64], sV[64];
+  __shared__ cplx<T> red[4][64];
   __shared__ double sh[8];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
   double* scal = w.scal + b * w.strideScal;
   const cplx<T>* V = w.V + b * w.strideV;
   const cplx<T> tau{(T)scal[S_TAU], (T)scal[S_TAU + 1]};
+  const int lane = threadIdx.x % 64, sub = threadIdx.x / 64;
 
   for (int k = threadIdx.x; k < 2 * jj; k += blockDim.x) {
     const cplx<T> s = w.dpart[b * w.strideDp + k];
@@ -239,16 +244,21 @@ __global__ void __launch_bounds__(64) k_fin(TridiagWork<T> w, int64_t j, int jj,
   }
   __syncthreads();
 
-  const int64_t rr = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  double pr = 0, pi = 0;
+  const int64_t rr = blockIdx.x * (int64_t)64 + lane;
+  const int64_t r = j + 1 + rr;
+  cplx<T> acc{0, 0};
   if (rr < m) {
-    const int64_t r = j + 1 + rr;
-    cplx<T> y = w.ypart[b * w.strideY + r];
-#pragma unroll 4
-    for (int k = 0; k < jj; ++k) {
-      y = csub(y, cmul(V[r + (p + k) * n], sW[k]));
-      y = csub(y, cmul(weff(w, b, r, k, p, scal), sV[k]));
+    for (int k = sub; k < jj; k += 4) {
+      acc = cadd(acc, cmul(V[r + (p + k) * n], sW[k]));
+      acc = cadd(acc, cmul(weff(w, b, r, k, p, scal), sV[k]));
     }
+  }
+  red[sub][lane] = acc;
+  __syncthreads();
+  double pr = 0, pi = 0;
+  if (sub == 0 && rr < m) {
+    cplx<T> y = w.ypart[b * w.strideY + r];
+    y = csub(y, cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane])));
     y = cmul(tau, y);
     w.W[b * w.strideW + r + (int64_t)jj * n] = y;
     const cplx<T> v = V[r + j * n];
@@ -323,15 +333,15 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
     for (int jj = 0; jj < pw; ++jj) {
       const int64_t j = p + jj;
       const int64_t m = n - j - 1;
-      hipLaunchKernelGGL(k_col<T>, dim3((unsigned)ceil_div(n - j, 64), 1, B), dim3(64), 0, s,
+      hipLaunchKernelGGL(k_col<T>, dim3((unsigned)ceil_div(n - j, 64), 1, B), dim3(256), 0, s,
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
       const int64_t nt = ceil_div(m, 64);
-      const int seg = 12;
+      const int seg = kHemvSeg;
       hipLaunchKernelGGL(k_hemv_sym<T>, dim3((unsigned)nt, (unsigned)ceil_div(nt, seg), B), dim3(256),
                          0, s, w, j, jj, p, seg);
       PPH_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 64), 1, B), dim3(64), 0, s,
+      hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 64), 1, B), dim3(256), 0, s,
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
     }
