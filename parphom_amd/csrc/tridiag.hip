@@ -79,12 +79,13 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
     if (r == j) a.y = 0;
     A[r + j * w.lda] = a;
     if (r >= j + 2) part = (double)cabs2(a);
+    // the last workgroup reads A(j,j) and A(j+1,j); everything else is read by
+    // the next kernel, so only these two writers publish at device scope
+    if (r <= j + 1) __threadfence();
   }
-  __threadfence();
   part = block_sum(part, sh);
   if (threadIdx.x == 0) {
     atomic_add_relaxed(&scal[S_NORM], part);
-    __threadfence();
     const unsigned done = atomic_inc_acqrel(&w.cnt[b * w.strideCnt + 0]);
     if (done == gridDim.x - 1) {
       __threadfence();
@@ -265,13 +266,12 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
     const cplx<T> t = cmulc(y, v);
     pr = t.x; pi = t.y;
   }
-  __threadfence();
+  // the last workgroup only needs the atomically accumulated w^H v
   pr = block_sum(pr, sh);
   pi = block_sum(pi, sh);
   if (threadIdx.x == 0) {
     atomic_add_relaxed(&scal[S_YV], pr);
     atomic_add_relaxed(&scal[S_YV + 1], pi);
-    __threadfence();
     const unsigned done = atomic_inc_acqrel(&w.cnt[b * w.strideCnt + 1]);
     if (done == gridDim.x - 1) {
       __threadfence();

@@ -1,0 +1,161 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+symbol the headers declare (no GPU calls), and the input readers follow
+read_input.F90 / lammps_read.F90 / q_read.F90."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from parphom_amd import formats, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in ("parphom_hip.h", "parphom_hip_debug.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"\b(pph_[a-z_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from parphom_amd import lib
+    L = lib.load()
+    names = _declared()
+    assert {"pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
+            "pph_destroy", "pph_last_error", "pph_version"} <= names
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(lib.EXPORTED) >= names
+    assert L.pph_version().decode().startswith("parphom_hip")
+
+
+def test_argument_errors_are_return_codes():
+    from parphom_amd import lib
+    L = lib.load()
+    h = ctypes.c_void_p()
+    rc = L.pph_create(None, None, None, ctypes.byref(h))
+    assert rc == -1 and b"NULL" in L.pph_last_error(None)
+    o = lib.make_opts(rng="I", il=5, iu=2)
+    w = np.zeros(4)
+    m = np.zeros(1, np.int64)
+    info = np.zeros(1, np.int32)
+    A = np.zeros((2, 2), np.complex128)
+    rc = L.pph_heevx(2, 1, A.ctypes.data, ctypes.byref(o), lib._ptr(w), None,
+                     lib._ptr(m, ctypes.c_int64), lib._ptr(info, ctypes.c_int32), 0)
+    assert rc == -1 and b"il <= iu" in L.pph_last_error(None)
+
+
+INP = """# comment line
+! another comment
+lammps file location         : ./data
+LAMMPS_FILE_NAME             : cell.dat   # trailing comment
+natom                        : 28
+atom types                   : 1
+atom style                   : molecular
+q file location              : ./
+q file name                  : q.dat
+nqpt                         : 3
+force constant file location : fc
+force constant file name     : FORCE_CONSTANTS
+force constant dataset       : force_constants
+compute eigvecs              : YES
+range                        : I
+min index                    : 13
+max index                    : 16
+group velocity               : yes
+velocity method              : cd
+abstol                       : 1E-14
+orfac                        : 1d-14
+num qpools                   : 3
+output file name             : out
+output_file_location         : ./
+print progress               : no
+"""
+
+
+def test_read_input_keys_defaults_and_sanitise(tmp_path):
+    p = tmp_path / "inp.dat"
+    p.write_text(INP)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        cfg = formats.read_input(str(p))
+    finally:
+        os.chdir(cwd)
+    assert cfg.lammps_location == "./data/" and cfg.lammps_name == "cell.dat"
+    assert cfg.natom == 28 and cfg.atom_style == "molecular"
+    assert cfg.fc_location == "fc/" and cfg.fc_dataset == "force_constants"
+    assert cfg.evec_comp and cfg.jobz == "V"
+    assert cfg.range_ == "I" and (cfg.il, cfg.iu) == (13, 16) and cfg.vl == cfg.vu == 0.0
+    assert cfg.comp_vel and cfg.vel_method == "C"
+    assert cfg.abstol == 1e-14 and cfg.orfac == 1e-14
+    assert cfg.output_name == "out.hdf5" and not cfg.print_progress
+    assert any("NUM QPOOLS" in m and "not recognised" in m for m in cfg.messages)
+    # mb = nb = min(32, natom) / nprocs + 1
+    assert cfg.mb == cfg.nb == 29
+
+
+def test_read_input_defaults_and_quirks(tmp_path):
+    p = tmp_path / "inp.dat"
+    p.write_text("natom : 10\nrange : a\ngroup velocity : yes\ncompute eigvecs : maybe\n")
+    cfg = formats.read_input(str(p))
+    # lower-case range is not recognised and leaves 'A'; il/iu become 1..natom
+    assert cfg.range_ == "A" and (cfg.il, cfg.iu) == (1, 10)
+    # velocities force JOBZ='V' but not evec_comp (read_input.F90:483-485)
+    assert cfg.jobz == "V" and not cfg.evec_comp
+    assert cfg.abstol == float(np.float32(1e-15))
+    assert any("Could not interpret" in m for m in cfg.messages)
+
+
+def test_sanitise_errors_and_output_rename(tmp_path):
+    p = tmp_path / "inp.dat"
+    p.write_text("range : I\nmin index : 5\nmax index : 2\n")
+    with pytest.raises(formats.InputError):
+        formats.read_input(str(p))
+    p.write_text("range : V\nmin eigval : 5\nmax eigval : 2\n")
+    with pytest.raises(formats.InputError):
+        formats.read_input(str(p))
+    (tmp_path / "res.hdf5").write_text("x")
+    (tmp_path / "res_v1.hdf5").write_text("x")
+    p.write_text(f"output file name : res\noutput file location : {tmp_path}\n")
+    cfg = formats.read_input(str(p))
+    assert cfg.output_name == "res_v2.hdf5"
+
+
+@pytest.mark.parametrize("style", ["atomic", "molecular", "full"])
+def test_lammps_roundtrip(tmp_path, style):
+    st = synth.tbg_structure(1, 2)
+    f = tmp_path / "cell.dat"
+    synth.write_lammps_data(str(f), st, style)
+    ld = formats.read_lammps_data(str(f), st.natom, 1, style)
+    assert np.allclose(ld.lat, st.lat, atol=1e-14)
+    assert np.allclose(ld.pos, st.pos, atol=1e-14)
+    assert np.array_equal(ld.types, st.types) and ld.masses[0] == st.masses[0]
+    with pytest.raises(formats.InputError):
+        formats.read_lammps_data(str(f), st.natom, 1, "charge")
+
+
+def test_q_file_extra_columns_and_trailing_lines(tmp_path):
+    f = tmp_path / "q.dat"
+    f.write_text("0.1 0.2 0.0 0.5\n\n0.3 0.4 0.0 0.5\n7\n")
+    q = formats.read_q_file(str(f), 2)
+    assert np.allclose(q, [[0.1, 0.2, 0.0], [0.3, 0.4, 0.0]])
+    with pytest.raises(formats.InputError):
+        formats.read_q_file(str(f), 5)
+
+
+def test_synthetic_structures():
+    for (m, n) in [(1, 2), (3, 4), (5, 6)]:
+        st = synth.tbg_structure(m, n)
+        assert st.natom == 4 * (m * m + m * n + n * n)
+        # no two atoms closer than the C-C bond within a layer
+        from scipy.spatial.distance import pdist
+        assert pdist(st.pos).min() > 1.3
+    assert abs(synth.tbg_structure(3, 4).theta_deg - 9.43) < 0.01
+    assert abs(synth.tbg_structure(11, 12).theta_deg - 2.88) < 0.01
+    q = synth.q_path_gkmg()
+    assert q.shape == (61, 3) and np.allclose(q[26], synth.K_POINT)
