@@ -98,15 +98,27 @@ __global__ void __launch_bounds__(256) k_tri_prepare(TriEigWork w) {
   }
 }
 
-// one thread per (matrix, selected eigenvalue)
-__global__ void __launch_bounds__(256) k_tri_bisect(TriEigWork w) {
+// one thread per (matrix, selected eigenvalue); d and e^2 are staged in LDS
+// when the tridiagonal fits (use_lds), so the Sturm recurrences read LDS.
+__global__ void __launch_bounds__(256) k_tri_bisect(TriEigWork w, int use_lds) {
+  extern __shared__ double tsh[];
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.y;
   const int64_t lo_i = w.ilo[b], hi_i = w.ihi[b];
-  if (k >= hi_i - lo_i) return;
-  const int64_t kk = lo_i + k;
+  if ((int64_t)blockIdx.x * blockDim.x >= hi_i - lo_i) return;     // whole block idle
   const double* d = w.dd + b * w.strideT;
   const double* e2 = w.e2 + b * w.strideT;
+  if (use_lds) {
+    for (int64_t i = threadIdx.x; i < w.n; i += blockDim.x) {
+      tsh[i] = d[i];
+      tsh[w.n + i] = e2[i];
+    }
+    __syncthreads();
+    d = tsh;
+    e2 = tsh + w.n;
+  }
+  if (k >= hi_i - lo_i) return;
+  const int64_t kk = lo_i + k;
   const double* ti = w.tinfo + b * 4;
   const double pivmin = ti[3];
   double lo = ti[0], hi = ti[1];
@@ -208,14 +220,25 @@ __device__ double perturbed_shift(const double* __restrict__ wv, int64_t k, doub
   return x;
 }
 
-__global__ void __launch_bounds__(256) k_tri_inviter(TriEigWork w) {
+__global__ void __launch_bounds__(256) k_tri_inviter(TriEigWork w, int use_lds) {
+  extern __shared__ double tsh[];
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.y;
   const int64_t M = w.ihi[b] - w.ilo[b];
-  if (k >= M) return;
+  if ((int64_t)blockIdx.x * blockDim.x >= M) return;               // whole block idle
   const int64_t n = w.n, ld = w.ldzt;
   const double* d = w.dd + b * w.strideT;
   const double* e = w.ee + b * w.strideT;
+  if (use_lds) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      tsh[i] = d[i];
+      tsh[n + i] = e[i];
+    }
+    __syncthreads();
+    d = tsh;
+    e = tsh + n;
+  }
+  if (k >= M) return;
   const double tnorm = w.tinfo[b * 4 + 2];
   const double* wv = w.w + b * w.ldw;
   double* y = w.zt + b * w.strideZt;
@@ -295,15 +318,28 @@ void tri_prepare(const TriEigWork& w, hipStream_t s) {
   PPH_CHECK_LAUNCH();
 }
 
+// tridiagonals up to this size are staged in LDS (2 n doubles)
+constexpr int64_t kTriLdsMax = 9216;
+
 void tri_bisect(const TriEigWork& w, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(w.mmax, 64), (unsigned)w.batch);
-  hipLaunchKernelGGL(k_tri_bisect, grid, dim3(64), 0, s, w);
+  const int use_lds = w.n <= kTriLdsMax ? 1 : 0;
+  const size_t shm = use_lds ? (size_t)2 * w.n * sizeof(double) : 0;
+  dim3 grid((unsigned)ceil_div(w.mmax, 256), (unsigned)w.batch);
+  if (use_lds)
+    PPH_HIP(hipFuncSetAttribute((const void*)k_tri_bisect,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  hipLaunchKernelGGL(k_tri_bisect, grid, dim3(256), shm, s, w, use_lds);
   PPH_CHECK_LAUNCH();
 }
 
 void tri_vectors(const TriEigWork& w, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(w.mmax, 64), (unsigned)w.batch);
-  hipLaunchKernelGGL(k_tri_inviter, grid, dim3(64), 0, s, w);
+  const int use_lds = w.n <= kTriLdsMax ? 1 : 0;
+  const size_t shm = use_lds ? (size_t)2 * w.n * sizeof(double) : 0;
+  dim3 grid((unsigned)ceil_div(w.mmax, 256), (unsigned)w.batch);
+  if (use_lds)
+    PPH_HIP(hipFuncSetAttribute((const void*)k_tri_inviter,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  hipLaunchKernelGGL(k_tri_inviter, grid, dim3(256), shm, s, w, use_lds);
   PPH_CHECK_LAUNCH();
   const double ctol_rel = fmax(w.orfac, 1e-9);
   hipLaunchKernelGGL(k_tri_cluster, dim3((unsigned)w.mmax, (unsigned)w.batch), dim3(256), 0, s, w,

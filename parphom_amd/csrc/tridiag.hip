@@ -117,13 +117,26 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   }
 }
 
+// ---------------------------------------------------------------- k_hemv_sym
+// y = S v for the Hermitian trailing matrix S = A(j+1:, j+1:), reading only
+// its lower triangle once: each element of a 64 x 64 tile S_IJ (J <= I)
+// contributes S_IJ v_J to y_I and S_IJ^H v_I to y_J.  grid: x = tile row I,
+// y = segment of `seg` tiles along the row, z = batch.  Lane = row of the tile
+// row, wave w = columns 16w..16w+15 of every tile; the next tile's elements
+// are loaded while the current one is used.  Row sums stay in registers; the
+// 16 column sums of a wave are reduced across lanes by a butterfly that halves
+// the number of live columns at each step.  Results go to the accumulator
+// with device atomics.  Blocks of segment 0 also form the correction dots
+// W^H v, V^H v for their rows and write v into V.
 constexpr int kHemvSeg = 12;      // tiles per workgroup along a tile row
+
+template <typename T>
+__device__ __forceinline__ T xsel(bool h, T a, T b) { return h ? a : b; }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
                                                   int seg) {
-  __shared__ cplx<T> tile[64][65];        // tile[c][r] = S(I*64 + r, J*64 + c)
-  __shared__ cplx<T> vI[64], vseg[kHemvSeg * 64];
+  __shared__ cplx<T> vseg[kHemvSeg * 64];
   __shared__ cplx<T> red[4][64];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
@@ -142,60 +155,76 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     if (o >= m) return cplx<T>{0, 0};
     return (o == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + o) + j * w.lda], sc);
   };
-  if (tid < 64) vI[tid] = vval((int64_t)I * 64 + tid);
-  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = vval((int64_t)J0 * 64 + o);
   const int64_t rI = (int64_t)I * 64 + lane;          // this lane's row within S
-  cplx<T> rowacc{0, 0};
-  // register double buffer: the next tile's 16 elements per thread are in
-  // flight while the current tile is used from LDS
-  cplx<T> nxt[16];
-  auto load_tile = [&](int J) {
+  const cplx<T> vI = vval(rI);
+  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = vval((int64_t)J0 * 64 + o);
+
+  cplx<T> cur[16], nxt[16];
+  auto load_tile = [&](int J, cplx<T>(&dst)[16]) {
 #pragma unroll
     for (int cc = 0; cc < 16; ++cc) {
       const int64_t col = (int64_t)J * 64 + wv * 16 + cc;
       const bool ok = (rI < m) && (col < m);
       const cplx<T> a = S[(ok ? rI : 0) + (ok ? col : 0) * w.lda];
-      nxt[cc] = ok ? a : cplx<T>{0, 0};
+      dst[cc] = ok ? a : cplx<T>{0, 0};
     }
   };
-  load_tile(J0);
+  load_tile(J0, nxt);
+  __syncthreads();                                   // vseg ready
+  cplx<T> rowacc{0, 0};
+  const bool h5 = (lane >> 5) & 1, h4 = (lane >> 4) & 1, h3 = (lane >> 3) & 1, h2 = (lane >> 2) & 1;
+  const int mycol = wv * 16 + (h5 ? 8 : 0) + (h4 ? 4 : 0) + (h3 ? 2 : 0) + (h2 ? 1 : 0);
   for (int J = J0; J < J1; ++J) {
-    __syncthreads();
-    const cplx<T>* vJ = vseg + (J - J0) * 64;
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) tile[wv * 16 + cc][lane] = nxt[cc];
-    __syncthreads();
-    if (J + 1 < J1) load_tile(J + 1);
+    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+    if (J + 1 < J1) load_tile(J + 1, nxt);
     const bool diag = (J == I);
-    // row sums: row = lane, columns of this wave
-#pragma unroll 4
+    const cplx<T>* vJ = vseg + (J - J0) * 64 + wv * 16;
+    T pr[16], pi[16];
+#pragma unroll
     for (int cc = 0; cc < 16; ++cc) {
       const int c = wv * 16 + cc;
-      if (diag && c > lane) continue;
-      cplx<T> a = tile[c][lane];
-      if (diag && c == lane) a.y = 0;
-      rowacc = cadd(rowacc, cmul(a, vJ[c]));
+      const cplx<T> a = cur[cc];
+      // row sum: lower triangle including the (real) diagonal
+      cplx<T> ar = a;
+      if (diag && c == lane) ar.y = 0;
+      if (!(diag && c > lane)) rowacc = cadd(rowacc, cmul(ar, vJ[cc]));
+      // column sum: strictly lower elements, conj(S(r, c)) v(r)
+      const cplx<T> q = (diag && lane <= c) ? cplx<T>{0, 0} : cmulc(a, vI);
+      pr[cc] = q.x;
+      pi[cc] = q.y;
     }
-    // column sums: column = lane, rows of this wave (strictly below the diagonal)
-    cplx<T> colacc{0, 0};
-#pragma unroll 4
-    for (int rr = 0; rr < 16; ++rr) {
-      const int r = wv * 16 + rr;
-      if (diag && r <= lane) continue;
-      colacc = cadd(colacc, cmulc(tile[lane][r], vI[r]));
+    // butterfly reduce-scatter of the 16 column sums over the 64 lanes
+    T r8[8], i8[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      r8[k] = xsel(h5, pr[8 + k], pr[k]) + __shfl_xor(xsel(h5, pr[k], pr[8 + k]), 32, WAVE);
+      i8[k] = xsel(h5, pi[8 + k], pi[k]) + __shfl_xor(xsel(h5, pi[k], pi[8 + k]), 32, WAVE);
     }
-    red[wv][lane] = colacc;
-    __syncthreads();
-    if (wv == 0) {
-      cplx<T> s = cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane]));
-      const int64_t cJ = (int64_t)J * 64 + lane;
-      if (cJ < m) {
-        atomicAdd(&y[cJ].x, s.x);
-        atomicAdd(&y[cJ].y, s.y);
-      }
+    T r4[4], i4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      r4[k] = xsel(h4, r8[4 + k], r8[k]) + __shfl_xor(xsel(h4, r8[k], r8[4 + k]), 16, WAVE);
+      i4[k] = xsel(h4, i8[4 + k], i8[k]) + __shfl_xor(xsel(h4, i8[k], i8[4 + k]), 16, WAVE);
+    }
+    T r2[2], i2[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      r2[k] = xsel(h3, r4[2 + k], r4[k]) + __shfl_xor(xsel(h3, r4[k], r4[2 + k]), 8, WAVE);
+      i2[k] = xsel(h3, i4[2 + k], i4[k]) + __shfl_xor(xsel(h3, i4[k], i4[2 + k]), 8, WAVE);
+    }
+    T r1 = xsel(h2, r2[1], r2[0]) + __shfl_xor(xsel(h2, r2[0], r2[1]), 4, WAVE);
+    T i1 = xsel(h2, i2[1], i2[0]) + __shfl_xor(xsel(h2, i2[0], i2[1]), 4, WAVE);
+    r1 += __shfl_xor(r1, 2, WAVE);
+    i1 += __shfl_xor(i1, 2, WAVE);
+    r1 += __shfl_xor(r1, 1, WAVE);
+    i1 += __shfl_xor(i1, 1, WAVE);
+    const int64_t cJ = (int64_t)J * 64 + mycol;
+    if ((lane & 3) == 0 && cJ < m) {
+      atomicAdd(&y[cJ].x, r1);
+      atomicAdd(&y[cJ].y, i1);
     }
   }
-  __syncthreads();
   red[wv][lane] = rowacc;
   __syncthreads();
   if (wv == 0 && rI < m) {
@@ -205,7 +234,7 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
   }
   if (blockIdx.y == 0) {
     // v for this tile row: store into V and form the correction dots
-    const cplx<T> vr = (rI < m) ? vI[lane] : cplx<T>{0, 0};
+    const cplx<T> vr = (rI < m) ? vI : cplx<T>{0, 0};
     if (wv == 0 && rI < m) V[(j + 1 + rI) + j * n] = vr;
     for (int k = wv; k < 2 * jj; k += 4) {
       cplx<T> t{0, 0};
