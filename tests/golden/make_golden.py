@@ -9,6 +9,7 @@ container, where the reference checkout is readable):
   phbands_layout.json   group/dataset names, shapes and dtypes of the shipped output
                         Examples/tblg_21.8/phbands_21.8.hdf5 (layout only: its values
                         were not produced from the shipped force constants)
+  tblg_21.8_FORCE_CONSTANTS.h5  a byte copy of that HDF5 file (reader test)
   tblg_21.8_oracle.npz  FP64 oracle frequencies at Gamma, K, M from the shipped inputs
   tbg_1_2_oracle.npz    synthetic (1,2) cell: oracle frequencies, velocities and D(q)
                         at a few q, for regression checks
@@ -19,6 +20,7 @@ Usage:  python tests/golden/make_golden.py
 """
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -57,6 +59,9 @@ np.savez('@OUT@', fc=d[()], chunks=np.array(d.chunks is None))
     qsym = formats.read_q_file(os.path.join(EX, "q_pt_symm.dat"), 3)
     np.savez_compressed(os.path.join(HERE, "tblg_21.8_input.npz"), lat=ld.lat, pos=ld.pos,
                         types=ld.types, masses=ld.masses, fc=fc, q_symm=qsym)
+    # the shipped HDF5 file itself (58 KB), to test the HDF5 reader on a file
+    # the reference's producer wrote
+    shutil.copyfile(FCF, os.path.join(HERE, "tblg_21.8_FORCE_CONSTANTS.h5"))
 
     lay = h5_dump(f"""
 import h5py, numpy as np, json
