@@ -112,6 +112,17 @@ int pph_run_device(pph_handle* h, int64_t nq, const double* q, int64_t ldq, pph_
 int pph_assemble(pph_handle* h, int64_t nq, const double* q, int64_t ldq, int32_t what, void* out,
                  int32_t on_device);
 
+/* Expectation values diag(Z^H X Z) for nq q-points, with X = D(q) (what 0)
+ * or dD/dq_x,y,z (what 1..3) assembled from this handle's force constants:
+ * out[iq*nz + k] = Re(z_k^H X(q) z_k), Z complex128 n x nz per q at
+ * Z + iq*n*zcols (column-major).  With a handle built on the derivative of
+ * the force constants with respect to a model parameter this is the
+ * Hellmann-Feynman derivative d lambda_k / d parameter; with what = 1..3 it
+ * is <e|dD/dq|e> of write_output.F90:415-421.  Pointers are host or device
+ * per `on_device`. */
+int pph_expect(pph_handle* h, int64_t nq, const double* q, int64_t ldq, int32_t what,
+               const void* Z, int64_t nz, int64_t zcols, double* out, int32_t on_device);
+
 /* The eigensolver alone (pzheevx semantics, diagonalize.F90:73-96) on nmat
  * Hermitian n x n complex128 matrices (column-major, leading dimension n,
  * consecutive in memory; destroyed).  Precision, jobz, range, vl, vu

@@ -862,6 +862,69 @@ int pph_assemble(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32
   PPH_API_CATCH(h)
 }
 
+int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t what,
+               const void* Z, int64_t nz, int64_t zcols, double* out, int32_t on_device) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_expect: handle is NULL", PPH_ERR_ARG);
+  if (what < 0 || what > 3) return fail(h, "pph_expect: what must be 0..3", PPH_ERR_ARG);
+  if (nq < 0 || nz < 0 || zcols < nz || (nq > 0 && (!q || !Z || !out || ldq < nq)))
+    return fail(h, "pph_expect: bad arguments", PPH_ERR_ARG);
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(h->device));
+  hipStream_t s = h->stream;
+  const int64_t n = h->n;
+  const int64_t B = std::max<int64_t>(1, std::min<int64_t>(nq, 4));
+  DevBuf zb, ob;
+  std::vector<double> qrows;
+  for (int64_t q0 = 0; q0 < nq; q0 += B) {
+    const int64_t nb = std::min(B, nq - q0);
+    qrows.resize(3 * nb);
+    for (int64_t b = 0; b < nb; ++b)
+      for (int c = 0; c < 3; ++c) qrows[3 * b + c] = q[(q0 + b) + ldq * c];
+    h->qdev.ensure((size_t)nb * 3 * sizeof(double));
+    PPH_HIP(hipMemcpyAsync(h->qdev.p, qrows.data(), nb * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    AsmParams ap{};
+    ap.phi = h->phi.as<double>(); ap.crys = h->crys.as<double>(); ap.lat = h->lat.as<double>();
+    ap.rmass = h->rmass.as<double>(); ap.mask = h->mask.as<uint16_t>(); ap.q = h->qdev.as<double>();
+    ap.cd_dc = h->cd_dc.as<double>(); ap.cd_norm = h->cd_norm;
+    ap.N = h->N; ap.nb = nb; ap.mode = (h->opts.vel_method == 'C') ? 1 : 0;
+    h->A64.ensure((size_t)nb * n * n * sizeof(cplx<double>));
+    ap.D64 = h->A64.as<cplx<double>>();
+    if (what > 0) {
+      h->dD.ensure((size_t)nb * 3 * n * n * sizeof(cplx<double>));
+      ap.dD64 = h->dD.as<cplx<double>>();
+    }
+    launch_assemble(ap, s);
+    const cplx<double>* Zq = reinterpret_cast<const cplx<double>*>(Z) + (size_t)q0 * n * zcols;
+    if (!on_device) {
+      zb.ensure((size_t)nb * n * zcols * sizeof(cplx<double>));
+      PPH_HIP(hipMemcpyAsync(zb.p, Zq, (size_t)nb * n * zcols * 16, hipMemcpyHostToDevice, s));
+      Zq = zb.as<cplx<double>>();
+    }
+    double* od = on_device ? out + q0 * nz : nullptr;
+    if (!on_device) {
+      ob.ensure((size_t)nb * nz * sizeof(double));
+      od = ob.as<double>();
+    }
+    launch_fill_zero(od, nb * nz * sizeof(double), s);
+    GemmParams<double> g{};
+    g.opA = 'N'; g.opB = 'N'; g.m = n; g.n = nz; g.k = n;
+    g.alpha = cplx<double>{1, 0};
+    g.A = (what == 0) ? h->A64.as<cplx<double>>() : h->dD.as<cplx<double>>() + (size_t)(what - 1) * n * n;
+    g.lda = n; g.strideA = (what == 0) ? n * n : 3 * n * n;
+    g.B = Zq; g.ldb = n; g.strideB = n * zcols;
+    g.batch = nb;
+    g.dot_out = od; g.dot_stride = nz;
+    g.dotX = Zq; g.lddx = n; g.strideX = n * zcols;
+    launch_gemm<double>(g, s);
+    if (!on_device)
+      PPH_HIP(hipMemcpyAsync(out + q0 * nz, od, nb * nz * sizeof(double), hipMemcpyDeviceToHost, s));
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  return PPH_OK;
+  PPH_API_CATCH(h)
+}
+
 int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w, void* Z,
               int64_t* m, int32_t* info, int32_t on_device) {
   std::string why;
