@@ -37,7 +37,7 @@ class PphOut(C.Structure):
                 ("evec", C.c_void_p), ("evec_cols", C.c_int64), ("vel", _dp), ("info", _i32p)]
 
 
-EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
+EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
             "pph_destroy", "pph_last_error", "pph_version", "pph_debug_zgemm", "pph_profile")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total")
@@ -65,6 +65,9 @@ def load(path: str = LIB_PATH):
     lib.pph_last_error.argtypes = [C.c_void_p]
     lib.pph_last_error.restype = C.c_char_p
     lib.pph_version.restype = C.c_char_p
+    lib.pph_expect.argtypes = [C.c_void_p, C.c_int64, _dp, C.c_int64, C.c_int32, C.c_void_p,
+                               C.c_int64, C.c_int64, _dp, C.c_int32]
+    lib.pph_expect.restype = C.c_int
     lib.pph_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _i64p]
     lib.pph_profile.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
