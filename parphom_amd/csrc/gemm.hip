@@ -2,8 +2,11 @@
 // "diagonal dot" epilogue used by the group-velocity and Rayleigh-quotient
 // products (sum_rows Re(conj(X) .* (A B))) so the product is never stored.
 //
-// A complex product is four real MFMAs per 16x16x4 step:
-//   Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br.
+// A complex product takes three real MFMAs per 16x16x4 step (the 3M scheme):
+// P1 += Ar Br, P2 += Ai Bi, P3 += (Ar + Ai)(Br + Bi), Re = P1 - P2,
+// Im = P3 - P1 - P2.  The kernel is held to two waves per SIMD so that the FP64
+// instance keeps its three accumulator sets in VGPRs without losing occupancy
+// (the four-MFMA form, P1 = Re and P2 = Im, stays selectable through k3M).
 // Tile: 64x64 per 256-thread workgroup (2x2 waves, each 32x32 = 2x2 MFMA
 // blocks), K staged through LDS in steps of 16.
 #include "common.hpp"
@@ -36,7 +39,7 @@ template <> struct mfma<float> {
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
 template <typename T, bool CONJA, bool CONJB, bool DOT>
-__global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_zgemm(GemmParams<T> p) {
   using acc_t = typename mfma<T>::acc;
   __shared__ cplx<T> As[GBK][GBM];
   __shared__ cplx<T> Bs[GBK][GBN];
@@ -68,13 +71,18 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
   }
   const int64_t K1 = two ? p.k1 : K;
 
-  acc_t accR[2][2], accI[2][2];
+  constexpr bool k3M = true;
+  auto product = [](T p1, T p2, T p3) {
+    return k3M ? cplx<T>{p1 - p2, p3 - p1 - p2} : cplx<T>{p1, p2};
+  };
+  acc_t acc1[2][2], acc2[2][2], acc3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      accR[i][j] = acc_t{0, 0, 0, 0};
-      accI[i][j] = acc_t{0, 0, 0, 0};
+      acc1[i][j] = acc_t{0, 0, 0, 0};
+      acc2[i][j] = acc_t{0, 0, 0, 0};
+      acc3[i][j] = acc_t{0, 0, 0, 0};
     }
 
   cplx<T> ra[4], rb[4];
@@ -140,14 +148,27 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
       for (int mi = 0; mi < 2; ++mi) a[mi] = As[kk][wr0 + mi * 16 + lane % 16];
 #pragma unroll
       for (int ni = 0; ni < 2; ++ni) bb[ni] = Bs[kk][wc0 + ni * 16 + lane % 16];
+      T as_[2] = {}, bs_[2] = {};
+      if constexpr (k3M) {
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi) as_[mi] = a[mi].x + a[mi].y;
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) bs_[ni] = bb[ni].x + bb[ni].y;
+      }
 #pragma unroll
       for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
-          accR[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].x, accR[mi][ni]);
-          accR[mi][ni] = mfma<T>::op(-a[mi].y, bb[ni].y, accR[mi][ni]);
-          accI[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].y, accI[mi][ni]);
-          accI[mi][ni] = mfma<T>::op(a[mi].y, bb[ni].x, accI[mi][ni]);
+          if constexpr (k3M) {
+            acc1[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].x, acc1[mi][ni]);
+            acc2[mi][ni] = mfma<T>::op(a[mi].y, bb[ni].y, acc2[mi][ni]);
+            acc3[mi][ni] = mfma<T>::op(as_[mi], bs_[ni], acc3[mi][ni]);
+          } else {
+            acc1[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].x, acc1[mi][ni]);
+            acc1[mi][ni] = mfma<T>::op(-a[mi].y, bb[ni].y, acc1[mi][ni]);
+            acc2[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].y, acc2[mi][ni]);
+            acc2[mi][ni] = mfma<T>::op(a[mi].y, bb[ni].x, acc2[mi][ni]);
+          }
         }
     }
   }
@@ -165,7 +186,7 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
         for (int i = 0; i < 4; ++i) {
           const int64_t row = row0 + wr0 + mi * 16 + mfma<T>::row(lane, i);
           if (row >= M) continue;
-          cplx<T> v{accR[mi][ni][i], accI[mi][ni][i]};
+          cplx<T> v = product(acc1[mi][ni][i], acc2[mi][ni][i], acc3[mi][ni][i]);
           v = cmul(p.alpha, v);
           cplx<T>* c = C + row + col * p.ldc;
           if (ksplit > 1) {
@@ -192,8 +213,9 @@ __global__ void __launch_bounds__(256) k_zgemm(GemmParams<T> p) {
             const int64_t row = row0 + wr0 + mi * 16 + mfma<T>::row(lane, i);
             if (row >= M) continue;
             const cplx<T> x = X[row + col * p.lddx];
+            const cplx<T> v = product(acc1[mi][ni][i], acc2[mi][ni][i], acc3[mi][ni][i]);
             // Re(conj(x) * acc)
-            s += (double)x.x * (double)accR[mi][ni][i] + (double)x.y * (double)accI[mi][ni][i];
+            s += (double)x.x * (double)v.x + (double)x.y * (double)v.y;
           }
       }
       s += __shfl_xor(s, 16, WAVE);
