@@ -132,7 +132,7 @@ struct Handle {
   DevBuf phi, crys, pos, lat, rmass, mask, cd_dc;
   double cd_norm = 0.0;
   // batch workspaces
-  DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, cnt;
+  DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
   int nb_panel = 32;       // reflectors per reduction panel
@@ -399,7 +399,8 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   h.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
   h.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
   h.scal.ensure((size_t)B * 256 * sizeof(double));
-  h.cnt.ensure((size_t)B * 4 * sizeof(unsigned));
+  const int64_t nparts = n / 64 + 2;
+  h.part.ensure((size_t)B * 3 * nparts * sizeof(double));
   TridiagWork<T> tw{};
   tw.n = n; tw.lda = n; tw.batch = B; tw.nb = nb;
   tw.A = A; tw.strideA = n * n;
@@ -410,7 +411,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   tw.ypart = h.ypart.as<cplx<T>>(); tw.strideY = n;
   tw.dpart = h.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
   tw.scal = h.scal.as<double>(); tw.strideScal = 256;
-  tw.cnt = h.cnt.as<unsigned>(); tw.strideCnt = 4;
+  tw.part = h.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
   tw.nchunk = h.nchunk;
   h.tm.begin(PH_TRID, s);
   tridiagonalise<T>(tw, s);

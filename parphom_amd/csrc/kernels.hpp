@@ -70,8 +70,9 @@ struct TridiagWork {
   T* d; T* e; int64_t strideDE;       // diagonal / off-diagonal (n each)
   cplx<T>* ypart; int64_t strideY;    // y = A v accumulator: n
   cplx<T>* dpart; int64_t strideDp;   // correction dots W^H v, V^H v: 2 nb
-  double* scal; int64_t strideScal;   // per-column scalars (norm acc, alpha2, scale...)
-  unsigned* cnt; int64_t strideCnt;   // last-block counters
+  double* scal; int64_t strideScal;   // per-column scalars (tau, scale, alpha2 of the panel)
+  double* part; int64_t strideP;      // per-workgroup partials: |x|^2 at [0, n/64 + 1),
+  int64_t yvOff;                      //   w^H v (re, im pairs) from part[yvOff]
   int nchunk;                          // hemv column chunks
 };
 

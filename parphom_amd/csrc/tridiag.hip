@@ -5,13 +5,18 @@
 // matrix is not touched inside a panel; each new column is corrected with the
 // panel's V and W), then the trailing matrix gets one rank-2nb update
 // A -= V W^H + W V^H on the matrix cores.  Per column three kernels run:
-//   k_col   : correct column j with the panel so far, |x|^2 partial sums; the
-//             last workgroup forms the reflector (beta, tau, 1/(alpha-beta)).
-//   k_hemv  : y = A(j+1:, j+1:) v over a 2-D grid of row blocks x column
-//             chunks, accumulated with device atomics, plus the dots W^H v,
-//             V^H v; writes v into V.
-//   k_fin   : y - V (W^H v) - W (V^H v), w = tau y, and the last workgroup
-//             forms alpha2 = -tau/2 (w^H v).
+//   k_col   : alpha2 of the previous column (from k_fin's partial dots),
+//             correct column j with the panel so far, per-workgroup partial
+//             sums of |x|^2.
+//   k_hemv  : every workgroup forms the reflector (beta, tau, 1/(alpha-beta))
+//             from those partials, then y = A(j+1:, j+1:) v over a 2-D grid of
+//             tile rows x column segments, accumulated with device atomics,
+//             plus the dots W^H v, V^H v; writes v into V.
+//   k_fin   : y - V (W^H v) - W (V^H v), w = tau y, per-workgroup partials of
+//             w^H v.
+// Every cross-workgroup reduction ends at a kernel boundary: producers store
+// per-workgroup partials and the next kernel sums them in a fixed order, so no
+// kernel needs device-scope fences and the sums are deterministic.
 // The trailing matrix is kept correct in its lower triangle only (the hemv
 // reads the lower triangle and uses it twice).
 // The final w = tau y + alpha2 v is never written separately: every consumer
@@ -25,15 +30,86 @@
 namespace pph {
 
 // per-matrix scalar slots
-constexpr int S_NORM = 0, S_YV = 1, S_TAU = 3, S_SCALE = 5, S_A2 = 8;
+constexpr int S_TAU = 3, S_SCALE = 5, S_A2 = 8;
+
+// Reflector of column j: H = I - tau v v^H with v = (1, x / (alpha - beta)),
+// beta = -sign(Re alpha) sqrt(|alpha|^2 + |x|^2), tau = (beta - alpha) / beta
+// (the ZLARFG formulas without the rescaling loop).
+struct Reflector { double beta, tr, ti, sr, si; };
+
+__device__ __forceinline__ Reflector make_reflector(double xn2, double ar, double ai) {
+  Reflector r{ar, 0, 0, 0, 0};
+  if (!(xn2 == 0.0 && ai == 0.0)) {
+    const double nrm = sqrt(ar * ar + ai * ai + xn2);
+    r.beta = (ar >= 0.0) ? -nrm : nrm;
+    r.tr = (r.beta - ar) / r.beta;
+    r.ti = -ai / r.beta;
+    // scale = 1 / (alpha - beta)
+    const double dr = ar - r.beta, di = ai;
+    const double den = dr * dr + di * di;
+    r.sr = dr / den;
+    r.si = -di / den;
+  }
+  return r;
+}
+
+// Sum of np per-workgroup partials (stride `st` doubles apart), in a fixed order.
+__device__ __forceinline__ double sum_partials(const double* x, int64_t np, int st, double* sh) {
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += blockDim.x) s += x[i * st];
+  return block_sum(s, sh);
+}
+
+
+template <typename T>
+__device__ __forceinline__ cplx<T> weff_a(const TridiagWork<T>& w, int64_t b, int64_t r, int k,
+                                          int64_t p, cplx<T> a2) {
+  const cplx<T> wv = w.W[b * w.strideW + r + (int64_t)k * w.n];
+  const cplx<T> vv = w.V[b * w.strideV + r + (p + k) * w.n];
+  return cadd(wv, cmul(a2, vv));
+}
 
 template <typename T>
 __device__ __forceinline__ cplx<T> weff(const TridiagWork<T>& w, int64_t b, int64_t r, int k,
                                         int64_t p, const double* scal) {
-  const cplx<T> wv = w.W[b * w.strideW + r + (int64_t)k * w.n];
-  const cplx<T> vv = w.V[b * w.strideV + r + (p + k) * w.n];
-  const cplx<T> a2{(T)scal[S_A2 + 2 * k], (T)scal[S_A2 + 2 * k + 1]};
-  return cadd(wv, cmul(a2, vv));
+  return weff_a(w, b, r, k, p, cplx<T>{(T)scal[S_A2 + 2 * k], (T)scal[S_A2 + 2 * k + 1]});
+}
+
+// alpha2 = -tau/2 (w^H v) for the column whose k_fin ran over m rows: its
+// per-workgroup partials of w^H v sit at part[yv + 2 i], tau at scal[S_TAU].
+__device__ __forceinline__ void alpha2_from_partials(const double* part, int64_t yv, int64_t m,
+                                                     const double* scal, double* sh, double& a2r,
+                                                     double& a2i) {
+  const int64_t np = (m + 63) / 64;
+  const double yr = sum_partials(part + yv, np, 2, sh);
+  const double yi = sum_partials(part + yv + 1, np, 2, sh);
+  const double tr = scal[S_TAU], ti = scal[S_TAU + 1];
+  a2r = -0.5 * (tr * yr - ti * yi);
+  a2i = -0.5 * (tr * yi + ti * yr);
+}
+
+// Row r of the panel terms sum_k X(r, k) a_k + Y(r, k) b_k for k = sub, sub + 4,
+// ..., < jj: the loads of eight terms are issued together from clamped
+// addresses and masked afterwards, so a wave waits for one memory latency per
+// eight terms instead of one per term.
+template <typename T, typename FX, typename FY>
+__device__ __forceinline__ cplx<T> panel_terms(int sub, int jj, FX&& term_x, FY&& term_y) {
+  cplx<T> acc{0, 0};
+  for (int k0 = sub; k0 < jj; k0 += 32) {
+    cplx<T> tx[8], ty[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = k0 + 4 * i;
+      const bool ok = k < jj;
+      const int kc = ok ? k : sub;
+      tx[i] = term_x(kc);
+      ty[i] = term_y(kc);
+      if (!ok) { tx[i] = cplx<T>{0, 0}; ty[i] = cplx<T>{0, 0}; }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc = cadd(acc, cadd(tx[i], ty[i]));
+  }
+  return acc;
 }
 
 // ---------------------------------------------------------------- k_col
@@ -42,18 +118,34 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
   __shared__ double sh[8];
   __shared__ cplx<T> vj[64], wj[64];     // row j of V and W_eff for k < jj
+  __shared__ cplx<T> a2s[64];            // alpha2 of the panel columns k < jj
   __shared__ cplx<T> red[4][64];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n;
   double* scal = w.scal + b * w.strideScal;
+  double* part = w.part + b * w.strideP;
   cplx<T>* A = w.A + b * w.strideA;
   const cplx<T>* V = w.V + b * w.strideV;
   const int lane = threadIdx.x % 64, sub = threadIdx.x / 64;
 
+  if (jj > 0) {
+    // alpha2 of column j - 1, whose k_fin ran over m = n - j rows
+    double a2r, a2i;
+    alpha2_from_partials(part, w.yvOff, n - j, scal, sh, a2r, a2i);
+    if (threadIdx.x == 0) {
+      a2s[jj - 1] = cplx<T>{(T)a2r, (T)a2i};
+      if (blockIdx.x == 0) { scal[S_A2 + 2 * (jj - 1)] = a2r; scal[S_A2 + 2 * (jj - 1) + 1] = a2i; }
+    }
+  }
+  if ((int)threadIdx.x < jj - 1) {
+    const int k = threadIdx.x;
+    a2s[k] = cplx<T>{(T)scal[S_A2 + 2 * k], (T)scal[S_A2 + 2 * k + 1]};
+  }
+  __syncthreads();
   if ((int)threadIdx.x < jj) {
     const int k = threadIdx.x;
     vj[k] = V[j + (p + k) * n];
-    wj[k] = weff(w, b, j, k, p, scal);
+    wj[k] = weff_a(w, b, j, k, p, a2s[k]);
   }
   __syncthreads();
 
@@ -63,58 +155,24 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   if (blockIdx.x == 0)
     for (int k = threadIdx.x; k < 2 * (int)w.nb; k += blockDim.x) w.dpart[b * w.strideDp + k] = cplx<T>{0, 0};
   cplx<T> acc{0, 0};
-  if (r < n) {
-    for (int k = sub; k < jj; k += 4) {
-      // V(r,k) conj(W(j,k)) + W(r,k) conj(V(j,k))
-      acc = cadd(acc, cmul(V[r + (p + k) * n], cconj(wj[k])));
-      acc = cadd(acc, cmul(weff(w, b, r, k, p, scal), cconj(vj[k])));
-    }
+  if (r < n && sub < jj) {
+    // V(r,k) conj(W(j,k)) + W(r,k) conj(V(j,k))
+    acc = panel_terms<T>(
+        sub, jj, [&](int k) { return cmul(V[r + (p + k) * n], cconj(wj[k])); },
+        [&](int k) { return cmul(weff_a(w, b, r, k, p, a2s[k]), cconj(vj[k])); });
   }
   red[sub][lane] = acc;
   __syncthreads();
-  double part = 0.0;
+  double xpart = 0.0;
   if (sub == 0 && r < n) {
     cplx<T> a = A[r + j * w.lda];
     a = csub(a, cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane])));
     if (r == j) a.y = 0;
     A[r + j * w.lda] = a;
-    if (r >= j + 2) part = (double)cabs2(a);
-    // the last workgroup reads A(j,j) and A(j+1,j); everything else is read by
-    // the next kernel, so only these two writers publish at device scope
-    if (r <= j + 1) __threadfence();
+    if (r >= j + 2) xpart = (double)cabs2(a);
   }
-  part = block_sum(part, sh);
-  if (threadIdx.x == 0) {
-    atomic_add_relaxed(&scal[S_NORM], part);
-    const unsigned done = atomic_inc_acqrel(&w.cnt[b * w.strideCnt + 0]);
-    if (done == gridDim.x - 1) {
-      __threadfence();
-      const double xn2 = atomic_load_acq(&scal[S_NORM]);
-      const cplx<T> ajj = A[j + j * w.lda];
-      w.d[b * w.strideDE + j] = ajj.x;
-      double ar = 0, ai = 0;
-      if (j + 1 < n) { const cplx<T> al = A[(j + 1) + j * w.lda]; ar = al.x; ai = al.y; }
-      double tr = 0, ti = 0, sr = 0, si = 0, beta = ar;
-      if (!(xn2 == 0.0 && ai == 0.0)) {
-        const double nrm = sqrt(ar * ar + ai * ai + xn2);
-        beta = (ar >= 0.0) ? -nrm : nrm;
-        tr = (beta - ar) / beta;
-        ti = -ai / beta;
-        // scale = 1 / (alpha - beta)
-        const double dr = ar - beta, di = ai;
-        const double den = dr * dr + di * di;
-        sr = dr / den;
-        si = -di / den;
-      }
-      w.e[b * w.strideDE + j] = (T)beta;
-      w.tau[b * w.strideTau + j] = cplx<T>{(T)tr, (T)ti};
-      scal[S_TAU] = tr; scal[S_TAU + 1] = ti;
-      scal[S_SCALE] = sr; scal[S_SCALE + 1] = si;
-      scal[S_NORM] = 0.0;
-      __threadfence();
-      atomic_store_rel(&w.cnt[b * w.strideCnt + 0], 0u);
-    }
-  }
+  xpart = block_sum(xpart, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = xpart;   // read by k_hemv_sym
 }
 
 // ---------------------------------------------------------------- k_hemv_sym
@@ -144,20 +202,13 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
   const int J0 = blockIdx.y * seg;
   if (J0 > I) return;
   const int J1 = min(I + 1, J0 + seg);
-  const double* scal = w.scal + b * w.strideScal;
+  double* scal = w.scal + b * w.strideScal;
   const cplx<T>* A = w.A + b * w.strideA;
   const cplx<T>* S = A + (j + 1) + (j + 1) * w.lda;
   cplx<T>* V = w.V + b * w.strideV;
   cplx<T>* y = w.ypart + b * w.strideY + (j + 1);
-  const cplx<T> sc{(T)scal[S_SCALE], (T)scal[S_SCALE + 1]};
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
-  auto vval = [&](int64_t o) -> cplx<T> {
-    if (o >= m) return cplx<T>{0, 0};
-    return (o == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + o) + j * w.lda], sc);
-  };
   const int64_t rI = (int64_t)I * 64 + lane;          // this lane's row within S
-  const cplx<T> vI = vval(rI);
-  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = vval((int64_t)J0 * 64 + o);
 
   cplx<T> cur[16], nxt[16];
   auto load_tile = [&](int J, cplx<T>(&dst)[16]) {
@@ -170,6 +221,25 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     }
   };
   load_tile(J0, nxt);
+  // the reflector of column j, from k_col's partial norms (k_col ran over n - j rows)
+  __shared__ double sh[8];
+  const double xn2 = sum_partials(w.part + b * w.strideP, (n - j + 63) / 64, 1, sh);
+  const cplx<T> ajj = A[j + j * w.lda], al = A[(j + 1) + j * w.lda];
+  const Reflector rf = make_reflector(xn2, al.x, al.y);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+    w.d[b * w.strideDE + j] = ajj.x;
+    w.e[b * w.strideDE + j] = (T)rf.beta;
+    w.tau[b * w.strideTau + j] = cplx<T>{(T)rf.tr, (T)rf.ti};
+    scal[S_TAU] = rf.tr; scal[S_TAU + 1] = rf.ti;
+    scal[S_SCALE] = rf.sr; scal[S_SCALE + 1] = rf.si;
+  }
+  const cplx<T> sc{(T)rf.sr, (T)rf.si};
+  auto vval = [&](int64_t o) -> cplx<T> {
+    if (o >= m) return cplx<T>{0, 0};
+    return (o == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + o) + j * w.lda], sc);
+  };
+  const cplx<T> vI = vval(rI);
+  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = vval((int64_t)J0 * 64 + o);
   __syncthreads();                                   // vseg ready
   cplx<T> rowacc{0, 0};
   const bool h5 = (lane >> 5) & 1, h4 = (lane >> 4) & 1, h3 = (lane >> 3) & 1, h2 = (lane >> 2) & 1;
@@ -277,11 +347,10 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   const int64_t rr = blockIdx.x * (int64_t)64 + lane;
   const int64_t r = j + 1 + rr;
   cplx<T> acc{0, 0};
-  if (rr < m) {
-    for (int k = sub; k < jj; k += 4) {
-      acc = cadd(acc, cmul(V[r + (p + k) * n], sW[k]));
-      acc = cadd(acc, cmul(weff(w, b, r, k, p, scal), sV[k]));
-    }
+  if (rr < m && sub < jj) {
+    acc = panel_terms<T>(
+        sub, jj, [&](int k) { return cmul(V[r + (p + k) * n], sW[k]); },
+        [&](int k) { return cmul(weff(w, b, r, k, p, scal), sV[k]); });
   }
   red[sub][lane] = acc;
   __syncthreads();
@@ -295,25 +364,14 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
     const cplx<T> t = cmulc(y, v);
     pr = t.x; pi = t.y;
   }
-  // the last workgroup only needs the atomically accumulated w^H v
+  // per-workgroup partials of w^H v; alpha2 = -tau/2 (w^H v) is formed by
+  // the next k_col (or k_fixw at the end of the panel)
   pr = block_sum(pr, sh);
   pi = block_sum(pi, sh);
   if (threadIdx.x == 0) {
-    atomic_add_relaxed(&scal[S_YV], pr);
-    atomic_add_relaxed(&scal[S_YV + 1], pi);
-    const unsigned done = atomic_inc_acqrel(&w.cnt[b * w.strideCnt + 1]);
-    if (done == gridDim.x - 1) {
-      __threadfence();
-      const double yr = atomic_load_acq(&scal[S_YV]), yi = atomic_load_acq(&scal[S_YV + 1]);
-      const double tr = scal[S_TAU], ti = scal[S_TAU + 1];
-      // alpha2 = -0.5 * tau * (w^H v)
-      scal[S_A2 + 2 * jj] = -0.5 * (tr * yr - ti * yi);
-      scal[S_A2 + 2 * jj + 1] = -0.5 * (tr * yi + ti * yr);
-      scal[S_YV] = 0.0;
-      scal[S_YV + 1] = 0.0;
-      __threadfence();
-      atomic_store_rel(&w.cnt[b * w.strideCnt + 1], 0u);
-    }
+    double* part = w.part + b * w.strideP + w.yvOff;
+    part[2 * blockIdx.x] = pr;
+    part[2 * blockIdx.x + 1] = pi;
   }
 }
 
@@ -322,13 +380,23 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
 template <typename T>
 __global__ void __launch_bounds__(256) k_fixw(TridiagWork<T> w, int64_t r0, int kw, int kpad,
                                               int64_t p) {
+  __shared__ double sh[8];
   const int64_t b = blockIdx.z;
   const double* scal = w.scal + b * w.strideScal;
   const int64_t r = r0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
+  cplx<T> a2{0, 0};
+  if (k == kw - 1) {
+    // alpha2 of the panel's last column, whose k_fin ran over n - r0 rows
+    double a2r, a2i;
+    alpha2_from_partials(w.part + b * w.strideP, w.yvOff, w.n - r0, scal, sh, a2r, a2i);
+    a2 = cplx<T>{(T)a2r, (T)a2i};
+  } else if (k < kw) {
+    a2 = cplx<T>{(T)scal[S_A2 + 2 * k], (T)scal[S_A2 + 2 * k + 1]};
+  }
   if (r >= w.n || k >= kpad) return;
   cplx<T>* Wc = w.W + b * w.strideW + r + (int64_t)k * w.n;
-  if (k < kw) *Wc = weff(w, b, r, k, p, scal);
+  if (k < kw) *Wc = weff_a(w, b, r, k, p, a2);
   else *Wc = cplx<T>{0, 0};
 }
 
@@ -345,7 +413,6 @@ template <typename T>
 __global__ void k_reset_scal(TridiagWork<T> w) {
   const int64_t b = blockIdx.x;
   for (int i = threadIdx.x; i < w.strideScal; i += blockDim.x) w.scal[b * w.strideScal + i] = 0.0;
-  if (threadIdx.x < 4) w.cnt[b * w.strideCnt + threadIdx.x] = 0u;
 }
 
 template <typename T>
@@ -366,7 +433,11 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
       const int64_t nt = ceil_div(m, 64);
-      const int seg = kHemvSeg;
+      // tiles per workgroup: up to kHemvSeg, fewer once the lower-triangle
+      // tiles of the batch no longer give ~8 workgroups per CU, so that small
+      // trailing matrices are not left to a few long serial chains
+      const int64_t tiles = nt * (nt + 1) / 2 * (int64_t)B;
+      const int seg = (int)std::max<int64_t>(1, std::min<int64_t>(kHemvSeg, tiles / 2048));
       hipLaunchKernelGGL(k_hemv_sym<T>, dim3((unsigned)nt, (unsigned)ceil_div(nt, seg), B), dim3(256),
                          0, s, w, j, jj, p, seg);
       PPH_CHECK_LAUNCH();
