@@ -8,12 +8,17 @@
 //   k_col   : alpha2 of the previous column (from k_fin's partial dots),
 //             correct column j with the panel so far, per-workgroup partial
 //             sums of |x|^2.
-//   k_hemv  : every workgroup forms the reflector (beta, tau, 1/(alpha-beta))
-//             from those partials, then y = A(j+1:, j+1:) v over a 2-D grid of
-//             tile rows x column segments, accumulated with device atomics,
-//             plus the dots W^H v, V^H v; writes v into V.
-//   k_fin   : y - V (W^H v) - W (V^H v), w = tau y, per-workgroup partials of
-//             w^H v.
+//   k_hemv  : u = S x for the trailing matrix S = A(j+1:, j+1:) and the
+//             unscaled column x = (0, A(j+2:, j)) over a 2-D grid of tile rows
+//             x column segments, accumulated with device atomics, plus the
+//             dots W^H x, V^H x; workgroup (0, 0) forms the reflector (beta,
+//             tau, scale = 1/(alpha-beta)) from k_col's partials.
+//   k_fin   : with v = e_0 + scale x: y = S(:, 0) + scale u, the dots
+//             W^H v = conj(W(j+1, :)) + scale W^H x (same for V), then
+//             y - V (W^H v) - W (V^H v), w = tau y; writes v into V and
+//             per-workgroup partials of w^H v.
+// Keeping the hemv linear in the unscaled x lets it start without waiting for
+// the reflector.
 // Every cross-workgroup reduction ends at a kernel boundary: producers store
 // per-workgroup partials and the next kernel sums them in a fixed order, so no
 // kernel needs device-scope fences and the sums are deterministic.
@@ -185,8 +190,9 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
 // 16 column sums of a wave are reduced across lanes by a butterfly that halves
 // the number of live columns at each step.  Results go to the accumulator
 // with device atomics.  Blocks of segment 0 also form the correction dots
-// W^H v, V^H v for their rows and write v into V.
-constexpr int kHemvSeg = 12;      // tiles per workgroup along a tile row
+// W^H x, V^H x for their rows.
+// most tiles per workgroup along a tile row (FP32: 4, FP64: 12, by measurement)
+template <typename T> constexpr int kHemvSeg = sizeof(T) == 4 ? 4 : 12;
 
 template <typename T>
 __device__ __forceinline__ T xsel(bool h, T a, T b) { return h ? a : b; }
@@ -194,7 +200,7 @@ __device__ __forceinline__ T xsel(bool h, T a, T b) { return h ? a : b; }
 template <typename T>
 __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
                                                   int seg) {
-  __shared__ cplx<T> vseg[kHemvSeg * 64];
+  __shared__ cplx<T> vseg[kHemvSeg<T> * 64];
   __shared__ cplx<T> red[4][64];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
@@ -205,7 +211,7 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
   double* scal = w.scal + b * w.strideScal;
   const cplx<T>* A = w.A + b * w.strideA;
   const cplx<T>* S = A + (j + 1) + (j + 1) * w.lda;
-  cplx<T>* V = w.V + b * w.strideV;
+  const cplx<T>* V = w.V + b * w.strideV;
   cplx<T>* y = w.ypart + b * w.strideY + (j + 1);
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const int64_t rI = (int64_t)I * 64 + lane;          // this lane's row within S
@@ -221,25 +227,26 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     }
   };
   load_tile(J0, nxt);
-  // the reflector of column j, from k_col's partial norms (k_col ran over n - j rows)
-  __shared__ double sh[8];
-  const double xn2 = sum_partials(w.part + b * w.strideP, (n - j + 63) / 64, 1, sh);
-  const cplx<T> ajj = A[j + j * w.lda], al = A[(j + 1) + j * w.lda];
-  const Reflector rf = make_reflector(xn2, al.x, al.y);
-  if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
-    w.d[b * w.strideDE + j] = ajj.x;
-    w.e[b * w.strideDE + j] = (T)rf.beta;
-    w.tau[b * w.strideTau + j] = cplx<T>{(T)rf.tr, (T)rf.ti};
-    scal[S_TAU] = rf.tr; scal[S_TAU + 1] = rf.ti;
-    scal[S_SCALE] = rf.sr; scal[S_SCALE + 1] = rf.si;
-  }
-  const cplx<T> sc{(T)rf.sr, (T)rf.si};
-  auto vval = [&](int64_t o) -> cplx<T> {
-    if (o >= m) return cplx<T>{0, 0};
-    return (o == 0) ? cplx<T>{1, 0} : cmul(A[(j + 1 + o) + j * w.lda], sc);
+  // x = (0, A(j+2:, j)): the reflector's direction before scaling
+  auto xval = [&](int64_t o) -> cplx<T> {
+    return (o == 0 || o >= m) ? cplx<T>{0, 0} : A[(j + 1 + o) + j * w.lda];
   };
-  const cplx<T> vI = vval(rI);
-  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = vval((int64_t)J0 * 64 + o);
+  const cplx<T> vI = xval(rI);
+  for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = xval((int64_t)J0 * 64 + o);
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    // the reflector of column j, from k_col's partial norms (k_col ran over n - j rows)
+    __shared__ double sh[8];
+    const double xn2 = sum_partials(w.part + b * w.strideP, (n - j + 63) / 64, 1, sh);
+    if (tid == 0) {
+      const cplx<T> ajj = A[j + j * w.lda], al = A[(j + 1) + j * w.lda];
+      const Reflector rf = make_reflector(xn2, al.x, al.y);
+      w.d[b * w.strideDE + j] = ajj.x;
+      w.e[b * w.strideDE + j] = (T)rf.beta;
+      w.tau[b * w.strideTau + j] = cplx<T>{(T)rf.tr, (T)rf.ti};
+      scal[S_TAU] = rf.tr; scal[S_TAU + 1] = rf.ti;
+      scal[S_SCALE] = rf.sr; scal[S_SCALE + 1] = rf.si;
+    }
+  }
   __syncthreads();                                   // vseg ready
   cplx<T> rowacc{0, 0};
   const bool h5 = (lane >> 5) & 1, h4 = (lane >> 4) & 1, h3 = (lane >> 3) & 1, h2 = (lane >> 2) & 1;
@@ -303,9 +310,8 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     atomicAdd(&y[rI].y, s.y);
   }
   if (blockIdx.y == 0) {
-    // v for this tile row: store into V and form the correction dots
+    // correction dots W^H x, V^H x over this tile row
     const cplx<T> vr = (rI < m) ? vI : cplx<T>{0, 0};
-    if (wv == 0 && rI < m) V[(j + 1 + rI) + j * n] = vr;
     for (int k = wv; k < 2 * jj; k += 4) {
       cplx<T> t{0, 0};
       if (rI < m) {
@@ -334,13 +340,17 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
   double* scal = w.scal + b * w.strideScal;
-  const cplx<T>* V = w.V + b * w.strideV;
+  cplx<T>* V = w.V + b * w.strideV;
+  const cplx<T>* A = w.A + b * w.strideA;
   const cplx<T> tau{(T)scal[S_TAU], (T)scal[S_TAU + 1]};
+  const cplx<T> sc{(T)scal[S_SCALE], (T)scal[S_SCALE + 1]};
   const int lane = threadIdx.x % 64, sub = threadIdx.x / 64;
 
+  // W^H v = conj(W_eff(j+1, k)) + scale (W^H x), V^H v likewise
   for (int k = threadIdx.x; k < 2 * jj; k += blockDim.x) {
     const cplx<T> s = w.dpart[b * w.strideDp + k];
-    if (k < jj) sW[k] = s; else sV[k - jj] = s;
+    if (k < jj) sW[k] = cadd(cconj(weff(w, b, j + 1, k, p, scal)), cmul(sc, s));
+    else sV[k - jj] = cadd(cconj(V[(j + 1) + (p + k - jj) * n]), cmul(sc, s));
   }
   __syncthreads();
 
@@ -356,11 +366,15 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   __syncthreads();
   double pr = 0, pi = 0;
   if (sub == 0 && rr < m) {
-    cplx<T> y = w.ypart[b * w.strideY + r];
+    // y = S(:, 0) + scale u, S(:, 0) = A(j+1:, j+1) with a real diagonal
+    cplx<T> s0 = A[r + (j + 1) * w.lda];
+    if (rr == 0) s0.y = 0;
+    cplx<T> y = cadd(s0, cmul(sc, w.ypart[b * w.strideY + r]));
     y = csub(y, cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane])));
     y = cmul(tau, y);
     w.W[b * w.strideW + r + (int64_t)jj * n] = y;
-    const cplx<T> v = V[r + j * n];
+    const cplx<T> v = (rr == 0) ? cplx<T>{1, 0} : cmul(sc, A[r + j * w.lda]);
+    V[r + j * n] = v;
     const cplx<T> t = cmulc(y, v);
     pr = t.x; pi = t.y;
   }
@@ -433,11 +447,13 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
       const int64_t nt = ceil_div(m, 64);
-      // tiles per workgroup: up to kHemvSeg, fewer once the lower-triangle
-      // tiles of the batch no longer give ~8 workgroups per CU, so that small
-      // trailing matrices are not left to a few long serial chains
+      // tiles per workgroup: up to kHemvSeg, fewer once the batch has less
+      // than 2048 kHemvSeg lower-triangle tiles, so that small trailing
+      // matrices are not left to a few serial chains (measured on 16 x 4764^2
+      // FP32: 4 tiles stream 5.2 TB/s at m = 4763 where 12 give 5.0; 1-2 are
+      // best below m ~ 1200)
       const int64_t tiles = nt * (nt + 1) / 2 * (int64_t)B;
-      const int seg = (int)std::max<int64_t>(1, std::min<int64_t>(kHemvSeg, tiles / 2048));
+      const int seg = (int)std::max<int64_t>(1, std::min<int64_t>(kHemvSeg<T>, tiles / 2048));
       hipLaunchKernelGGL(k_hemv_sym<T>, dim3((unsigned)nt, (unsigned)ceil_div(nt, seg), B), dim3(256),
                          0, s, w, j, jj, p, seg);
       PPH_CHECK_LAUNCH();
