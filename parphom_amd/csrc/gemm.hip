@@ -230,6 +230,8 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0 || p.batch <= 0) return;
   const int64_t ks = p.ksplit > 1 ? p.ksplit : 1, bin = p.binner > 1 ? p.binner : 1;
   if (ks > 1 && p.A2 != nullptr) throw std::runtime_error("launch_gemm: split-K with two products");
+  // a K step must not straddle the two products
+  if (p.A2 != nullptr && p.k1 % GBK != 0) throw std::runtime_error("launch_gemm: k1 must be a multiple of 16");
   dim3 grid((unsigned)ceil_div(p.m, GBM), (unsigned)ceil_div(p.n, GBN),
             (unsigned)(p.batch * bin * ks));
   const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);

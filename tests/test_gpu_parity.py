@@ -14,12 +14,15 @@ def _lib():
     return lib
 
 
+# the second shape is above one large tile (128 x 128 FP32, 64 x 128 FP64)
+# with ragged edges in every dimension
 @pytest.mark.parametrize("prec,tol", [("D", 1e-12), ("S", 2e-5)])
 @pytest.mark.parametrize("ops", ["NN", "CN", "NC", "CC"])
-def test_gemm_ops(gpu_available, prec, tol, ops):
+@pytest.mark.parametrize("mnk", [(77, 130, 45), (300, 261, 97)])
+def test_gemm_ops(gpu_available, prec, tol, ops, mnk):
     lib = _lib()
     rng = np.random.default_rng(1)
-    m, n, k = 77, 130, 45
+    m, n, k = mnk
     opA, opB = ops
     A = rng.standard_normal((m, k) if opA == "N" else (k, m)) + 1j * rng.standard_normal((m, k) if opA == "N" else (k, m))
     B = rng.standard_normal((k, n) if opB == "N" else (n, k)) + 1j * rng.standard_normal((k, n) if opB == "N" else (n, k))
@@ -31,10 +34,11 @@ def test_gemm_ops(gpu_available, prec, tol, ops):
 
 
 @pytest.mark.parametrize("prec,tol", [("D", 1e-12), ("S", 2e-5)])
-def test_gemm_dot_splitk_twoproduct(gpu_available, prec, tol):
+@pytest.mark.parametrize("mnk", [(150, 70, 64), (333, 290, 96)])
+def test_gemm_dot_splitk_twoproduct(gpu_available, prec, tol, mnk):
     lib = _lib()
     rng = np.random.default_rng(2)
-    m, n, k = 150, 70, 64
+    m, n, k = mnk
     A = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
     B = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
     X = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
