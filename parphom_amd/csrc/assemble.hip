@@ -79,7 +79,7 @@ __device__ __forceinline__ void pair_phase(const double* q, double ci0, double c
   o.p0r = o.p0i = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) o.pdr[a] = o.pdi[a] = 0.0;
-#pragma unroll
+#pragma unroll 1   // images are masked; a rolled loop keeps the kernel at 3 waves/SIMD
   for (int l = 0; l < 9; ++l) {
     if (!(m & (1u << l))) continue;
     const double t1 = (double)(l / 3 - 1), t2 = (double)(l % 3 - 1);
@@ -102,8 +102,12 @@ __device__ __forceinline__ void pair_phase(const double* q, double ci0, double c
   }
 }
 
-// Batched D(q) assembly.  One thread per atom pair (i, j) and all q of the
-// batch; Phi_sym's 3x3 block is read once and reused for every q.
+// Batched D(q) assembly for all q of the batch.  A workgroup owns 256 atoms i
+// (rows 3 i0 .. 3 i0 + 767) against one atom j (columns 3j .. 3j + 2).  Per q
+// each thread forms the phase sums of one pair (i, j) into LDS; then each
+// thread produces rows tid, tid + 256, tid + 512 of the three columns, so that
+// consecutive lanes store consecutive rows (contiguous 4 KB per store in
+// complex128).  The thread's nine Phi_sym values are read once for all q.
 // Outputs (column-major, leading dimension n, matrix stride n*n):
 //   D64 / D32   : D(q) in complex128 / complex64 (either may be null)
 //   dD64 / dD32 : 3 matrices per q (x, y, z) of dD/dq (null = not computed)
@@ -124,78 +128,101 @@ struct AsmArgs {
   cplx<float>* dD32;
 };
 
+template <int MODE>
 __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
 #pragma clang fp contract(off)
+  // per pair: phase sum (2), then either the analytic derivative sums
+  // (dr, di) or the central-difference sums at q + dc/2 (dr, di) and q - dc/2 (er, ei)
+  __shared__ double sp0r[256], sp0i[256];
+  __shared__ double sdr[3][256], sdi[3][256], ser[MODE == 1 ? 3 : 1][256], sei[MODE == 1 ? 3 : 1][256];
+  __shared__ double smult[256], ssmm[256];
   const int64_t N = A.N, n = 3 * N;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * 256;
   const int64_t j = blockIdx.y;
-  if (i >= N) return;
+  const int64_t i = i0 + tid;                       // the pair whose phase this thread forms
+  const int64_t nrow = 3 * min((int64_t)256, N - i0);
   double L[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) L[r][c] = A.lat[r + 3 * c];
-  const uint16_t m = A.mask[i + j * N];
-  const double mult = (double)__popc((unsigned)m);
-  const double smm = sqrt(A.rmass[i] * A.rmass[j]);
-  const double ci0 = A.crys[i], ci1 = A.crys[i + N], ci2 = A.crys[i + 2 * N];
+  uint16_t m = 0;
+  double ci0 = 0, ci1 = 0, ci2 = 0;
   const double cj0 = A.crys[j], cj1 = A.crys[j + N], cj2 = A.crys[j + 2 * N];
-  double ph[3][3];
+  if (i < N) {
+    m = A.mask[i + j * N];
+    smult[tid] = (double)__popc((unsigned)m);
+    ssmm[tid] = sqrt(A.rmass[i] * A.rmass[j]);
+    ci0 = A.crys[i]; ci1 = A.crys[i + N]; ci2 = A.crys[i + 2 * N];
+  }
+  // Phi_sym(3 i0 + r, 3 j + b) for this thread's rows r = tid + 256 k
+  double f[3][3];
 #pragma unroll
-  for (int b = 0; b < 3; ++b)
+  for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) ph[a][b] = A.phi[(3 * i + a) + (3 * j + b) * n];
+    for (int b = 0; b < 3; ++b) {
+      const int64_t r = tid + 256 * k;
+      f[k][b] = (r < nrow) ? A.phi[(3 * i0 + r) + (3 * j + b) * n] : 0.0;
+    }
 
   const bool want_d = (A.dD64 != nullptr) || (A.dD32 != nullptr);
   const size_t mstride = (size_t)n * n;
   for (int64_t iq = 0; iq < A.nb; ++iq) {
     const double* q = A.q + 3 * iq;
-    PhaseOut po;
-    pair_phase(q, ci0, ci1, ci2, cj0, cj1, cj2, m, L, want_d && A.mode == 0, po);
-    // analytic: (dr, di) = derivative phase sums; central difference:
-    // (dr, di) = phase sum at q + dc/2 and (er, ei) = phase sum at q - dc/2
-    double dr[3], di[3], er[3], ei[3];
-    if (want_d && A.mode == 1) {
-      for (int a = 0; a < 3; ++a) {
-        double qp[3], qm[3];
-        for (int c = 0; c < 3; ++c) {
-          qp[c] = q[c] + A.cd_dc[3 * a + c] / 2.0;
-          qm[c] = q[c] - A.cd_dc[3 * a + c] / 2.0;
+    if (i < N) {
+      PhaseOut po;
+      pair_phase(q, ci0, ci1, ci2, cj0, cj1, cj2, m, L, want_d && MODE == 0, po);
+      sp0r[tid] = po.p0r;
+      sp0i[tid] = po.p0i;
+      if (MODE == 1 && want_d) {
+        for (int a = 0; a < 3; ++a) {
+          double qp[3], qm[3];
+          for (int c = 0; c < 3; ++c) {
+            qp[c] = q[c] + A.cd_dc[3 * a + c] / 2.0;
+            qm[c] = q[c] - A.cd_dc[3 * a + c] / 2.0;
+          }
+          PhaseOut pp, pm;
+          pair_phase(qp, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pp);
+          pair_phase(qm, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pm);
+          sdr[a][tid] = pp.p0r; sdi[a][tid] = pp.p0i;
+          ser[MODE == 1 ? a : 0][tid] = pm.p0r; sei[MODE == 1 ? a : 0][tid] = pm.p0i;
         }
-        PhaseOut pp, pm;
-        pair_phase(qp, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pp);
-        pair_phase(qm, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pm);
-        dr[a] = pp.p0r; di[a] = pp.p0i;
-        er[a] = pm.p0r; ei[a] = pm.p0i;
+      } else if (want_d) {
+        for (int a = 0; a < 3; ++a) { sdr[a][tid] = po.pdr[a]; sdi[a][tid] = po.pdi[a]; }
       }
-    } else if (want_d) {
-      for (int a = 0; a < 3; ++a) { dr[a] = po.pdr[a]; di[a] = po.pdi[a]; er[a] = ei[a] = 0.0; }
     }
+    __syncthreads();
     const size_t base = (size_t)iq * mstride;
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
+    for (int k = 0; k < 3; ++k) {
+      const int64_t r = tid + 256 * k;
+      if (r >= nrow) continue;
+      const int pr = (int)(r / 3);                   // pair within the workgroup
+      const double p0r = sp0r[pr], p0i = sp0i[pr], mult = smult[pr], smm = ssmm[pr];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const size_t idx = base + (size_t)(3 * i + a) + (size_t)(3 * j + b) * n;
-        const double f = ph[a][b];
+      for (int b = 0; b < 3; ++b) {
+        const size_t idx = base + (size_t)(3 * i0 + r) + (size_t)(3 * j + b) * n;
+        const double fv = f[k][b];
         // dij = Phi * phase / multiplicity / sqrt(m_i m_j)
-        const double vr = ((f * po.p0r) / mult) / smm;
-        const double vi = ((f * po.p0i) / mult) / smm;
+        const double vr = ((fv * p0r) / mult) / smm;
+        const double vi = ((fv * p0i) / mult) / smm;
         if (A.D64) A.D64[idx] = cplx<double>{vr, vi};
         if (A.D32) A.D32[idx] = cplx<float>{(float)vr, (float)vi};
         if (want_d) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             double wr, wi;
-            if (A.mode == 1) {
+            if (MODE == 1) {
               // (D(q+) - D(q-)) / |delta| elementwise
-              const double pr = ((f * dr[d]) / mult) / smm, pi_ = ((f * di[d]) / mult) / smm;
-              const double mr = ((f * er[d]) / mult) / smm, mi = ((f * ei[d]) / mult) / smm;
-              wr = (pr - mr) / A.cd_norm;
+              const double pr_ = ((fv * sdr[d][pr]) / mult) / smm, pi_ = ((fv * sdi[d][pr]) / mult) / smm;
+              const double mr = ((fv * ser[MODE == 1 ? d : 0][pr]) / mult) / smm;
+              const double mi = ((fv * sei[MODE == 1 ? d : 0][pr]) / mult) / smm;
+              wr = (pr_ - mr) / A.cd_norm;
               wi = (pi_ - mi) / A.cd_norm;
             } else {
-              wr = ((f * dr[d]) / mult) / smm;
-              wi = ((f * di[d]) / mult) / smm;
+              wr = ((fv * sdr[d][pr]) / mult) / smm;
+              wi = ((fv * sdi[d][pr]) / mult) / smm;
             }
             const size_t didx = (size_t)(3 * iq + d) * mstride + (idx - base);
             if (A.dD64) A.dD64[didx] = cplx<double>{wr, wi};
@@ -204,6 +231,7 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
         }
       }
     }
+    __syncthreads();                                 // LDS reused by the next q
   }
 }
 
@@ -228,7 +256,8 @@ void launch_assemble(const AsmParams& P, hipStream_t s) {
   A.N = P.N; A.nb = P.nb; A.mode = P.mode;
   A.D64 = P.D64; A.D32 = P.D32; A.dD64 = P.dD64; A.dD32 = P.dD32;
   dim3 grid((unsigned)ceil_div(P.N, 256), (unsigned)P.N);
-  hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, s, A);
+  if (A.mode == 1) hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, s, A);
+  else hipLaunchKernelGGL(k_assemble<0>, grid, dim3(256), 0, s, A);
   PPH_CHECK_LAUNCH();
 }
 
