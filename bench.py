@@ -130,8 +130,10 @@ def main():
     asm_bytes = 8 * n * n + B * (16 + 8 + 3 * 8) * n * n
     asm_s = phases["assemble"] / 1e3
     # roofline of the velocity products (one launch per batch, 3 directions):
-    # algorithmic flops = B * 24 n^3 (complex n x n x n per direction)
-    vel_flops = B * 24.0 * n ** 3
+    # diag(Z^H dD Z) with dD Hermitian is formed from its lower triangle,
+    # 2 Re(Z^H A' Z), a triangular n x n times n x n complex product: n^3 / 2
+    # complex multiply-adds = 4 n^3 flops per direction, B * 12 n^3 per launch
+    vel_flops = B * 12.0 * n ** 3
     vel_s = phases["velocity"] / 1e3
     rl = {"assemble": {"bound": "hbm", "achieved": asm_bytes / asm_s / 1e9, "peak": HBM_PEAK / 1e9,
                        "unit": "GB/s", "frac": asm_bytes / asm_s / HBM_PEAK, "traffic": None},

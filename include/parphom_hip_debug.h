@@ -8,6 +8,9 @@
  *   mode 1: dot[j] = sum_i Re(conj(C_in[i,j]) (op(A) op(B))[i,j]), C unchanged
  *   mode 2: C = op(A) op(B) through the split-K path (4 pieces)
  *   mode 3: C = op(A) op(B) through the two-product path (opA = opB = 'N')
+ *   mode 4: as mode 1 for Hermitian A (m = k, opA = 'N'), reading only the
+ *           lower triangle of A (the half-work path of the velocity and
+ *           Rayleigh-quotient products)
  * prec 'D' complex128, 'S' complex64; column-major; returns 0 or < 0.
  */
 #ifndef PARPHOM_HIP_DEBUG_H

@@ -504,6 +504,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
       g.B = h.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
       g.batch = B;
       g.dot_out = h.num.as<double>(); g.dot_stride = mcap;
+      g.herm_dot = 1;            // D is Hermitian: lower triangle, half the work
       g.dotX = h.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
       launch_gemm<double>(g, s);
       launch_colnorm2<double>(h.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, h.den.as<double>(),
@@ -582,6 +583,7 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
       g.B = h.Z.as<cplx<float>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
       g.batch = B; g.binner = 3;
       g.dot_out = h.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
+      g.herm_dot = 1;            // dD/dq is Hermitian: lower triangle, half the work
       g.dotX = h.Z.as<cplx<float>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
       launch_gemm<float>(g, s);
     } else {
@@ -593,6 +595,7 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
       g.B = h.Z.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
       g.batch = B; g.binner = 3;
       g.dot_out = h.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
+      g.herm_dot = 1;            // dD/dq is Hermitian: lower triangle, half the work
       g.dotX = h.Z.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
       launch_gemm<double>(g, s);
     }
@@ -916,6 +919,7 @@ int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t
     g.B = Zq; g.ldb = n; g.strideB = n * zcols;
     g.batch = nb;
     g.dot_out = od; g.dot_stride = nz;
+    g.herm_dot = 1;
     g.dotX = Zq; g.lddx = n; g.strideX = n * zcols;
     launch_gemm<double>(g, s);
     if (!on_device)
@@ -1036,8 +1040,10 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
     g.alpha = cplx<T>{1, 0}; g.beta = cplx<T>{0, 0};
     g.A = dA.as<cplx<T>>(); g.lda = lda; g.B = dB.as<cplx<T>>(); g.ldb = ldb;
     g.C = dC.as<cplx<T>>(); g.ldc = ldc; g.batch = 1;
-    if (mode == 1) {
+    if (mode == 1 || mode == 4) {
+      // mode 4: A Hermitian (lower triangle read), herm_dot epilogue
       g.dot_out = dX.as<double>(); g.dotX = dC.as<cplx<T>>(); g.lddx = ldc;
+      g.herm_dot = (mode == 4) ? 1 : 0;
     } else if (mode == 2) {
       launch_fill_zero(dC.p, es * ldc * n, s);
       g.ksplit = 4;

@@ -69,6 +69,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     kbeg = ks_idx * kc;
     K = min(p.k, kbeg + kc);
   }
+  // Hermitian dot: A'(r, k) = 0 for k > r, so the row tile needs k < row0 + GBM
+  const bool herm = p.herm_dot != 0;
+  if (herm) K = min(K, row0 + GBM);
   const int64_t K1 = two ? p.k1 : K;
 
   constexpr bool k3M = true;
@@ -105,6 +108,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const int r = tid % 64, kk = tid / 64 + 4 * i;
         const int64_t gr = row0 + r, gk = k0 + kk;
         ra[i] = ld(A, gr + gk * p.lda, gr < M && gk < Kseg, false);
+        if (herm && gk >= gr) ra[i] = (gk == gr) ? cplx<T>{T(0.5) * ra[i].x, T(0)} : cplx<T>{0, 0};
       } else {
         const int kk = tid % 16, r = tid / 16 + 16 * i;
         const int64_t gr = row0 + r, gk = k0 + kk;
@@ -218,6 +222,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             s += (double)x.x * (double)v.x + (double)x.y * (double)v.y;
           }
       }
+      if (herm) s *= 2.0;
       s += __shfl_xor(s, 16, WAVE);
       s += __shfl_xor(s, 32, WAVE);
       if (lane < 16 && col < N) atomicAdd(dout + col, s);
@@ -230,6 +235,8 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0 || p.batch <= 0) return;
   const int64_t ks = p.ksplit > 1 ? p.ksplit : 1, bin = p.binner > 1 ? p.binner : 1;
   if (ks > 1 && p.A2 != nullptr) throw std::runtime_error("launch_gemm: split-K with two products");
+  if (p.herm_dot && (p.dot_out == nullptr || p.opA != 'N' || p.A2 != nullptr || p.m != p.k))
+    throw std::runtime_error("launch_gemm: herm_dot needs the dot epilogue, opA = 'N', square A, one product");
   // a K step must not straddle the two products
   if (p.A2 != nullptr && p.k1 % GBK != 0) throw std::runtime_error("launch_gemm: k1 must be a multiple of 16");
   dim3 grid((unsigned)ceil_div(p.m, GBM), (unsigned)ceil_div(p.n, GBN),

@@ -55,6 +55,12 @@ struct GemmParams {
   // with X = dotX (leading dimension lddx, stride strideX).
   double* dot_out; int64_t dot_stride;
   const cplx<T>* dotX; int64_t lddx, strideX;
+  // herm_dot != 0 (with the dot epilogue, opA = 'N', X = op(B)): A is Hermitian
+  // and only its lower triangle is read.  The product uses A' = strict lower
+  // triangle + Re(diag)/2 and the dot is doubled, since
+  // x^H A x = 2 Re(x^H A' x); each row tile stops at its own diagonal, which
+  // halves the work.
+  int herm_dot;
 };
 
 template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);

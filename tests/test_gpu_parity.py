@@ -52,6 +52,23 @@ def test_gemm_dot_splitk_twoproduct(gpu_available, prec, tol, mnk):
     assert np.abs(C3 - ref).max() <= tol * np.abs(ref).max() * 10
 
 
+@pytest.mark.parametrize("prec,tol", [("D", 1e-12), ("S", 2e-5)])
+@pytest.mark.parametrize("n,nx", [(64, 10), (200, 77), (333, 290)])
+def test_gemm_hermitian_dot(gpu_available, prec, tol, n, nx):
+    """diag(X^H A X) from the lower triangle of a Hermitian A; the upper
+    triangle holds garbage that must not be read."""
+    lib = _lib()
+    rng = np.random.default_rng(3)
+    H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = H + H.conj().T
+    X = rng.standard_normal((n, nx)) + 1j * rng.standard_normal((n, nx))
+    dref = np.einsum("ij,ij->j", X.conj(), H @ X).real
+    A = np.tril(H) + np.triu(np.full((n, n), 1e3 + 7e2j), 1)
+    A[np.diag_indices(n)] += 0.25j          # an imaginary diagonal is ignored too
+    _, dot = lib.debug_zgemm(A, X, prec=prec, mode=4, Cin=X)
+    assert np.abs(dot - dref).max() <= tol * np.abs(H).max() * np.abs(X).max() ** 2 * n
+
+
 @pytest.mark.parametrize("mn,seed", [((1, 2), 11), ((3, 4), 1003)])
 @pytest.mark.parametrize("method", ["A", "C"])
 def test_assembly_matches_oracle(gpu_available, mn, seed, method):
