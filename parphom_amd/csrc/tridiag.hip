@@ -202,7 +202,10 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
                                                   int seg) {
   __shared__ cplx<T> vseg[kHemvSeg<T> * 64];
   __shared__ cplx<T> red[4][64];
-  const int64_t b = blockIdx.z;
+  // odd columns walk the batch backwards: the first workgroups then read the
+  // matrices the previous launch read last, which the memory-side cache
+  // (256 MB) may still hold
+  const int64_t b = (j & 1) ? (int64_t)gridDim.z - 1 - blockIdx.z : (int64_t)blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
   const int I = blockIdx.x;
   const int J0 = blockIdx.y * seg;
