@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12            # B/s
 FP32_PEAK = 157.3e12         # FLOP/s (dense, matrix and vector)
 FP64_PEAK = 78.6e12
+HEMV_TIME_STRIDE = 8         # the library times every 8th hemv launch
 
 
 def log(*a):
@@ -76,6 +77,34 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup):
     return dt, prof, ev
 
 
+def split_profile(prof, n):
+    """Phase ms per batch, and the mean duration of one hemv launch in ms (the
+    library times every HEMV_TIME_STRIDE-th of the n - 1 launches per batch;
+    the per-batch hemv figure is that mean times n - 1)."""
+    phases = {k: v[0] / max(1, v[1]) for k, v in prof.items() if k != "hemv_launches"}
+    ms, cnt = prof["hemv_launches"]
+    launch_ms = ms / max(1, cnt)
+    phases["hemv"] = launch_ms * (n - 1)
+    return phases, launch_ms
+
+
+def hemv_traffic(workload, n, B):
+    """HBM bytes per k_hemv_sym launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of the same workload and batch (counters scaled by a
+    1 GiB copy), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"k_hemv_sym_{workload}_b{B}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if line.startswith("- ") and ":" in line:
+            k, v = line[2:].split(":", 1)
+            kv[k.strip()] = v.strip().strip("`")
+    if kv.get("n") != str(n) or kv.get("batch") != str(B):
+        return None
+    return float(kv["traffic_bytes_per_launch"])
+
+
 def cpu_baseline(st, fc, q, seconds_cap=60.0):
     """The FP64 oracle (numpy assembly, LAPACK zheevx, numpy GEMMs) on one
     q-point of the same workload, on this host's cores."""
@@ -92,10 +121,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--workload", default="tbg_11_12")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device 3 times and exit (a copy of known "
+                         "size, to scale the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the phonon path has no multi-GPU mode "
@@ -103,6 +135,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if a.copy_calibration:
+        src = torch.ones(a.copy_calibration // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "copies": 3}))
+        return
     from parphom_amd import lib, synth
     lib.load()
     t0 = time.perf_counter()
@@ -122,7 +162,7 @@ def main():
            "config": {"workload": a.workload, "natom": st.natom, "n": n, "q_per_step": B,
                       "range": "A", "eigenvectors": True, "group_velocity": True,
                       "precision": "S"}}
-    phases = {k: v[0] / max(1, v[1]) for k, v in prof32.items()}
+    phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
 
     # roofline of the assembly kernel (one launch per batch, HBM-bound):
@@ -140,10 +180,27 @@ def main():
           "velocity": {"bound": "mfma", "achieved": vel_flops / vel_s / 1e12,
                        "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
                        "frac": vel_flops / vel_s / FP32_PEAK, "traffic": None}}
+    # roofline of the dominant kernel, the hemv of the reduction (k_hemv_sym,
+    # one launch per column j, HBM-bound): algorithmic bytes per launch = the
+    # lower triangle of the trailing matrix, m (m + 1) / 2 complex64 elements
+    # with m = n - 1 - j, plus the vector in and the product out (2 m
+    # elements), for each of the B matrices.  The library times the launches
+    # of columns j = 0, 8, 16, ... with HIP-event pairs on its stream during
+    # the timed steps; bytes are averaged over the same columns.
+    m = (n - 1 - np.arange(0, n - 1, HEMV_TIME_STRIDE)).astype(np.float64)
+    hemv_bytes = B * 8.0 * float(np.mean(m * (m + 1) / 2 + 2 * m))
+    hemv_s = hemv_launch_ms / 1e3
+    rl["hemv"] = {"bound": "hbm", "achieved": hemv_bytes / hemv_s / 1e9, "peak": HBM_PEAK / 1e9,
+                  "unit": "GB/s", "frac": hemv_bytes / hemv_s / HBM_PEAK,
+                  "traffic": hemv_traffic(a.workload, n, B)}
     dominant = max(("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity"),
                    key=lambda k: phases[k])
     res["dominant_phase"] = dominant
-    res["roofline"] = dict(rl["velocity"], kernel="velocity products (k_zgemm dot epilogue, FP32)")
+    res["roofline"] = dict(rl["hemv"], kernel="k_hemv_sym<float>",
+                           bytes_per_launch=hemv_bytes, avg_launch_us=hemv_launch_ms * 1e3,
+                           ms_per_batch=round(phases["hemv"], 3))
+    res["roofline_velocity"] = dict(rl["velocity"],
+                                    kernel="velocity products (k_zgemm dot epilogue, FP32)")
     res["roofline_assembly"] = dict(rl["assemble"], kernel="k_assemble")
 
     if not a.no_fp64:
@@ -151,9 +208,9 @@ def main():
                                           if a.warmup == 0 else 1)
         qps64 = max(1, a.steps - 1) * B / dt64
         dw = float(np.abs(ev32 - ev64).max())
+        ph64, _ = split_profile(prof64, n)
         res["fp64_path"] = {"value": qps64, "unit": "q-pts/s",
-                            "phase_ms_per_batch": {k: round(v[0] / max(1, v[1]), 3)
-                                                   for k, v in prof64.items()}}
+                            "phase_ms_per_batch": {k: round(v, 3) for k, v in ph64.items()}}
         res["speedup_vs_fp64"] = qps32 / qps64
         res["max_abs_domega_vs_fp64_cm"] = dw
 

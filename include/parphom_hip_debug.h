@@ -25,9 +25,12 @@ int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t
 /* Phase timers measured with HIP events on the handle's stream around each
  * phase of pph_run / pph_run_device.  Slots: 0 assembly, 1 tridiagonal
  * reduction, 2 tridiagonal eigensolver, 3 back-transformation, 4 FP64
- * refinement, 5 velocity products, 6 whole batch.  ms/counts (7 entries
- * each, may be NULL) receive the accumulated milliseconds and batch counts;
- * reset != 0 clears them; enable turns recording on or off. */
+ * refinement, 5 velocity products, 6 whole batch, 7 the k_hemv_sym launches
+ * of the reduction alone, sampled at columns j = 0, 8, 16, ... (an event
+ * pair around each sampled launch; the count is the number of launches
+ * timed).  ms/counts (8 entries each, may be NULL) receive
+ * the accumulated milliseconds and counts; reset != 0 clears them; enable
+ * turns recording on or off. */
 int pph_profile(void* h, int32_t enable, int32_t reset, double* ms, int64_t* counts);
 
 #ifdef __cplusplus

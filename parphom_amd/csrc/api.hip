@@ -81,7 +81,8 @@ static void linsolve(const double* x, const double* A, double* u) {
 }
 
 // phase timers recorded on the handle's stream (HIP events)
-enum Phase { PH_ASM = 0, PH_TRID, PH_TRIEIG, PH_BACK, PH_REFINE, PH_VEL, PH_TOTAL, PH_N };
+// PH_HEMV: the k_hemv_sym launches alone (ms summed, count = launches)
+enum Phase { PH_ASM = 0, PH_TRID, PH_TRIEIG, PH_BACK, PH_REFINE, PH_VEL, PH_TOTAL, PH_HEMV, PH_N };
 
 struct Timers {
   bool on = false;
@@ -102,8 +103,29 @@ struct Timers {
     if (!on) return;
     PPH_HIP(hipEventRecord(ev[2 * p + 1], s));
   }
+  // per-launch event pairs for one kernel (the hemv of the reduction)
+  std::vector<hipEvent_t> kev;
+  int64_t kev_pending = 0;
+  hipEvent_t* kernel_events(int64_t pairs) {
+    if (!on) return nullptr;
+    while ((int64_t)kev.size() < 2 * pairs) {
+      hipEvent_t e;
+      PPH_HIP(hipEventCreate(&e));
+      kev.push_back(e);
+    }
+    kev_pending = pairs;
+    return kev.data();
+  }
   void collect() {
     if (!on) return;
+    for (int64_t i = 0; i < kev_pending; ++i) {
+      PPH_HIP(hipEventSynchronize(kev[2 * i + 1]));
+      float t = 0.f;
+      PPH_HIP(hipEventElapsedTime(&t, kev[2 * i], kev[2 * i + 1]));
+      ms[PH_HEMV] += t;
+      cnt[PH_HEMV] += 1;
+    }
+    kev_pending = 0;
     for (int p = 0; p < PH_N; ++p) {
       if (!used[p]) continue;
       PPH_HIP(hipEventSynchronize(ev[2 * p + 1]));
@@ -117,6 +139,7 @@ struct Timers {
   ~Timers() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : kev) (void)hipEventDestroy(e);
   }
 };
 
@@ -126,6 +149,10 @@ struct Handle {
   pph_opts opts{};
   Geometry g;
   int64_t N = 0, n = 0;
+  // leading dimension of the matrices the reduction works on (D64 / D32):
+  // n rounded up to 16 elements, so that columns start on 128-byte lines
+  int64_t ldA = 0;
+  void set_n(int64_t nn) { n = nn; ldA = (nn + 15) / 16 * 16; }
   hipStream_t stream = nullptr;
   std::string err;
   // persistent device state
@@ -265,8 +292,8 @@ static int64_t range_mcap(const pph_opts& o, int64_t n) {
 
 // Low-frequency modes after the Rayleigh quotients: their frequencies
 // omega = K sqrt(lambda) magnify eigenvalue errors, so the lowest modes
-// (omega < 50 cm^-1, at most 16 per matrix) get a Rayleigh-Ritz step inside
-// each cluster of close eigenvalues and a second-order correction
+// (omega < 50 cm^-1, at most 16 per matrix) get one Rayleigh-Ritz step over
+// the span of all of them and a second-order correction
 // -sum_j |x_j^H r|^2 / ((rho_j - theta) |x_j|^2) from every other computed
 // mode j, all in FP64: W = D X_T, C1 = X^H W, C2 = X^H X_T on the GPU, the
 // k x k algebra on the host.
@@ -302,7 +329,7 @@ static void refine_low_modes(Handle& h, int64_t B, const cplx<double>* D64, cons
     GemmParams<double> g{};                 // W_T = D X(:, 0:K)
     g.opA = 'N'; g.opB = 'N'; g.m = n; g.n = K; g.k = n;
     g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
-    g.A = D64; g.lda = n; g.strideA = n * n;
+    g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
     g.B = X; g.ldb = n; g.strideB = n * mcap;
     g.C = WT; g.ldc = n; g.strideC = n * Kp;
     g.batch = B;
@@ -338,10 +365,14 @@ static void refine_low_modes(Handle& h, int64_t B, const cplx<double>* D64, cons
     double* lb = lam.data() + b * mcap;
     const double* nb = nrm.data() + b * mcap;
     std::vector<double> newl(lb, lb + kt);
+    // one Rayleigh-Ritz block over all kt target modes: near-degenerate low
+    // modes (the acoustic triplet at Gamma) may be closer to each other than
+    // the FP32 residual, where a per-mode second-order correction is not
+    // valid; Ritz values from the larger span are never worse, and what is
+    // left couples only to modes above the 50 cm^-1 threshold
     int64_t c0 = 0;
     while (c0 < kt) {
-      int64_t c1e = c0 + 1;
-      while (c1e < kt && lb[c1e] - lb[c1e - 1] <= ctol) ++c1e;
+      const int64_t c1e = kt;
       // extend over the full cluster in the whole spectrum (members beyond the targets
       // are not refined, but they are excluded from the coupling sums)
       int64_t cend = c1e;
@@ -402,8 +433,8 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   const int64_t nparts = n / 64 + 2;
   h.part.ensure((size_t)B * 3 * nparts * sizeof(double));
   TridiagWork<T> tw{};
-  tw.n = n; tw.lda = n; tw.batch = B; tw.nb = nb;
-  tw.A = A; tw.strideA = n * n;
+  tw.n = n; tw.lda = h.ldA; tw.batch = B; tw.nb = nb;
+  tw.A = A; tw.strideA = h.ldA * n;
   tw.V = h.V.as<cplx<T>>(); tw.strideV = n * (n + kVpad);
   tw.W = h.W.as<cplx<T>>(); tw.strideW = n * nb;
   tw.tau = h.tau.as<cplx<T>>(); tw.strideTau = n;
@@ -413,6 +444,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   tw.scal = h.scal.as<double>(); tw.strideScal = 256;
   tw.part = h.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
   tw.nchunk = h.nchunk;
+  tw.hemv_ev = h.tm.kernel_events(ceil_div(n - 1, kHemvTimeStride));
   h.tm.begin(PH_TRID, s);
   tridiagonalise<T>(tw, s);
   h.tm.end(PH_TRID, s);
@@ -500,7 +532,7 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
       g.opA = 'N'; g.opB = 'N';
       g.m = n; g.n = mmax; g.k = n;
       g.alpha = cplx<double>{1, 0};
-      g.A = D64; g.lda = n; g.strideA = n * n;
+      g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
       g.B = h.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
       g.batch = B;
       g.dot_out = h.num.as<double>(); g.dot_stride = mcap;
@@ -537,8 +569,8 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
 
   h.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(h.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
-  h.A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
-  if (fp32) h.A32.ensure((size_t)B * n * n * sizeof(cplx<float>));
+  h.A64.ensure((size_t)B * h.ldA * n * sizeof(cplx<double>));
+  if (fp32) h.A32.ensure((size_t)B * h.ldA * n * sizeof(cplx<float>));
   if (vel) h.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
 
   AsmParams ap{};
@@ -548,6 +580,7 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
   ap.N = N; ap.nb = B; ap.mode = (o.vel_method == 'C') ? 1 : 0;
   ap.D64 = h.A64.as<cplx<double>>();
   ap.D32 = fp32 ? h.A32.as<cplx<float>>() : nullptr;
+  ap.ldD = h.ldA;
   if (vel) {
     if (fp32) ap.dD32 = h.dD.as<cplx<float>>();
     else ap.dD64 = h.dD.as<cplx<double>>();
@@ -606,12 +639,21 @@ static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 r
   h.tm.end(PH_TOTAL, s);
 }
 
+// q-points per device batch when the caller does not fix it: as many as fit
+// in half of the device's free memory (larger batches keep more of the
+// per-column launches of the reduction busy), at most 64.  Per matrix: D in
+// FP64 (16 n^2) and FP32 (8 n^2), dD (48 or 96 n^2), Householder vectors,
+// eigenvectors (FP32 + FP64 copies, the FP64 tridiagonal vectors and their
+// LU factors), about 6 n^2 x 16 bytes more.
 static int64_t auto_batch(const Handle& h, int64_t nq) {
   if (h.opts.batch > 0) return std::min(nq, h.opts.batch);
   const int64_t n = h.n;
-  const double per = (double)n * n * (16.0 + (h.opts.precision == 'S' ? 8.0 : 0.0) +
-                                      (h.opts.vel ? 48.0 : 0.0) + 16.0 + 80.0 + 24.0);
-  int64_t b = (int64_t)std::floor(24e9 / std::max(per, 1.0));
+  const bool s = h.opts.precision == 'S';
+  const double per = (double)n * n * (16.0 + (s ? 8.0 : 0.0) + (h.opts.vel ? (s ? 48.0 : 96.0) : 0.0) +
+                                      96.0);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)24e9;
+  int64_t b = (int64_t)std::floor(0.5 * (double)free_b / std::max(per, 1.0));
   b = std::max<int64_t>(1, std::min<int64_t>(b, 64));
   return std::min(nq, b);
 }
@@ -679,7 +721,7 @@ int pph_create(const pph_system* sys, const double* fc, const pph_opts* opts, pp
   PPH_HIP(hipSetDevice(h->device));
   PPH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   const int64_t N = sys->natom, n = 3 * N;
-  h->N = N; h->n = n;
+  h->N = N; h->set_n(n);
   Geometry& g = h->g;
   g.N = N;
   std::memcpy(g.lat, sys->lat, sizeof(g.lat));
@@ -841,24 +883,23 @@ int pph_assemble(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32
     ap.rmass = h->rmass.as<double>(); ap.mask = h->mask.as<uint16_t>(); ap.q = h->qdev.as<double>();
     ap.cd_dc = h->cd_dc.as<double>(); ap.cd_norm = h->cd_norm;
     ap.N = h->N; ap.nb = B; ap.mode = (h->opts.vel_method == 'C') ? 1 : 0;
-    cplx<double>* dst;
-    if (what == 0) {
-      h->A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
-      ap.D64 = h->A64.as<cplx<double>>();
-      dst = ap.D64;
-    } else {
+    const int64_t ldA = h->ldA;
+    h->A64.ensure((size_t)B * ldA * n * sizeof(cplx<double>));
+    ap.D64 = h->A64.as<cplx<double>>();
+    ap.ldD = ldA;
+    if (what != 0) {
       h->dD.ensure((size_t)B * 3 * n * n * sizeof(cplx<double>));
-      h->A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
-      ap.D64 = h->A64.as<cplx<double>>();
       ap.dD64 = h->dD.as<cplx<double>>();
-      dst = ap.dD64;
     }
     launch_assemble(ap, s);
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     for (int64_t b = 0; b < B; ++b) {
-      const cplx<double>* src = (what == 0) ? dst + (size_t)b * n * n
-                                            : dst + (size_t)(3 * b + (what - 1)) * n * n;
-      PPH_HIP(hipMemcpyAsync((char*)out + (size_t)(q0 + b) * n * n * 16, src, (size_t)n * n * 16,
-                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+      char* o = (char*)out + (size_t)(q0 + b) * n * n * 16;
+      if (what == 0)   // D is stored with leading dimension ldA
+        PPH_HIP(hipMemcpy2DAsync(o, n * 16, ap.D64 + (size_t)b * ldA * n, ldA * 16, n * 16, n, kind, s));
+      else
+        PPH_HIP(hipMemcpyAsync(o, ap.dD64 + (size_t)(3 * b + (what - 1)) * n * n, (size_t)n * n * 16,
+                               kind, s));
     }
     PPH_HIP(hipStreamSynchronize(s));
   }
@@ -892,8 +933,9 @@ int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t
     ap.rmass = h->rmass.as<double>(); ap.mask = h->mask.as<uint16_t>(); ap.q = h->qdev.as<double>();
     ap.cd_dc = h->cd_dc.as<double>(); ap.cd_norm = h->cd_norm;
     ap.N = h->N; ap.nb = nb; ap.mode = (h->opts.vel_method == 'C') ? 1 : 0;
-    h->A64.ensure((size_t)nb * n * n * sizeof(cplx<double>));
+    h->A64.ensure((size_t)nb * h->ldA * n * sizeof(cplx<double>));
     ap.D64 = h->A64.as<cplx<double>>();
+    ap.ldD = h->ldA;
     if (what > 0) {
       h->dD.ensure((size_t)nb * 3 * n * n * sizeof(cplx<double>));
       ap.dD64 = h->dD.as<cplx<double>>();
@@ -915,7 +957,8 @@ int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t
     g.opA = 'N'; g.opB = 'N'; g.m = n; g.n = nz; g.k = n;
     g.alpha = cplx<double>{1, 0};
     g.A = (what == 0) ? h->A64.as<cplx<double>>() : h->dD.as<cplx<double>>() + (size_t)(what - 1) * n * n;
-    g.lda = n; g.strideA = (what == 0) ? n * n : 3 * n * n;
+    g.lda = (what == 0) ? h->ldA : n;
+    g.strideA = (what == 0) ? h->ldA * n : 3 * n * n;
     g.B = Zq; g.ldb = n; g.strideB = n * zcols;
     g.batch = nb;
     g.dot_out = od; g.dot_stride = nz;
@@ -943,7 +986,7 @@ int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w,
   h.device = opts->device;
   PPH_HIP(hipSetDevice(h.device));
   PPH_HIP(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
-  h.n = n; h.N = 0;
+  h.set_n(n); h.N = 0;
   hipStream_t s = h.stream;
   const bool fp32 = opts->precision == 'S';
   const hipMemcpyKind in_kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -951,12 +994,15 @@ int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w,
   const int64_t bmax = std::max<int64_t>(1, std::min<int64_t>(nmat, opts->batch > 0 ? opts->batch : 8));
   for (int64_t m0 = 0; m0 < nmat; m0 += bmax) {
     const int64_t B = std::min(bmax, nmat - m0);
-    h.A64.ensure((size_t)B * n * n * sizeof(cplx<double>));
-    PPH_HIP(hipMemcpyAsync(h.A64.p, (char*)A + (size_t)m0 * n * n * 16, (size_t)B * n * n * 16, in_kind, s));
+    const int64_t ldA = h.ldA;
+    h.A64.ensure((size_t)B * ldA * n * sizeof(cplx<double>));
+    // the caller's matrices (leading dimension n) into the padded layout
+    PPH_HIP(hipMemcpy2DAsync(h.A64.p, ldA * 16, (char*)A + (size_t)m0 * n * n * 16, n * 16, n * 16,
+                             B * n, in_kind, s));
     EigResult R;
     if (fp32) {
-      h.A32.ensure((size_t)B * n * n * sizeof(cplx<float>));
-      launch_convert<double, float>(h.A64.as<cplx<double>>(), h.A32.as<cplx<float>>(), B * n * n, s);
+      h.A32.ensure((size_t)B * ldA * n * sizeof(cplx<float>));
+      launch_convert<double, float>(h.A64.as<cplx<double>>(), h.A32.as<cplx<float>>(), B * ldA * n, s);
       eig_batch<float>(h, B, h.A32.as<cplx<float>>(), h.A64.as<cplx<double>>(), Z != nullptr,
                        opts->vl, opts->vu, R);
     } else {

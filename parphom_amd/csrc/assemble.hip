@@ -122,6 +122,7 @@ struct AsmArgs {
   const double* cd_dc;      // 3 x 3: crystal step for each Cartesian direction (row a)
   double cd_norm;           // |delta|
   int64_t N, nb, mode;
+  int64_t ldD;              // leading dimension of D64 / D32 (>= n; dD uses n)
   cplx<double>* D64;
   cplx<float>* D32;
   cplx<double>* dD64;
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
     }
     __syncthreads();
     const size_t base = (size_t)iq * mstride;
+    const size_t baseD = (size_t)iq * A.ldD * n;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int64_t r = tid + 256 * k;
@@ -203,12 +205,13 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         const size_t idx = base + (size_t)(3 * i0 + r) + (size_t)(3 * j + b) * n;
+        const size_t idxD = baseD + (size_t)(3 * i0 + r) + (size_t)(3 * j + b) * A.ldD;
         const double fv = f[k][b];
         // dij = Phi * phase / multiplicity / sqrt(m_i m_j)
         const double vr = ((fv * p0r) / mult) / smm;
         const double vi = ((fv * p0i) / mult) / smm;
-        if (A.D64) A.D64[idx] = cplx<double>{vr, vi};
-        if (A.D32) A.D32[idx] = cplx<float>{(float)vr, (float)vi};
+        if (A.D64) A.D64[idxD] = cplx<double>{vr, vi};
+        if (A.D32) A.D32[idxD] = cplx<float>{(float)vr, (float)vi};
         if (want_d) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
@@ -254,6 +257,7 @@ void launch_assemble(const AsmParams& P, hipStream_t s) {
   A.phi = P.phi; A.crys = P.crys; A.lat = P.lat; A.rmass = P.rmass; A.mask = P.mask;
   A.q = P.q; A.cd_dc = P.cd_dc; A.cd_norm = P.cd_norm;
   A.N = P.N; A.nb = P.nb; A.mode = P.mode;
+  A.ldD = P.ldD > 0 ? P.ldD : 3 * P.N;
   A.D64 = P.D64; A.D32 = P.D32; A.dD64 = P.dD64; A.dD32 = P.dD32;
   dim3 grid((unsigned)ceil_div(P.N, 256), (unsigned)P.N);
   if (A.mode == 1) hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, s, A);

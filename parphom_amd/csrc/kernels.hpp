@@ -15,6 +15,7 @@ struct AsmParams {
   const double* cd_dc;      // 3 x 3 central-difference crystal steps (row per direction)
   double cd_norm;
   int64_t N, nb, mode;      // mode 0 analytic, 1 central difference
+  int64_t ldD;              // leading dimension of D64 / D32 (0 = n); dD64 / dD32 use n
   cplx<double>* D64;
   cplx<float>* D32;
   cplx<double>* dD64;
@@ -80,7 +81,10 @@ struct TridiagWork {
   double* part; int64_t strideP;      // per-workgroup partials: |x|^2 at [0, n/64 + 1),
   int64_t yvOff;                      //   w^H v (re, im pairs) from part[yvOff]
   int nchunk;                          // hemv column chunks
-};
+  hipEvent_t* hemv_ev;                 // optional: event pairs around the hemv of every
+};                                     // kHemvTimeStride-th column (j = 0, stride, ...)
+
+constexpr int64_t kHemvTimeStride = 8;
 
 template <typename T> void tridiagonalise(const TridiagWork<T>& w, hipStream_t s);
 

@@ -191,6 +191,10 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
 // the number of live columns at each step.  Results go to the accumulator
 // with device atomics.  Blocks of segment 0 also form the correction dots
 // W^H x, V^H x for their rows.
+// Tiles start `sh` rows (and columns) before j + 1, at a multiple of 16: with
+// lda a multiple of 16 every 64-row column segment then covers exactly four
+// 128-byte lines instead of straddling five; local index l = o + sh for
+// S-relative index o, valid for sh <= l < m + sh.
 // most tiles per workgroup along a tile row (FP32: 4, FP64: 12, by measurement)
 template <typename T> constexpr int kHemvSeg = sizeof(T) == 4 ? 4 : 12;
 
@@ -199,40 +203,42 @@ __device__ __forceinline__ T xsel(bool h, T a, T b) { return h ? a : b; }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
-                                                  int seg) {
+                                                  int seg, int sh) {
   __shared__ cplx<T> vseg[kHemvSeg<T> * 64];
   __shared__ cplx<T> red[4][64];
   // odd columns walk the batch backwards: the first workgroups then read the
   // matrices the previous launch read last, which the memory-side cache
   // (256 MB) may still hold
   const int64_t b = (j & 1) ? (int64_t)gridDim.z - 1 - blockIdx.z : (int64_t)blockIdx.z;
-  const int64_t n = w.n, m = n - j - 1;
+  const int64_t n = w.n, m = n - j - 1, mL = m + sh;
+  const int64_t r0 = j + 1 - sh;                      // absolute row / column of local 0
   const int I = blockIdx.x;
   const int J0 = blockIdx.y * seg;
   if (J0 > I) return;
   const int J1 = min(I + 1, J0 + seg);
   double* scal = w.scal + b * w.strideScal;
   const cplx<T>* A = w.A + b * w.strideA;
-  const cplx<T>* S = A + (j + 1) + (j + 1) * w.lda;
+  const cplx<T>* S = A + r0 + r0 * w.lda;
   const cplx<T>* V = w.V + b * w.strideV;
-  cplx<T>* y = w.ypart + b * w.strideY + (j + 1);
+  cplx<T>* y = w.ypart + b * w.strideY + r0;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
-  const int64_t rI = (int64_t)I * 64 + lane;          // this lane's row within S
+  const int64_t rI = (int64_t)I * 64 + lane;          // this lane's local row
+  const bool rowok = rI >= sh && rI < mL;
 
   cplx<T> cur[16], nxt[16];
   auto load_tile = [&](int J, cplx<T>(&dst)[16]) {
 #pragma unroll
     for (int cc = 0; cc < 16; ++cc) {
       const int64_t col = (int64_t)J * 64 + wv * 16 + cc;
-      const bool ok = (rI < m) && (col < m);
-      const cplx<T> a = S[(ok ? rI : 0) + (ok ? col : 0) * w.lda];
+      const bool ok = rowok && col >= sh && col < mL;
+      const cplx<T> a = S[(ok ? rI : sh) + (ok ? col : sh) * w.lda];
       dst[cc] = ok ? a : cplx<T>{0, 0};
     }
   };
   load_tile(J0, nxt);
-  // x = (0, A(j+2:, j)): the reflector's direction before scaling
-  auto xval = [&](int64_t o) -> cplx<T> {
-    return (o == 0 || o >= m) ? cplx<T>{0, 0} : A[(j + 1 + o) + j * w.lda];
+  // x = (0, A(j+2:, j)): the reflector's direction before scaling (local index)
+  auto xval = [&](int64_t l) -> cplx<T> {
+    return (l <= sh || l >= mL) ? cplx<T>{0, 0} : A[(r0 + l) + j * w.lda];
   };
   const cplx<T> vI = xval(rI);
   for (int o = tid; o < (J1 - J0) * 64; o += blockDim.x) vseg[o] = xval((int64_t)J0 * 64 + o);
@@ -300,26 +306,26 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     r1 += __shfl_xor(r1, 1, WAVE);
     i1 += __shfl_xor(i1, 1, WAVE);
     const int64_t cJ = (int64_t)J * 64 + mycol;
-    if ((lane & 3) == 0 && cJ < m) {
+    if ((lane & 3) == 0 && cJ >= sh && cJ < mL) {
       atomicAdd(&y[cJ].x, r1);
       atomicAdd(&y[cJ].y, i1);
     }
   }
   red[wv][lane] = rowacc;
   __syncthreads();
-  if (wv == 0 && rI < m) {
+  if (wv == 0 && rowok) {
     const cplx<T> s = cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane]));
     atomicAdd(&y[rI].x, s.x);
     atomicAdd(&y[rI].y, s.y);
   }
   if (blockIdx.y == 0) {
     // correction dots W^H x, V^H x over this tile row
-    const cplx<T> vr = (rI < m) ? vI : cplx<T>{0, 0};
+    const cplx<T> vr = rowok ? vI : cplx<T>{0, 0};
     for (int k = wv; k < 2 * jj; k += 4) {
       cplx<T> t{0, 0};
-      if (rI < m) {
-        const cplx<T> x = (k < jj) ? weff(w, b, j + 1 + rI, k, p, scal)
-                                   : V[(j + 1 + rI) + (p + k - jj) * n];
+      if (rowok) {
+        const cplx<T> x = (k < jj) ? weff(w, b, r0 + rI, k, p, scal)
+                                   : V[(r0 + rI) + (p + k - jj) * n];
         t = cmulc(x, vr);
       }
       t.x = wave_sum(t.x);
@@ -449,7 +455,9 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
       hipLaunchKernelGGL(k_col<T>, dim3((unsigned)ceil_div(n - j, 64), 1, B), dim3(256), 0, s,
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();
-      const int64_t nt = ceil_div(m, 64);
+      // tiles start at a multiple of 16 rows when lda allows 128-byte alignment
+      const int sh = (w.lda % 16 == 0) ? (int)((j + 1) % 16) : 0;
+      const int64_t nt = ceil_div(m + sh, 64);
       // tiles per workgroup: up to kHemvSeg, fewer once the batch has less
       // than 2048 kHemvSeg lower-triangle tiles, so that small trailing
       // matrices are not left to a few serial chains (measured on 16 x 4764^2
@@ -457,9 +465,12 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
       // best below m ~ 1200)
       const int64_t tiles = nt * (nt + 1) / 2 * (int64_t)B;
       const int seg = (int)std::max<int64_t>(1, std::min<int64_t>(kHemvSeg<T>, tiles / 2048));
+      const bool timed = w.hemv_ev && (j % kHemvTimeStride == 0);
+      if (timed) PPH_HIP(hipEventRecord(w.hemv_ev[2 * (j / kHemvTimeStride)], s));
       hipLaunchKernelGGL(k_hemv_sym<T>, dim3((unsigned)nt, (unsigned)ceil_div(nt, seg), B), dim3(256),
-                         0, s, w, j, jj, p, seg);
+                         0, s, w, j, jj, p, seg, sh);
       PPH_CHECK_LAUNCH();
+      if (timed) PPH_HIP(hipEventRecord(w.hemv_ev[2 * (j / kHemvTimeStride) + 1], s));
       hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 64), 1, B), dim3(256), 0, s,
                          w, j, jj, p);
       PPH_CHECK_LAUNCH();

@@ -39,7 +39,7 @@ def run(input_file: str, precision: str = None, batch: int = None, log=print) ->
                     vel=cfg.comp_vel, vel_method=cfg.vel_method, precision=prec, vl=cfg.vl,
                     vu=cfg.vu, il=cfg.il, iu=cfg.iu, abstol=cfg.abstol, orfac=cfg.orfac,
                     batch=nb) as hd, h5.Output(out_path) as out:
-        chunk = nb if nb > 0 else 16
+        chunk = nb if nb > 0 else 32
         for q0 in range(0, len(q), chunk):
             qs = q[q0:q0 + chunk]
             res = hd.run(qs)

@@ -40,7 +40,8 @@ class PphOut(C.Structure):
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
             "pph_destroy", "pph_last_error", "pph_version", "pph_debug_zgemm", "pph_profile")
 
-PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total")
+PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
+          "hemv_launches")
 
 _lib = None
 
