@@ -133,6 +133,34 @@ int pph_expect(pph_handle* h, int64_t nq, const double* q, int64_t ldq, int32_t 
 int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w, void* Z,
               int64_t* m, int32_t* info, int32_t on_device);
 
+/* Density of states by linear triangulation: the loops of
+ * bz_integration.f90:12-47 (linear_triangulation) with sort_freq (:49-64) and
+ * integrate_triangle (:67-134), the hot part of
+ * moire_phonon_utils.density_of_states (pyphutil.py:227-306).
+ * freq, func: nk x nbands row-major (freq[k*nbands + b], the `eigenvalues`
+ * of each q group, cm^-1); func may be NULL for 1 everywhere (the
+ * reference's default, pyphutil.py:235-236); tri: ntri x 3 row-major,
+ * 0-based indices into the nk rows; E: nE energies, ascending.
+ * dos[i], nd[i]: the reference's dos_loc and ndos_loc summed over all
+ * triangles (before the 1/(3 ntri)/area factors of pyphutil.py:301-302).
+ * Host or device pointers per `on_device`; the call is synchronous. */
+int pph_dos_triangulation(int64_t nk, int64_t nbands, const double* freq, const double* func,
+                          int64_t ntri, const int64_t* tri, int64_t nE, const double* E,
+                          double* dos, double* nd, int32_t device, int32_t on_device);
+
+/* Density of states by smearing, bz_integration.f90:140-199
+ * (standard_integration with gaussian / lorentzian) summed over q-points as
+ * pyphutil.py:311-325 does: dos[i] = sum_q weight[q]/natom sum_b
+ * g(E[i] - freq[q*nbands + b]), method 1 Gaussian, 2 Lorentzian of `width`.
+ * Each q-point's frequencies must be ascending (as parphom.x writes them).
+ * Note: pyphutil passes its array of local weights where the Fortran expects
+ * one number, so each MPI rank of the reference applies the weight of its
+ * first q-point to all of its q-points; this entry applies weight[q] to
+ * q-point q (the same result for uniform weights). */
+int pph_dos_smearing(int32_t method, int64_t nq, int64_t nbands, const double* freq,
+                     const double* weight, double width, int64_t natom, int64_t nE,
+                     const double* E, double* dos, int32_t device, int32_t on_device);
+
 int pph_destroy(pph_handle* h);
 
 /* Last error text for a handle (or the thread's last error when h is NULL). */

@@ -1032,6 +1032,123 @@ int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w,
   PPH_API_CATCH(nullptr)
 }
 
+// ---------------------------------------------------------------- density of states
+namespace {
+// a host or device input array on the device
+struct DevIn {
+  DevBuf buf;
+  const double* p = nullptr;
+  void from(const void* src, size_t bytes, bool on_device, hipStream_t s) {
+    if (on_device || bytes == 0) { p = static_cast<const double*>(src); return; }
+    buf.ensure(bytes);
+    PPH_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, s));
+    p = buf.as<double>();
+  }
+};
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() { PPH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~Stream() { if (s) (void)hipStreamDestroy(s); }
+};
+bool ascending(const double* E, int64_t nE) {
+  for (int64_t i = 1; i < nE; ++i)
+    if (!(E[i - 1] <= E[i])) return false;
+  return true;
+}
+}  // namespace
+
+int pph_dos_triangulation(int64_t nk, int64_t nbands, const double* freq, const double* func,
+                          int64_t ntri, const int64_t* tri, int64_t nE, const double* E, double* dos,
+                          double* nd, int32_t device, int32_t on_device) {
+  if (nk < 0 || nbands < 0 || ntri < 0 || nE < 0)
+    return fail(nullptr, "pph_dos_triangulation: negative size", PPH_ERR_ARG);
+  if (nE > 0 && (!E || !dos || !nd)) return fail(nullptr, "pph_dos_triangulation: E, dos and nd are required", PPH_ERR_ARG);
+  if (ntri > 0 && nbands > 0 && (!freq || !tri))
+    return fail(nullptr, "pph_dos_triangulation: freq and tri are required", PPH_ERR_ARG);
+  if (!on_device) {
+    for (int64_t i = 0; i < 3 * ntri; ++i)
+      if (tri[i] < 0 || tri[i] >= nk) return fail(nullptr, "pph_dos_triangulation: vertex index out of range", PPH_ERR_ARG);
+    if (!ascending(E, nE)) return fail(nullptr, "pph_dos_triangulation: energies must be ascending", PPH_ERR_ARG);
+  }
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(device));
+  Stream st;
+  const hipStream_t s = st.s;
+  const size_t fb = (size_t)nk * nbands * sizeof(double);
+  DevIn df, dg, dE;
+  DevBuf ones, dtri, dout, dstep;
+  df.from(freq, fb, on_device, s);
+  if (func) {
+    dg.from(func, fb, on_device, s);
+  } else {
+    // the reference's default weight function is 1 everywhere
+    std::vector<double> h1((size_t)nk * nbands, 1.0);
+    ones.ensure(std::max<size_t>(fb, 8));
+    PPH_HIP(hipMemcpyAsync(ones.p, h1.data(), fb, hipMemcpyHostToDevice, s));
+    dg.p = ones.as<double>();
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  dE.from(E, nE * sizeof(double), on_device, s);
+  const int64_t* dt = tri;
+  if (!on_device && ntri > 0) {
+    dtri.ensure(3 * ntri * sizeof(int64_t));
+    PPH_HIP(hipMemcpyAsync(dtri.p, tri, 3 * ntri * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    dt = dtri.as<int64_t>();
+  }
+  double* od = dos;
+  double* on = nd;
+  if (!on_device) {
+    dout.ensure(std::max<size_t>(16 * nE, 16));
+    od = dout.as<double>();
+    on = od + nE;
+  }
+  dstep.ensure(std::max<size_t>(8 * nE, 8));
+  launch_dos_lt(df.p, dg.p, nbands, dt, ntri, dE.p, nE, od, on, dstep.as<double>(), s);
+  if (!on_device && nE > 0) {
+    PPH_HIP(hipMemcpyAsync(dos, od, nE * sizeof(double), hipMemcpyDeviceToHost, s));
+    PPH_HIP(hipMemcpyAsync(nd, on, nE * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  PPH_HIP(hipStreamSynchronize(s));
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
+int pph_dos_smearing(int32_t method, int64_t nq, int64_t nbands, const double* freq,
+                     const double* weight, double width, int64_t natom, int64_t nE, const double* E,
+                     double* dos, int32_t device, int32_t on_device) {
+  if (method != 1 && method != 2) return fail(nullptr, "pph_dos_smearing: method must be 1 (Gaussian) or 2 (Lorentzian)", PPH_ERR_ARG);
+  if (nq < 0 || nbands < 0 || nE < 0 || natom < 1 || !(width > 0.0))
+    return fail(nullptr, "pph_dos_smearing: bad sizes, natom or width", PPH_ERR_ARG);
+  if (nE > 0 && (!E || !dos)) return fail(nullptr, "pph_dos_smearing: E and dos are required", PPH_ERR_ARG);
+  if (nq > 0 && nbands > 0 && (!freq || !weight))
+    return fail(nullptr, "pph_dos_smearing: freq and weight are required", PPH_ERR_ARG);
+  if (!on_device) {
+    if (!ascending(E, nE)) return fail(nullptr, "pph_dos_smearing: energies must be ascending", PPH_ERR_ARG);
+    for (int64_t q = 0; q < nq; ++q)
+      if (!ascending(freq + q * nbands, nbands))
+        return fail(nullptr, "pph_dos_smearing: each q-point's frequencies must be ascending", PPH_ERR_ARG);
+  }
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(device));
+  Stream st;
+  const hipStream_t s = st.s;
+  DevIn df, dw, dE;
+  DevBuf dout;
+  df.from(freq, (size_t)nq * nbands * sizeof(double), on_device, s);
+  dw.from(weight, nq * sizeof(double), on_device, s);
+  dE.from(E, nE * sizeof(double), on_device, s);
+  double* od = dos;
+  if (!on_device) {
+    dout.ensure(std::max<size_t>(8 * nE, 8));
+    od = dout.as<double>();
+  }
+  launch_dos_smear(method, df.p, dw.p, nq, nbands, width, natom, dE.p, nE, od, s);
+  if (!on_device && nE > 0) PPH_HIP(hipMemcpyAsync(dos, od, nE * sizeof(double), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
 int pph_destroy(pph_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return PPH_OK;

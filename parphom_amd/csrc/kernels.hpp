@@ -137,4 +137,15 @@ template <typename Ti, typename To>
 void launch_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count, hipStream_t s);
 void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);
 
+// ---------------------------------------------------------------- density of states (dos.hip)
+// freq, func: nk x nb row-major (freq[k * nb + b]); tri: ntri x 3 row-major,
+// 0-based; E ascending; step: nE scratch.  All device pointers.
+void launch_dos_lt(const double* freq, const double* func, int64_t nb, const int64_t* tri,
+                   int64_t ntri, const double* E, int64_t nE, double* dos, double* nd, double* step,
+                   hipStream_t s);
+// method 1 Gaussian, 2 Lorentzian; freq: nq x nb row-major, each row ascending
+void launch_dos_smear(int method, const double* freq, const double* w, int64_t nq, int64_t nb,
+                      double width, int64_t natom, const double* E, int64_t nE, double* dos,
+                      hipStream_t s);
+
 }  // namespace pph

@@ -38,7 +38,8 @@ class PphOut(C.Structure):
 
 
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
-            "pph_destroy", "pph_last_error", "pph_version", "pph_debug_zgemm", "pph_profile")
+            "pph_dos_triangulation", "pph_dos_smearing", "pph_destroy", "pph_last_error",
+            "pph_version", "pph_debug_zgemm", "pph_profile")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
           "hemv_launches")
@@ -74,8 +75,12 @@ def load(path: str = LIB_PATH):
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
+    lib.pph_dos_triangulation.argtypes = [C.c_int64, C.c_int64, _dp, _dp, C.c_int64, _i64p,
+                                          C.c_int64, _dp, _dp, _dp, C.c_int32, C.c_int32]
+    lib.pph_dos_smearing.argtypes = [C.c_int32, C.c_int64, C.c_int64, _dp, _dp, C.c_double,
+                                     C.c_int64, C.c_int64, _dp, _dp, C.c_int32, C.c_int32]
     for name in ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
-                 "pph_destroy", "pph_debug_zgemm"):
+                 "pph_destroy", "pph_debug_zgemm", "pph_dos_triangulation", "pph_dos_smearing"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -253,3 +258,39 @@ def debug_zgemm(A, B, opA="N", opB="N", prec="D", mode=0, Cin=None):
                                A.shape[0], B.ctypes.data, B.shape[0], Cm.ctypes.data, m, mode,
                                _ptr(dot)))
     return Cm, dot
+
+
+def dos_triangulation(freq, tri, E, func=None, device=0):
+    """Linear-triangulation sums (dos, nd) over all triangles and bands, the
+    loops of bz_integration.f90:12-47 on the GPU.  freq, func: (nk, nbands);
+    tri: (ntri, 3) 0-based; E ascending."""
+    lib = load()
+    freq = np.ascontiguousarray(freq, np.float64)
+    nk, nb = freq.shape
+    fu = None if func is None else np.ascontiguousarray(func, np.float64)
+    if fu is not None and fu.shape != freq.shape:
+        raise ValueError("func must have the shape of freq")
+    tri = np.ascontiguousarray(tri, np.int64).reshape(-1, 3)
+    E = np.ascontiguousarray(E, np.float64)
+    dos = np.zeros(len(E))
+    nd = np.zeros(len(E))
+    _check(lib.pph_dos_triangulation(nk, nb, _ptr(freq), None if fu is None else _ptr(fu),
+                                     len(tri), _ptr(tri, C.c_int64), len(E), _ptr(E), _ptr(dos),
+                                     _ptr(nd), device, 0))
+    return dos, nd
+
+
+def dos_smearing(method, freq, weight, width, E, natom, device=0):
+    """sum_q weight[q] / natom sum_b g(E - freq[q, b]) with a Gaussian
+    (method 1 or "gaussian") or Lorentzian (2 or "lorentzian") of `width`,
+    bz_integration.f90:140-199 on the GPU."""
+    lib = load()
+    m = {"gaussian": 1, "lorentzian": 2}.get(method, method)
+    freq = np.ascontiguousarray(np.atleast_2d(freq), np.float64)
+    nq, nb = freq.shape
+    w = np.ascontiguousarray(np.broadcast_to(np.asarray(weight, np.float64), (nq,)))
+    E = np.ascontiguousarray(E, np.float64)
+    dos = np.zeros(len(E))
+    _check(lib.pph_dos_smearing(int(m), nq, nb, _ptr(freq), _ptr(w), float(width), int(natom),
+                                len(E), _ptr(E), _ptr(dos), device, 0))
+    return dos

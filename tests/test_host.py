@@ -159,3 +159,36 @@ def test_synthetic_structures():
     assert abs(synth.tbg_structure(11, 12).theta_deg - 2.88) < 0.01
     q = synth.q_path_gkmg()
     assert q.shape == (61, 3) and np.allclose(q[26], synth.K_POINT)
+
+
+def test_dos_argument_errors():
+    """Checked on the host before any device work."""
+    from parphom_amd import lib
+    freq = np.array([[1.0, 2.0], [1.5, 2.5], [0.5, 3.0]])
+    with pytest.raises(lib.PphError, match="out of range"):
+        lib.dos_triangulation(freq, [[0, 1, 3]], np.arange(0, 4, 0.5))
+    with pytest.raises(lib.PphError, match="ascending"):
+        lib.dos_triangulation(freq, [[0, 1, 2]], np.array([1.0, 0.5]))
+    with pytest.raises(lib.PphError, match="method"):
+        lib.dos_smearing(3, freq, 1.0, 0.1, np.arange(0, 4, 0.5), 1)
+    with pytest.raises(lib.PphError, match="ascending"):
+        lib.dos_smearing(1, freq[:, ::-1], 1.0, 0.1, np.arange(0, 4, 0.5), 1)
+
+
+def test_dos_grid_and_triangles(tmp_path):
+    """Full-grid file parsing and the Delaunay triangles on the pyphutil
+    construction of Cartesian points; every grid cell is split in two."""
+    from parphom_amd import dos
+    lat = np.array([[2.46, 0.0, 0.0], [1.23, 2.130422493309719, 0.0], [0.0, 0.0, 20.0]])
+    m = 4
+    g = np.array([[i / m, j / m, 0.0] for i in range(m) for j in range(m)])
+    p = tmp_path / "grid"
+    with open(p, "w") as f:
+        for r in g:
+            f.write("%.8f %.8f %.8f\n" % tuple(r))
+        f.write(" ".join(str(1 + (k % 5)) for k in range(len(g))) + "\n")
+    grid, mapping = dos.read_full_grid((m, m, 1), str(p))
+    assert np.allclose(grid, g) and mapping.tolist() == [1 + (k % 5) for k in range(len(g))]
+    tri = dos.triangles(lat, (m, m, 1), grid, mapping)
+    assert tri.shape == (2 * (m - 1) ** 2, 3)
+    assert tri.min() >= 0 and tri.max() < 5
