@@ -116,6 +116,90 @@ def cpu_baseline(st, fc, q, seconds_cap=60.0):
     return dt, r
 
 
+def dos_workload(nbands=4764, mesh=18, seed=7):
+    """Transport-DOS shaped workload (the reference's
+    Utilities/Examples/transport_dos.py: 18 x 18 full grid, linear
+    triangulation with |v|^2 weights, energies -2 .. 1800 cm^-1 every 0.025):
+    synthetic flat moire bands, base frequencies uniform in [0, 1600] cm^-1
+    with a nearest-neighbour dispersion of 0.5-3 cm^-1 per band."""
+    from parphom_amd import dos as D
+    rng = np.random.default_rng(seed)
+    base = np.sort(rng.uniform(0.0, 1600.0, nbands))
+    amp = rng.uniform(0.5, 3.0, nbands)
+    g = np.array([[i / mesh, j / mesh, 0.0] for i in range(mesh) for j in range(mesh)])
+    ph = 2 * np.pi * g
+    shape = np.cos(ph[:, 0]) + np.cos(ph[:, 1]) + np.cos(ph[:, 0] + ph[:, 1])
+    freq = base[None, :] + amp[None, :] * shape[:, None]
+    func = 1e6 * (1.0 + 0.5 * np.sin(ph[:, 0])[:, None] * np.cos(base / 50.0)[None, :]) ** 2
+    lat = np.array([[100.0, 0, 0], [50.0, 86.60254037844386, 0], [0, 0, 20.0]])
+    tri = D.triangles(lat, (mesh, mesh, 1), g, np.arange(1, len(g) + 1))
+    E = np.arange(-2.0, 1800.0, 0.025)
+    return freq, func, tri, E
+
+
+def dos_bench(lib, torch, a):
+    """One JSON line for the DOS row: linear triangulation on the GPU with
+    inputs resident in HBM, against the reference routine compiled from its
+    own source (oracle/_ref) on a slice of the energies."""
+    freq, func, tri, E = dos_workload()
+    nk, nb = freq.shape
+    dev = torch.device("cuda:0")
+    tf = torch.from_numpy(np.ascontiguousarray(freq)).to(dev)
+    tg = torch.from_numpy(np.ascontiguousarray(func)).to(dev)
+    tt = torch.from_numpy(np.ascontiguousarray(tri, np.int64)).to(dev)
+    tE = torch.from_numpy(E).to(dev)
+    td = torch.zeros(len(E), dtype=torch.float64, device=dev)
+    tn = torch.zeros(len(E), dtype=torch.float64, device=dev)
+    import ctypes as C
+    P = lambda t: C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
+    Pi = lambda t: C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_int64))
+
+    def run():
+        lib._check(lib.load().pph_dos_triangulation(nk, nb, P(tf), P(tg), len(tri), Pi(tt), len(E),
+                                                    P(tE), P(td), P(tn), 0, 1))
+    for _ in range(a.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    evals = float(len(E)) * len(tri) * nb          # the reference's loop count
+    # energies that fall inside a (triangle, band) window [min f, max f]: the
+    # ones the kernel evaluates (two FP64 atomics each); the rest are known
+    # constants (0 below, the sum of the weights above)
+    fv = freq[tri]                                   # (ntri, 3, nb)
+    inwin = float(np.sum(np.searchsorted(E, fv.max(axis=1), side="right")
+                         - np.searchsorted(E, fv.min(axis=1), side="left")))
+    res = {"metric": "linear-triangulation DOS evaluations/s (E x triangles x bands)",
+           "value": evals / dt, "unit": "evaluations/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic flat bands on an 18 x 18 full grid",
+           "config": {"workload": "transport_dos_18x18", "nk": nk, "nbands": nb,
+                      "triangles": len(tri), "energies": len(E)},
+           "in_window_evaluations": inwin, "in_window_per_s": inwin / dt,
+           "fp64_atomics_per_s": 2 * inwin / dt}
+    # CPU: the reference routine (oracle/_ref, compiled from bz_integration.f90)
+    # on the first 8 energies above 800 cm^-1, all triangles and bands
+    from oracle import dos as OD
+    if OD.ref_available() and not a.no_cpu:
+        i0 = int(np.searchsorted(E, 800.0))
+        Es = E[i0:i0 + 8]
+        t1 = time.perf_counter()
+        d0, n0 = OD.ref_linear_triangulation(tri, Es, freq, func)
+        ct = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": len(Es) * len(tri) * nb / ct, "unit": "evaluations/s",
+                               "cores": 1, "kind": "reference",
+                               "sample": f"{len(Es)} energies x {len(tri)} triangles x {nb} bands"}
+        got_d, got_n = td.cpu().numpy()[i0:i0 + 8], tn.cpu().numpy()[i0:i0 + 8]
+        res["max_rel_diff_vs_reference"] = float(max(
+            np.abs(got_d - d0).max() / max(1e-300, np.abs(d0).max()),
+            np.abs(got_n - n0).max() / max(1e-300, np.abs(n0).max())))
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +209,8 @@ def main():
     ap.add_argument("--workload", default="tbg_11_12")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dos", action="store_true",
+                    help="measure the linear-triangulation DOS kernel instead (see dos_bench)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device 3 times and exit (a copy of known "
                          "size, to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -145,6 +231,9 @@ def main():
         return
     from parphom_amd import lib, synth
     lib.load()
+    if a.dos:
+        dos_bench(lib, torch, a)
+        return
     t0 = time.perf_counter()
     st, fc, qs = synth.workload(a.workload)
     n = 3 * st.natom

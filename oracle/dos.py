@@ -20,7 +20,12 @@ Pins (tests/test_dos_oracle.py):
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
+
 import numpy as np
+
+REF_SO = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "libbzi_ref.so")
 
 # bz_integration.f90:183,192: `double precision, parameter :: PI = 3.141592653589793`
 # is a default-real literal, so PI holds the float32 value of pi
@@ -130,3 +135,51 @@ def lt_normalisation(lat, ntri, natom, nbands):
     area = abs(np.linalg.det(np.asarray(lat)[0:2, 0:2])) * 0.01
     fd = (1 / ntri / 3) / area
     return fd, fd * 3 * natom / nbands
+
+
+# ---------------------------------------------------------------- the compiled reference
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def _ref():
+    """oracle/_ref/libbzi_ref.so: the reference's bz_integration.f90 built by
+    oracle/Makefile with the C entry points of oracle/bzi_ref_wrap.f90."""
+    lib = C.CDLL(REF_SO)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    lib.ref_linear_triangulation.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int, dp, dp, dp,
+                                             dp, dp]
+    lib.ref_standard_integration.argtypes = [C.c_int, C.c_double, C.c_int, dp, C.c_double, C.c_int,
+                                             dp, C.c_int, dp]
+    return lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def ref_linear_triangulation(tri, E, freq, func):
+    """The reference's linear_triangulation on (tri 0-based, E, freq, func)."""
+    lib = _ref()
+    nk, nb = freq.shape
+    t1 = np.asfortranarray(np.asarray(tri, np.int32) + 1)
+    fr = np.asfortranarray(freq)
+    fu = np.asfortranarray(func)
+    E = np.ascontiguousarray(E, np.float64)
+    dos = np.zeros(len(E))
+    nd = np.zeros(len(E))
+    lib.ref_linear_triangulation(nk, nb, len(tri), _p(t1, C.c_int), len(E), _p(E), _p(fr), _p(fu),
+                                 _p(dos), _p(nd))
+    return dos, nd
+
+
+def ref_smearing(method, weight, freq, width, E, natom):
+    """The reference's standard_integration for one q-point."""
+    lib = _ref()
+    fr = np.ascontiguousarray(freq, np.float64)
+    E = np.ascontiguousarray(E, np.float64)
+    out = np.zeros(len(E))
+    lib.ref_standard_integration(method, weight, len(fr), _p(fr), width, len(E), _p(E), natom,
+                                 _p(out))
+    return out

@@ -1,57 +1,25 @@
 """Pins of the DOS oracle (CPU): against the reference's own bz_integration.f90
 compiled into oracle/_ref (skipped when it is not built), and against closed
 forms."""
-import ctypes as C
-import os
-
 import numpy as np
 import pytest
 
 from oracle import dos as OD
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-REF_SO = os.path.join(ROOT, "oracle", "_ref", "libbzi_ref.so")
 
-
-def _ref():
-    if not os.path.exists(REF_SO):
+def _need_ref():
+    if not OD.ref_available():
         pytest.skip("oracle/_ref not built (needs the reference checkout and amdflang)")
-    lib = C.CDLL(REF_SO)
-    dp = C.POINTER(C.c_double)
-    ip = C.POINTER(C.c_int)
-    lib.ref_linear_triangulation.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int, dp, dp, dp,
-                                             dp, dp]
-    lib.ref_standard_integration.argtypes = [C.c_int, C.c_double, C.c_int, dp, C.c_double, C.c_int,
-                                             dp, C.c_int, dp]
-    return lib
-
-
-def _p(a, t=C.c_double):
-    return a.ctypes.data_as(C.POINTER(t))
 
 
 def ref_lt(tri, E, freq, func):
-    lib = _ref()
-    nk, nb = freq.shape
-    t1 = np.asfortranarray(np.asarray(tri, np.int32) + 1)
-    fr = np.asfortranarray(freq)
-    fu = np.asfortranarray(func)
-    E = np.ascontiguousarray(E, np.float64)
-    dos = np.zeros(len(E))
-    nd = np.zeros(len(E))
-    lib.ref_linear_triangulation(nk, nb, len(tri), _p(t1, C.c_int), len(E), _p(E), _p(fr), _p(fu),
-                                 _p(dos), _p(nd))
-    return dos, nd
+    _need_ref()
+    return OD.ref_linear_triangulation(tri, E, freq, func)
 
 
 def ref_smear(method, weight, freq, width, E, natom):
-    lib = _ref()
-    fr = np.ascontiguousarray(freq, np.float64)
-    E = np.ascontiguousarray(E, np.float64)
-    out = np.zeros(len(E))
-    lib.ref_standard_integration(method, weight, len(fr), _p(fr), width, len(E), _p(E), natom,
-                                 _p(out))
-    return out
+    _need_ref()
+    return OD.ref_smearing(method, weight, freq, width, E, natom)
 
 
 def random_case(seed, nk=14, nb=6, ntri=20, ties=True):
