@@ -200,6 +200,55 @@ def dos_bench(lib, torch, a):
     print(json.dumps(res))
 
 
+def fcsym_bench(lib, torch, a, iterations=10):
+    """One JSON line for the force-constant symmetrisation (symmetrize_fc of
+    the reference's producer): `iterations` rounds of the acoustic sum rule
+    and inversion symmetry on the (N, N, 3, 3) force constants of the
+    headline system, resident in HBM."""
+    from parphom_amd import synth
+    st, fc, _ = synth.workload(a.workload)
+    N = st.natom
+    dev = torch.device("cuda:0")
+    t = torch.from_numpy(np.ascontiguousarray(fc)).to(dev)
+    import ctypes as C
+
+    def run():
+        lib._check(lib.load().pph_fc_symmetrize(N, C.cast(C.c_void_p(t.data_ptr()),
+                                                           C.POINTER(C.c_double)),
+                                                iterations, 0, 1))
+    for _ in range(a.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps / iterations      # one iteration
+    # per iteration each step reads and writes the 72 N^2-byte array once
+    alg = 2 * 2 * 72.0 * N * N
+    res = {"metric": "force-constant symmetrisation iterations/s", "value": 1.0 / dt,
+           "unit": "iterations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * dt * iterations, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic spring-model force constants",
+           "config": {"workload": a.workload, "natom": N, "iterations_per_step": iterations},
+           "roofline": {"bound": "hbm", "achieved": alg / dt / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": alg / dt / HBM_PEAK, "traffic": None,
+                        "kernel": "k_fc_asr + k_fc_sym (one iteration)"}}
+    if not a.no_cpu:
+        from oracle import fc as OF
+        t1 = time.perf_counter()
+        ref = OF.symmetrize(fc, 1)
+        ct = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": 1.0 / ct, "unit": "iterations/s", "cores": 1,
+                               "kind": "port", "sample": f"1 iteration at N = {N} (numpy)"}
+        got = t.cpu().numpy()
+        chk = OF.symmetrize(fc, (a.warmup + a.steps) * iterations) if N <= 2000 else None
+        if chk is not None:
+            res["max_rel_diff_vs_oracle"] = float(np.abs(got - chk).max() / np.abs(chk).max())
+        del ref
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +258,8 @@ def main():
     ap.add_argument("--workload", default="tbg_11_12")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fcsym", action="store_true",
+                    help="measure the force-constant symmetrisation instead (see fcsym_bench)")
     ap.add_argument("--dos", action="store_true",
                     help="measure the linear-triangulation DOS kernel instead (see dos_bench)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -233,6 +284,9 @@ def main():
     lib.load()
     if a.dos:
         dos_bench(lib, torch, a)
+        return
+    if a.fcsym:
+        fcsym_bench(lib, torch, a)
         return
     t0 = time.perf_counter()
     st, fc, qs = synth.workload(a.workload)

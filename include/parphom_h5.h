@@ -16,6 +16,9 @@ extern "C" {
 #endif
 int pph_h5_read_fc(const char* path, const char* dset, int64_t natom, double* out);
 int pph_h5_write_fc(const char* path, const char* dset, int64_t natom, const double* fc);
+/* overwrite the existing (natom, natom, 3, 3) dataset in place (the in-place
+ * update symmetrize_fc makes, get_force_constant.py:307-317) */
+int pph_h5_update_fc(const char* path, const char* dset, int64_t natom, const double* fc);
 int pph_h5_open_output(const char* path, int32_t create, int64_t* fid);
 int pph_h5_open_readonly(const char* path, int64_t* fid);
 int pph_h5_write_group(int64_t fid, int64_t qindex, const double* qvec, const double* eval_cm,

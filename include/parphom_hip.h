@@ -161,6 +161,16 @@ int pph_dos_smearing(int32_t method, int64_t nq, int64_t nbands, const double* f
                      const double* weight, double width, int64_t natom, int64_t nE,
                      const double* E, double* dos, int32_t device, int32_t on_device);
 
+/* Force-constant symmetrisation, the iterations of symmetrize_fc
+ * (Force_Constant_Generation/get_force_constant.py:298-348): `iterations`
+ * times, the acoustic sum rule (:320-331, fc[i, :, a, b] -= its mean over
+ * the second atom) and then inversion symmetry (:334-348, fc[i, j, a, b] and
+ * fc[j, i, b, a] both set to their mean), in place on the C-order
+ * (natom, natom, 3, 3) float64 array (the HDF5 `force_constants` dataset).
+ * Host or device pointer per `on_device`; synchronous. */
+int pph_fc_symmetrize(int64_t natom, double* fc, int64_t iterations, int32_t device,
+                      int32_t on_device);
+
 int pph_destroy(pph_handle* h);
 
 /* Last error text for a handle (or the thread's last error when h is NULL). */

@@ -1149,6 +1149,28 @@ int pph_dos_smearing(int32_t method, int64_t nq, int64_t nbands, const double* f
   PPH_API_CATCH(nullptr)
 }
 
+int pph_fc_symmetrize(int64_t natom, double* fc, int64_t iterations, int32_t device,
+                      int32_t on_device) {
+  if (natom < 1 || iterations < 0 || !fc)
+    return fail(nullptr, "pph_fc_symmetrize: bad arguments", PPH_ERR_ARG);
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(device));
+  Stream st;
+  const size_t bytes = (size_t)natom * natom * 9 * sizeof(double);
+  DevBuf buf;
+  double* d = fc;
+  if (!on_device) {
+    buf.ensure(bytes);
+    PPH_HIP(hipMemcpyAsync(buf.p, fc, bytes, hipMemcpyHostToDevice, st.s));
+    d = buf.as<double>();
+  }
+  launch_fc_symmetrize(d, natom, iterations, st.s);
+  if (!on_device) PPH_HIP(hipMemcpyAsync(fc, d, bytes, hipMemcpyDeviceToHost, st.s));
+  PPH_HIP(hipStreamSynchronize(st.s));
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
 int pph_destroy(pph_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return PPH_OK;

@@ -68,6 +68,30 @@ int pph_h5_write_fc(const char* path, const char* dset, int64_t natom, const dou
   return rc;
 }
 
+/* overwrite the (natom, natom, 3, 3) dataset of an existing file in place */
+int pph_h5_update_fc(const char* path, const char* dset, int64_t natom, const double* fc) {
+  quiet();
+  hid_t f = H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT);
+  if (f < 0) return fail("cannot open force-constant file for writing");
+  hid_t d = H5Dopen2(f, dset, H5P_DEFAULT);
+  if (d < 0) { H5Fclose(f); return fail("force-constant dataset not found"); }
+  hid_t s = H5Dget_space(d);
+  hsize_t dims[8];
+  int rc = 0;
+  if (H5Sget_simple_extent_ndims(s) != 4) rc = fail("force-constant dataset must have rank 4");
+  else {
+    H5Sget_simple_extent_dims(s, dims, NULL);
+    if ((int64_t)dims[0] != natom || (int64_t)dims[1] != natom || dims[2] != 3 || dims[3] != 3)
+      rc = fail("force-constant dataset shape is not (natom, natom, 3, 3)");
+    else if (H5Dwrite(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, fc) < 0)
+      rc = fail("writing the force-constant dataset failed");
+  }
+  H5Sclose(s);
+  H5Dclose(d);
+  H5Fclose(f);
+  return rc;
+}
+
 /* create (truncate) or open an output file; returns the file id in *fid */
 int pph_h5_open_output(const char* path, int32_t create, int64_t* fid) {
   quiet();

@@ -143,6 +143,8 @@ void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);
 void launch_dos_lt(const double* freq, const double* func, int64_t nb, const int64_t* tri,
                    int64_t ntri, const double* E, int64_t nE, double* dos, double* nd, double* step,
                    hipStream_t s);
+// acoustic sum rule then (M + M^T)/2, `iterations` times, in place (fcsym.hip)
+void launch_fc_symmetrize(double* fc, int64_t N, int64_t iterations, hipStream_t s);
 // method 1 Gaussian, 2 Lorentzian; freq: nq x nb row-major, each row ascending
 void launch_dos_smear(int method, const double* freq, const double* w, int64_t nq, int64_t nb,
                       double width, int64_t natom, const double* E, int64_t nE, double* dos,

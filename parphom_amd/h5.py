@@ -22,6 +22,7 @@ def load():
         L = C.CDLL(LIB_PATH)
         L.pph_h5_read_fc.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, _dp]
         L.pph_h5_write_fc.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, _dp]
+        L.pph_h5_update_fc.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, _dp]
         L.pph_h5_open_output.argtypes = [C.c_char_p, C.c_int32, _i64p]
         L.pph_h5_open_readonly.argtypes = [C.c_char_p, _i64p]
         L.pph_h5_write_group.argtypes = [C.c_int64, C.c_int64, _dp, _dp, C.c_int64, C.c_void_p,
@@ -54,6 +55,12 @@ def read_fc(path, dset, natom):
 def write_fc(path, dset, fc):
     fc = np.ascontiguousarray(fc, np.float64)
     _chk(load().pph_h5_write_fc(path.encode(), dset.encode(), fc.shape[0], fc.ctypes.data_as(_dp)))
+
+
+def update_fc(path, dset, fc):
+    """Overwrite an existing force-constant dataset in place."""
+    fc = np.ascontiguousarray(fc, np.float64)
+    _chk(load().pph_h5_update_fc(path.encode(), dset.encode(), fc.shape[0], fc.ctypes.data_as(_dp)))
 
 
 class Output:

@@ -38,7 +38,8 @@ class PphOut(C.Structure):
 
 
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
-            "pph_dos_triangulation", "pph_dos_smearing", "pph_destroy", "pph_last_error",
+            "pph_dos_triangulation", "pph_dos_smearing", "pph_fc_symmetrize", "pph_destroy",
+            "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
@@ -79,8 +80,10 @@ def load(path: str = LIB_PATH):
                                           C.c_int64, _dp, _dp, _dp, C.c_int32, C.c_int32]
     lib.pph_dos_smearing.argtypes = [C.c_int32, C.c_int64, C.c_int64, _dp, _dp, C.c_double,
                                      C.c_int64, C.c_int64, _dp, _dp, C.c_int32, C.c_int32]
+    lib.pph_fc_symmetrize.argtypes = [C.c_int64, _dp, C.c_int64, C.c_int32, C.c_int32]
     for name in ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
-                 "pph_destroy", "pph_debug_zgemm", "pph_dos_triangulation", "pph_dos_smearing"):
+                 "pph_destroy", "pph_debug_zgemm", "pph_dos_triangulation", "pph_dos_smearing",
+                 "pph_fc_symmetrize"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -294,3 +297,13 @@ def dos_smearing(method, freq, weight, width, E, natom, device=0):
     _check(lib.pph_dos_smearing(int(m), nq, nb, _ptr(freq), _ptr(w), float(width), int(natom),
                                 len(E), _ptr(E), _ptr(dos), device, 0))
     return dos
+
+
+def fc_symmetrize(fc, iterations, device=0):
+    """symmetrize_fc's iterations (acoustic sum rule, then inversion symmetry)
+    on a (N, N, 3, 3) float64 array on the GPU; returns the new array."""
+    out = np.array(fc, dtype=np.float64, order="C", copy=True)
+    if out.ndim != 4 or out.shape[0] != out.shape[1] or out.shape[2:] != (3, 3):
+        raise ValueError("fc must have shape (N, N, 3, 3)")
+    _check(load().pph_fc_symmetrize(out.shape[0], _ptr(out), int(iterations), device, 0))
+    return out
