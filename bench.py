@@ -45,36 +45,43 @@ def device_outputs(torch, B, n):
     }
 
 
-def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup):
+def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams):
+    """Each step is one pph_run_device call over streams x B q-points: the
+    library keeps `streams` batches of B in flight (one stream and host thread
+    each).  Returns the timed seconds, the phase profile of the timed steps
+    (concurrent batches), the profile of one batch run alone, and the
+    frequencies of the first B q-points."""
     n = 3 * st.natom
     h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision=prec,
-                   batch=B)
-    out = device_outputs(torch, B, n)
+                   batch=B, streams=streams)
+    per_step = B * streams
+    out = device_outputs(torch, per_step, n)
     nq = len(qs)
 
-    def batch(i):
-        idx = [(i * B + k) % nq for k in range(B)]
+    def step_q(i):
+        idx = [(i * per_step + k) % nq for k in range(per_step)]
         return qs[idx]
 
     for i in range(warmup):
-        h.run_device(batch(i), out)
+        h.run_device(step_q(i), out)
         log(f"[bench] {prec} warmup {i + 1}/{warmup} done")
     torch.cuda.synchronize()
     h.profile(enable=True, reset=True)
     t0 = time.perf_counter()
     for i in range(steps):
-        h.run_device(batch(warmup + i), out)
+        h.run_device(step_q(warmup + i), out)
         log(f"[bench] {prec} step {i + 1}/{steps} done")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    prof = h.profile(enable=False)
-    # accuracy check batch (untimed): the first B q-points
+    prof = h.profile(enable=True, reset=True)
+    # one batch alone (untimed; also the accuracy check): the first B q-points
     h.run_device(qs[:B], out)
-    ev = out["eval_cm"].cpu().numpy().copy()
+    prof_iso = h.profile(enable=False)
+    ev = out["eval_cm"][:B].cpu().numpy().copy()
     h.close()
     del out
     torch.cuda.empty_cache()
-    return dt, prof, ev
+    return dt, prof, prof_iso, ev
 
 
 def split_profile(prof, n):
@@ -255,6 +262,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="batches in flight per pph_run call (one stream each)")
     ap.add_argument("--workload", default="tbg_11_12")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -294,19 +303,22 @@ def main():
     log(f"[bench] workload {a.workload}: N={st.natom} n={n} nq={len(qs)} "
         f"(inputs built in {time.perf_counter() - t0:.1f}s)")
 
-    B = a.batch
-    dt32, prof32, ev32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup)
-    qps32 = a.steps * B / dt32
+    B, S = a.batch, a.streams
+    dt32, prof32, iso32, ev32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S)
+    qps32 = a.steps * B * S / dt32
     res = {"metric": "q-pts/s at 3N=4764 eigpairs+v_g: FP32+refine vs FP64, max|dw|<=1e-3 cm^-1",
            "value": qps32, "unit": "q-pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": 1e3 * dt32 / a.steps, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "c64 MFMA (FP32 acc) + f64 refinement",
            "data": "synthetic (seeded spring-model force constants, random q)",
-           "config": {"workload": a.workload, "natom": st.natom, "n": n, "q_per_step": B,
+           "config": {"workload": a.workload, "natom": st.natom, "n": n,
+                      "q_per_step": B * S, "batch": B, "streams": S,
                       "range": "A", "eigenvectors": True, "group_velocity": True,
                       "precision": "S"}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
+    iso, iso_launch_ms = split_profile(iso32, n)
+    res["phase_ms_one_batch_alone"] = {k: round(v, 3) for k, v in iso.items()}
 
     # roofline of the assembly kernel (one launch per batch, HBM-bound):
     # algorithmic bytes = Phi (8 n^2) + B * (D in c128 and c64, dD in c64)
@@ -341,18 +353,28 @@ def main():
     res["dominant_phase"] = dominant
     res["roofline"] = dict(rl["hemv"], kernel="k_hemv_sym<float>",
                            bytes_per_launch=hemv_bytes, avg_launch_us=hemv_launch_ms * 1e3,
-                           ms_per_batch=round(phases["hemv"], 3))
+                           ms_per_batch=round(phases["hemv"], 3),
+                           note=f"timed steps, {S} batches in flight sharing the GPU")
+    iso_s = iso_launch_ms / 1e3
+    res["roofline_one_batch_alone"] = {
+        "bound": "hbm", "achieved": hemv_bytes / iso_s / 1e9, "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s", "frac": hemv_bytes / iso_s / HBM_PEAK, "kernel": "k_hemv_sym<float>",
+        "avg_launch_us": iso_launch_ms * 1e3}
     res["roofline_velocity"] = dict(rl["velocity"],
                                     kernel="velocity products (k_zgemm dot epilogue, FP32)")
     res["roofline_assembly"] = dict(rl["assemble"], kernel="k_assemble")
 
     if not a.no_fp64:
-        dt64, prof64, ev64 = time_path(lib, torch, st, fc, qs, "D", B, max(1, a.steps - 1), 0
-                                          if a.warmup == 0 else 1)
-        qps64 = max(1, a.steps - 1) * B / dt64
+        # FP64 workspaces are larger: half the batch per stream keeps both in HBM
+        B64 = max(1, B // 2) if S > 1 else B
+        dt64, prof64, _, ev64 = time_path(lib, torch, st, fc, qs, "D", B64, max(1, a.steps - 1),
+                                          0 if a.warmup == 0 else 1, S)
+        qps64 = max(1, a.steps - 1) * B64 * S / dt64
+        ev64 = ev64[:min(B, B64)]
+        ev32 = ev32[:min(B, B64)]
         dw = float(np.abs(ev32 - ev64).max())
         ph64, _ = split_profile(prof64, n)
-        res["fp64_path"] = {"value": qps64, "unit": "q-pts/s",
+        res["fp64_path"] = {"value": qps64, "unit": "q-pts/s", "batch": B64, "streams": S,
                             "phase_ms_per_batch": {k: round(v, 3) for k, v in ph64.items()}}
         res["speedup_vs_fp64"] = qps32 / qps64
         res["max_abs_domega_vs_fp64_cm"] = dw

@@ -66,6 +66,10 @@ typedef struct pph_opts {
   int64_t il, iu;         /* range 'I', 1-based inclusive */
   double abstol, orfac;   /* pzheevx ABSTOL / ORFAC */
   int64_t batch;          /* q-points per device batch, 0 = automatic */
+  int32_t streams;        /* batches in flight in pph_run (one stream and host thread
+                             each, so one batch's memory-bound reduction overlaps
+                             another's matrix-core work); 0 = automatic (2 when the
+                             q-points make at least two batches) */
 } pph_opts;
 
 /* Outputs of pph_run (host pointers) or pph_run_device (device pointers).

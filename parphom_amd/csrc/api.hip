@@ -12,6 +12,8 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <exception>
+#include <thread>
 #include <vector>
 
 namespace pph {
@@ -143,9 +145,27 @@ struct Timers {
   }
 };
 
-struct Handle {
-  int device = 0;
+// The working state of one batch in flight: its stream, phase timers and
+// device workspaces.  pph_run keeps several lanes busy at once (one host
+// thread each), so that one batch's memory-bound reduction overlaps
+// another's matrix-core products.
+struct Lane {
+  hipStream_t stream = nullptr;
   Timers tm;
+  DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
+  DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
+  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
+  Lane() = default;
+  Lane(const Lane&) = delete;
+  Lane& operator=(const Lane&) = delete;
+  ~Lane() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// A handle is lane 0 plus the persistent per-system state.
+struct Handle : Lane {
+  int device = 0;
   pph_opts opts{};
   Geometry g;
   int64_t N = 0, n = 0;
@@ -153,21 +173,17 @@ struct Handle {
   // n rounded up to 16 elements, so that columns start on 128-byte lines
   int64_t ldA = 0;
   void set_n(int64_t nn) { n = nn; ldA = (nn + 15) / 16 * 16; }
-  hipStream_t stream = nullptr;
   std::string err;
   // persistent device state
   DevBuf phi, crys, pos, lat, rmass, mask, cd_dc;
   double cd_norm = 0.0;
-  // batch workspaces
-  DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
-  DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
-  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block
   int nchunk = 64;
-  ~Handle() {
-    if (stream) (void)hipStreamDestroy(stream);
-  }
+  // lanes 1.. (created on demand by pph_run)
+  std::vector<std::unique_ptr<Lane>> more;
+  Lane& lane(size_t i) { return i == 0 ? static_cast<Lane&>(*this) : *more[i - 1]; }
+  size_t lanes() const { return 1 + more.size(); }
 };
 
 
@@ -297,7 +313,7 @@ static int64_t range_mcap(const pph_opts& o, int64_t n) {
 // -sum_j |x_j^H r|^2 / ((rho_j - theta) |x_j|^2) from every other computed
 // mode j, all in FP64: W = D X_T, C1 = X^H W, C2 = X^H X_T on the GPU, the
 // k x k algebra on the host.
-static void refine_low_modes(Handle& h, int64_t B, const cplx<double>* D64, const cplx<double>* X,
+static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* D64, const cplx<double>* X,
                              double* lam_dev, const double* nrm2_dev, const std::vector<int64_t>& Mv,
                              int64_t mcap, hipStream_t s) {
   const int64_t n = h.n;
@@ -318,10 +334,10 @@ static void refine_low_modes(Handle& h, int64_t B, const cplx<double>* D64, cons
   }
   if (K == 0) return;
   const int64_t Kp = 16;
-  h.WT.ensure((size_t)B * n * Kp * sizeof(cplx<double>));
-  h.C12.ensure((size_t)B * 2 * mcap * Kp * sizeof(cplx<double>));
-  cplx<double>* WT = h.WT.as<cplx<double>>();
-  cplx<double>* C1 = h.C12.as<cplx<double>>();
+  L.WT.ensure((size_t)B * n * Kp * sizeof(cplx<double>));
+  L.C12.ensure((size_t)B * 2 * mcap * Kp * sizeof(cplx<double>));
+  cplx<double>* WT = L.WT.as<cplx<double>>();
+  cplx<double>* C1 = L.C12.as<cplx<double>>();
   cplx<double>* C2 = C1 + (size_t)B * mcap * Kp;
   int64_t mmax = 0;
   for (auto m : Mv) mmax = std::max(mmax, m);
@@ -410,10 +426,10 @@ static void refine_low_modes(Handle& h, int64_t B, const cplx<double>* D64, cons
 // A: B matrices (n x n, working precision, destroyed); D64: FP64 matrices for
 // the refinement (FP32 path only).  want_vec: caller needs eigenvectors.
 template <typename T>
-static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64, bool want_vec,
+static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<double>* D64, bool want_vec,
                       double vl_l, double vu_l, EigResult& R) {
   const int64_t n = h.n;
-  hipStream_t s = h.stream;
+  hipStream_t s = L.stream;
   const pph_opts& o = h.opts;
   const bool fp32 = std::is_same<T, float>::value;
   const bool vectors = want_vec || fp32;      // the FP32 path refines with its vectors
@@ -422,55 +438,55 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   R.mcap = mcap;
 
   // ---- tridiagonalisation
-  h.V.ensure((size_t)B * n * (n + kVpad) * sizeof(cplx<T>));
-  h.W.ensure((size_t)B * n * nb * sizeof(cplx<T>));
-  h.tau.ensure((size_t)B * n * sizeof(cplx<T>));
-  h.dgl.ensure((size_t)B * n * sizeof(T));
-  h.egl.ensure((size_t)B * n * sizeof(T));
-  h.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
-  h.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
-  h.scal.ensure((size_t)B * 256 * sizeof(double));
+  L.V.ensure((size_t)B * n * (n + kVpad) * sizeof(cplx<T>));
+  L.W.ensure((size_t)B * n * nb * sizeof(cplx<T>));
+  L.tau.ensure((size_t)B * n * sizeof(cplx<T>));
+  L.dgl.ensure((size_t)B * n * sizeof(T));
+  L.egl.ensure((size_t)B * n * sizeof(T));
+  L.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
+  L.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
+  L.scal.ensure((size_t)B * 256 * sizeof(double));
   const int64_t nparts = n / 64 + 2;
-  h.part.ensure((size_t)B * 3 * nparts * sizeof(double));
+  L.part.ensure((size_t)B * 3 * nparts * sizeof(double));
   TridiagWork<T> tw{};
   tw.n = n; tw.lda = h.ldA; tw.batch = B; tw.nb = nb;
   tw.A = A; tw.strideA = h.ldA * n;
-  tw.V = h.V.as<cplx<T>>(); tw.strideV = n * (n + kVpad);
-  tw.W = h.W.as<cplx<T>>(); tw.strideW = n * nb;
-  tw.tau = h.tau.as<cplx<T>>(); tw.strideTau = n;
-  tw.d = h.dgl.as<T>(); tw.e = h.egl.as<T>(); tw.strideDE = n;
-  tw.ypart = h.ypart.as<cplx<T>>(); tw.strideY = n;
-  tw.dpart = h.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
-  tw.scal = h.scal.as<double>(); tw.strideScal = 256;
-  tw.part = h.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
+  tw.V = L.V.as<cplx<T>>(); tw.strideV = n * (n + kVpad);
+  tw.W = L.W.as<cplx<T>>(); tw.strideW = n * nb;
+  tw.tau = L.tau.as<cplx<T>>(); tw.strideTau = n;
+  tw.d = L.dgl.as<T>(); tw.e = L.egl.as<T>(); tw.strideDE = n;
+  tw.ypart = L.ypart.as<cplx<T>>(); tw.strideY = n;
+  tw.dpart = L.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
+  tw.scal = L.scal.as<double>(); tw.strideScal = 256;
+  tw.part = L.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
   tw.nchunk = h.nchunk;
-  tw.hemv_ev = h.tm.kernel_events(ceil_div(n - 1, kHemvTimeStride));
-  h.tm.begin(PH_TRID, s);
+  tw.hemv_ev = L.tm.kernel_events(ceil_div(n - 1, kHemvTimeStride));
+  L.tm.begin(PH_TRID, s);
   tridiagonalise<T>(tw, s);
-  h.tm.end(PH_TRID, s);
+  L.tm.end(PH_TRID, s);
 
   // ---- tridiagonal eigenproblem in FP64
-  h.tdd.ensure((size_t)B * n * sizeof(double));
-  h.tee.ensure((size_t)B * n * sizeof(double));
-  h.te2.ensure((size_t)B * n * sizeof(double));
-  h.tilo.ensure((size_t)B * sizeof(int64_t));
-  h.tihi.ensure((size_t)B * sizeof(int64_t));
-  h.tinfo.ensure((size_t)B * 4 * sizeof(double));
-  h.tw.ensure((size_t)B * mcap * sizeof(double));
-  h.tinf.ensure((size_t)B * sizeof(int));
+  L.tdd.ensure((size_t)B * n * sizeof(double));
+  L.tee.ensure((size_t)B * n * sizeof(double));
+  L.te2.ensure((size_t)B * n * sizeof(double));
+  L.tilo.ensure((size_t)B * sizeof(int64_t));
+  L.tihi.ensure((size_t)B * sizeof(int64_t));
+  L.tinfo.ensure((size_t)B * 4 * sizeof(double));
+  L.tw.ensure((size_t)B * mcap * sizeof(double));
+  L.tinf.ensure((size_t)B * sizeof(int));
   TriEigWork te{};
   te.n = n; te.batch = B;
   te.d = tw.d; te.e = tw.e; te.strideDE = n; te.de_is_float = fp32 ? 1 : 0;
-  te.dd = h.tdd.as<double>(); te.ee = h.tee.as<double>(); te.e2 = h.te2.as<double>();
+  te.dd = L.tdd.as<double>(); te.ee = L.tee.as<double>(); te.e2 = L.te2.as<double>();
   te.strideT = n;
   te.range = o.range; te.vl = vl_l; te.vu = vu_l; te.il = o.il; te.iu = o.iu;
   te.abstol = o.abstol; te.orfac = o.orfac;
-  te.ilo = h.tilo.as<int64_t>(); te.ihi = h.tihi.as<int64_t>();
-  te.tinfo = h.tinfo.as<double>();
-  te.w = h.tw.as<double>(); te.ldw = mcap;
-  te.info = h.tinf.as<int>();
+  te.ilo = L.tilo.as<int64_t>(); te.ihi = L.tihi.as<int64_t>();
+  te.tinfo = L.tinfo.as<double>();
+  te.w = L.tw.as<double>(); te.ldw = mcap;
+  te.info = L.tinf.as<int>();
   te.want_vectors = vectors ? 1 : 0;
-  h.tm.begin(PH_TRIEIG, s);
+  L.tm.begin(PH_TRIEIG, s);
   tri_prepare(te, s);
   std::vector<int64_t> ilo(B), ihi(B);
   PPH_HIP(hipMemcpyAsync(ilo.data(), te.ilo, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -486,67 +502,67 @@ static void eig_batch(Handle& h, int64_t B, cplx<T>* A, const cplx<double>* D64,
   te.mmax = std::max<int64_t>(mmax, 1);
   tri_bisect(te, s);
   R.lam = te.w;
-  if (!(vectors && mmax > 0)) h.tm.end(PH_TRIEIG, s);
+  if (!(vectors && mmax > 0)) L.tm.end(PH_TRIEIG, s);
 
   if (vectors && mmax > 0) {
     const int64_t ldzt = mcap;
-    h.tzt.ensure((size_t)B * n * ldzt * sizeof(double));
-    h.tfac.ensure((size_t)B * 5 * n * ldzt * sizeof(double));
-    te.zt = h.tzt.as<double>(); te.ldzt = ldzt; te.strideZt = n * ldzt;
-    te.fac = h.tfac.as<double>(); te.strideFac = 5 * n * ldzt;
+    L.tzt.ensure((size_t)B * n * ldzt * sizeof(double));
+    L.tfac.ensure((size_t)B * 5 * n * ldzt * sizeof(double));
+    te.zt = L.tzt.as<double>(); te.ldzt = ldzt; te.strideZt = n * ldzt;
+    te.fac = L.tfac.as<double>(); te.strideFac = 5 * n * ldzt;
     tri_vectors(te, s);
-    h.tm.end(PH_TRIEIG, s);
+    L.tm.end(PH_TRIEIG, s);
 
     // ---- back-transformation, in blocks of nbt reflectors
     const int64_t nbt = h.nb_back;
-    h.Z.ensure((size_t)B * n * mcap * sizeof(cplx<T>));
+    L.Z.ensure((size_t)B * n * mcap * sizeof(cplx<T>));
     const int64_t strideWs = (n + nbt) * nbt;
-    h.Tm.ensure((size_t)B * strideWs * sizeof(cplx<T>));
+    L.Tm.ensure((size_t)B * strideWs * sizeof(cplx<T>));
     BackWork<T> bw{};
     bw.n = n; bw.batch = B; bw.nb = nbt; bw.m = mmax;
     bw.V = tw.V; bw.strideV = tw.strideV;
     bw.tau = tw.tau; bw.strideTau = n;
     bw.zt = te.zt; bw.ldzt = ldzt; bw.strideZt = te.strideZt;
-    bw.Z = h.Z.as<cplx<T>>(); bw.ldz = n; bw.strideZ = n * mcap;
-    bw.Tm = h.Tm.as<cplx<T>>(); bw.VT = h.VT.as<cplx<T>>(); bw.Wt = h.Wt.as<cplx<T>>();
+    bw.Z = L.Z.as<cplx<T>>(); bw.ldz = n; bw.strideZ = n * mcap;
+    bw.Tm = L.Tm.as<cplx<T>>(); bw.VT = L.VT.as<cplx<T>>(); bw.Wt = L.Wt.as<cplx<T>>();
     bw.strideWs = std::max<int64_t>(strideWs, std::max<int64_t>(n * nbt, nbt * mcap));
     // Tm/VT/Wt were sized with the same per-matrix stride
-    h.Tm.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
-    h.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
-    h.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
-    bw.Tm = h.Tm.as<cplx<T>>(); bw.VT = h.VT.as<cplx<T>>(); bw.Wt = h.Wt.as<cplx<T>>();
-    h.tm.begin(PH_BACK, s);
+    L.Tm.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
+    L.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
+    L.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
+    bw.Tm = L.Tm.as<cplx<T>>(); bw.VT = L.VT.as<cplx<T>>(); bw.Wt = L.Wt.as<cplx<T>>();
+    L.tm.begin(PH_BACK, s);
     back_transform<T>(bw, s);
-    h.tm.end(PH_BACK, s);
-    R.Zt = h.Z.p;
+    L.tm.end(PH_BACK, s);
+    R.Zt = L.Z.p;
 
     if (fp32) {
       // ---- FP64 refinement: lambda_k = Re(z_k^H D z_k) / |z_k|^2 with D in FP64
-      h.Z64.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
-      h.tm.begin(PH_REFINE, s);
-      launch_convert<float, double>(h.Z.as<cplx<float>>(), h.Z64.as<cplx<double>>(), B * n * mcap, s);
-      h.num.ensure((size_t)B * mcap * sizeof(double));
-      h.den.ensure((size_t)B * mcap * sizeof(double));
-      launch_fill_zero(h.num.p, B * mcap * sizeof(double), s);
+      L.Z64.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
+      L.tm.begin(PH_REFINE, s);
+      launch_convert<float, double>(L.Z.as<cplx<float>>(), L.Z64.as<cplx<double>>(), B * n * mcap, s);
+      L.num.ensure((size_t)B * mcap * sizeof(double));
+      L.den.ensure((size_t)B * mcap * sizeof(double));
+      launch_fill_zero(L.num.p, B * mcap * sizeof(double), s);
       GemmParams<double> g{};
       g.opA = 'N'; g.opB = 'N';
       g.m = n; g.n = mmax; g.k = n;
       g.alpha = cplx<double>{1, 0};
       g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
-      g.B = h.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
+      g.B = L.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
       g.batch = B;
-      g.dot_out = h.num.as<double>(); g.dot_stride = mcap;
+      g.dot_out = L.num.as<double>(); g.dot_stride = mcap;
       g.herm_dot = 1;            // D is Hermitian: lower triangle, half the work
-      g.dotX = h.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
+      g.dotX = L.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
       launch_gemm<double>(g, s);
-      launch_colnorm2<double>(h.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, h.den.as<double>(),
+      launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
                               mcap, s);
-      launch_rq_finish(h.num.as<double>(), h.den.as<double>(), te.w, B, mmax, mcap, s);
-      refine_low_modes(h, B, D64, h.Z64.as<cplx<double>>(), te.w, h.den.as<double>(), R.M, mcap, s);
-      h.tm.end(PH_REFINE, s);
-      R.Z64 = h.Z64.as<cplx<double>>();
+      launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
+      refine_low_modes(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), R.M, mcap, s);
+      L.tm.end(PH_REFINE, s);
+      R.Z64 = L.Z64.as<cplx<double>>();
     } else {
-      R.Z64 = reinterpret_cast<cplx<double>*>(h.Z.p);
+      R.Z64 = reinterpret_cast<cplx<double>*>(L.Z.p);
     }
   }
 }
@@ -558,94 +574,95 @@ struct BatchOut {
   double* vel = nullptr;       // device B x mcap x 3
 };
 
-static void run_batch(Handle& h, int64_t B, const double* q_host_rows /* B x 3 row-major */,
+static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /* B x 3 row-major */,
                       BatchOut& out) {
   const int64_t n = h.n, N = h.N;
-  hipStream_t s = h.stream;
+  hipStream_t s = L.stream;
   const pph_opts& o = h.opts;
   const bool fp32 = (o.precision == 'S');
   const bool vel = o.vel != 0;
   const bool want_vec = vel || o.jobz == 'V';
 
-  h.qdev.ensure((size_t)B * 3 * sizeof(double));
-  PPH_HIP(hipMemcpyAsync(h.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
-  h.A64.ensure((size_t)B * h.ldA * n * sizeof(cplx<double>));
-  if (fp32) h.A32.ensure((size_t)B * h.ldA * n * sizeof(cplx<float>));
-  if (vel) h.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
+  L.qdev.ensure((size_t)B * 3 * sizeof(double));
+  PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+  L.A64.ensure((size_t)B * h.ldA * n * sizeof(cplx<double>));
+  if (fp32) L.A32.ensure((size_t)B * h.ldA * n * sizeof(cplx<float>));
+  if (vel) L.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
 
   AsmParams ap{};
   ap.phi = h.phi.as<double>(); ap.crys = h.crys.as<double>(); ap.lat = h.lat.as<double>();
-  ap.rmass = h.rmass.as<double>(); ap.mask = h.mask.as<uint16_t>(); ap.q = h.qdev.as<double>();
+  ap.rmass = h.rmass.as<double>(); ap.mask = h.mask.as<uint16_t>(); ap.q = L.qdev.as<double>();
   ap.cd_dc = h.cd_dc.as<double>(); ap.cd_norm = h.cd_norm;
   ap.N = N; ap.nb = B; ap.mode = (o.vel_method == 'C') ? 1 : 0;
-  ap.D64 = h.A64.as<cplx<double>>();
-  ap.D32 = fp32 ? h.A32.as<cplx<float>>() : nullptr;
+  ap.D64 = L.A64.as<cplx<double>>();
+  ap.D32 = fp32 ? L.A32.as<cplx<float>>() : nullptr;
   ap.ldD = h.ldA;
   if (vel) {
-    if (fp32) ap.dD32 = h.dD.as<cplx<float>>();
-    else ap.dD64 = h.dD.as<cplx<double>>();
+    if (fp32) ap.dD32 = L.dD.as<cplx<float>>();
+    else ap.dD64 = L.dD.as<cplx<double>>();
   }
-  h.tm.begin(PH_TOTAL, s);
-  h.tm.begin(PH_ASM, s);
+  L.tm.begin(PH_TOTAL, s);
+  L.tm.begin(PH_ASM, s);
   launch_assemble(ap, s);
-  h.tm.end(PH_ASM, s);
+  L.tm.end(PH_ASM, s);
 
   const double K32 = (double)(float)(15.633302f * 33.35641f);
   const double vl_l = o.vl / K32, vu_l = o.vu / K32;
   EigResult& R = out.R;
-  if (fp32) eig_batch<float>(h, B, h.A32.as<cplx<float>>(), h.A64.as<cplx<double>>(), want_vec, vl_l, vu_l, R);
-  else eig_batch<double>(h, B, h.A64.as<cplx<double>>(), nullptr, want_vec, vl_l, vu_l, R);
+  if (fp32) eig_batch<float>(h, L, B, L.A32.as<cplx<float>>(), L.A64.as<cplx<double>>(), want_vec, vl_l, vu_l, R);
+  else eig_batch<double>(h, L, B, L.A64.as<cplx<double>>(), nullptr, want_vec, vl_l, vu_l, R);
 
   const int64_t mcap = R.mcap;
   int64_t mmax = 0;
   for (auto m : R.M) mmax = std::max(mmax, m);
-  h.om.ensure((size_t)B * mcap * sizeof(double));
-  launch_eval_to_cm(R.lam, h.om.as<double>(), B, mmax, mcap, s);
-  out.om = h.om.as<double>();
+  L.om.ensure((size_t)B * mcap * sizeof(double));
+  launch_eval_to_cm(R.lam, L.om.as<double>(), B, mmax, mcap, s);
+  out.om = L.om.as<double>();
 
   if (vel && mmax > 0) {
-    h.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
-    launch_fill_zero(h.vacc.p, B * mcap * 3 * sizeof(double), s);
-    h.tm.begin(PH_VEL, s);
+    L.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
+    launch_fill_zero(L.vacc.p, B * mcap * 3 * sizeof(double), s);
+    L.tm.begin(PH_VEL, s);
     if (fp32) {
       GemmParams<float> g{};
       g.opA = 'N'; g.opB = 'N';
       g.m = n; g.n = mmax; g.k = n;
       g.alpha = cplx<float>{1, 0};
-      g.A = h.dD.as<cplx<float>>(); g.lda = n; g.strideA = 3 * n * n; g.sAin = n * n;
-      g.B = h.Z.as<cplx<float>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
+      g.A = L.dD.as<cplx<float>>(); g.lda = n; g.strideA = 3 * n * n; g.sAin = n * n;
+      g.B = L.Z.as<cplx<float>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
       g.batch = B; g.binner = 3;
-      g.dot_out = h.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
+      g.dot_out = L.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
       g.herm_dot = 1;            // dD/dq is Hermitian: lower triangle, half the work
-      g.dotX = h.Z.as<cplx<float>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
+      g.dotX = L.Z.as<cplx<float>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
       launch_gemm<float>(g, s);
     } else {
       GemmParams<double> g{};
       g.opA = 'N'; g.opB = 'N';
       g.m = n; g.n = mmax; g.k = n;
       g.alpha = cplx<double>{1, 0};
-      g.A = h.dD.as<cplx<double>>(); g.lda = n; g.strideA = 3 * n * n; g.sAin = n * n;
-      g.B = h.Z.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
+      g.A = L.dD.as<cplx<double>>(); g.lda = n; g.strideA = 3 * n * n; g.sAin = n * n;
+      g.B = L.Z.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap; g.sBin = 0;
       g.batch = B; g.binner = 3;
-      g.dot_out = h.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
+      g.dot_out = L.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
       g.herm_dot = 1;            // dD/dq is Hermitian: lower triangle, half the work
-      g.dotX = h.Z.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
+      g.dotX = L.Z.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
       launch_gemm<double>(g, s);
     }
-    h.tm.end(PH_VEL, s);
-    launch_vel_scale(h.vacc.as<double>(), h.om.as<double>(), B, mmax, mcap, mcap, s);
-    out.vel = h.vacc.as<double>() + B * 3 * mcap;
+    L.tm.end(PH_VEL, s);
+    launch_vel_scale(L.vacc.as<double>(), L.om.as<double>(), B, mmax, mcap, mcap, s);
+    out.vel = L.vacc.as<double>() + B * 3 * mcap;
   }
-  h.tm.end(PH_TOTAL, s);
+  L.tm.end(PH_TOTAL, s);
 }
 
 // q-points per device batch when the caller does not fix it: as many as fit
-// in half of the device's free memory (larger batches keep more of the
+// in half of the device's free memory shared by `lanes` batches in flight
+// (larger batches keep more of the
 // per-column launches of the reduction busy), at most 64.  Per matrix: D in
 // FP64 (16 n^2) and FP32 (8 n^2), dD (48 or 96 n^2), Householder vectors,
 // eigenvectors (FP32 + FP64 copies, the FP64 tridiagonal vectors and their
 // LU factors), about 6 n^2 x 16 bytes more.
-static int64_t auto_batch(const Handle& h, int64_t nq) {
+static int64_t auto_batch(const Handle& h, int64_t nq, int64_t lanes) {
   if (h.opts.batch > 0) return std::min(nq, h.opts.batch);
   const int64_t n = h.n;
   const bool s = h.opts.precision == 'S';
@@ -653,7 +670,8 @@ static int64_t auto_batch(const Handle& h, int64_t nq) {
                                       96.0);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)24e9;
-  int64_t b = (int64_t)std::floor(0.5 * (double)free_b / std::max(per, 1.0));
+  int64_t b = (int64_t)std::floor(0.5 * (double)free_b / std::max<int64_t>(1, lanes) /
+                                  std::max(per, 1.0));
   b = std::max<int64_t>(1, std::min<int64_t>(b, 64));
   return std::min(nq, b);
 }
@@ -690,6 +708,7 @@ static int check_opts(const pph_opts* o, std::string& why) {
   if (o->range == 'I' && o->il > o->iu) { why = "range I needs il <= iu"; return -1; }
   if (o->range == 'V' && o->vl > o->vu) { why = "range V needs vl <= vu"; return -1; }
   if (o->range == 'I' && o->il < 1) { why = "range I needs il >= 1"; return -1; }
+  if (o->streams < 0 || o->streams > 8) { why = "streams must be 0..8"; return -1; }
   return 0;
 }
 
@@ -801,51 +820,86 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
     return fail(h, "pph_run: evec requested but jobz = 'N'", PPH_ERR_ARG);
   PPH_API_TRY
   PPH_HIP(hipSetDevice(h->device));
-  hipStream_t s = h->stream;
   const hipMemcpyKind kind = device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  const int64_t bmax = auto_batch(*h, std::max<int64_t>(nq, 1));
-  std::vector<double> qrows;
-  for (int64_t q0 = 0; q0 < nq; q0 += bmax) {
-    const int64_t B = std::min(bmax, nq - q0);
-    qrows.resize(3 * B);
-    for (int64_t b = 0; b < B; ++b)
-      for (int c = 0; c < 3; ++c) qrows[3 * b + c] = q[(q0 + b) + ldq * c];
-    BatchOut bo;
-    run_batch(*h, B, qrows.data(), bo);
-    h->tm.collect();
-    const EigResult& R = bo.R;
-    for (int64_t b = 0; b < B; ++b) {
-      const int64_t iq = q0 + b;
-      const int64_t m = R.M[b];
-      int32_t info = R.info[b];
-      int64_t nz = 0;
-      if (h->opts.jobz == 'V') {
-        nz = m;
-        if (out->evec || out->vel) {
-          if (nz > out->evec_cols) { info |= PPH_INFO_MEMORY; nz = out->evec_cols; }
+  // lanes: opts.streams, or two when the q-points make at least two batches
+  const int64_t want = h->opts.streams > 0 ? h->opts.streams : 2;
+  int64_t bmax = auto_batch(*h, std::max<int64_t>(nq, 1), want);
+  const int64_t nbatch = ceil_div(std::max<int64_t>(nq, 1), bmax);
+  const int64_t nl = std::max<int64_t>(1, std::min<int64_t>(want, nbatch));
+  if (nl == 1 && h->opts.batch <= 0) bmax = auto_batch(*h, std::max<int64_t>(nq, 1), 1);
+  while ((int64_t)h->lanes() < nl) {
+    std::unique_ptr<Lane> ln(new Lane());
+    ln->tm.on = h->tm.on;
+    PPH_HIP(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
+    h->more.push_back(std::move(ln));
+  }
+  // lane l runs batches l, l + nl, ... and copies their outputs on its own stream
+  auto work = [&](int64_t l) {
+    PPH_HIP(hipSetDevice(h->device));
+    Lane& L = h->lane((size_t)l);
+    const hipStream_t s = L.stream;
+    std::vector<double> qrows;
+    for (int64_t q0 = l * bmax; q0 < nq; q0 += nl * bmax) {
+      const int64_t B = std::min(bmax, nq - q0);
+      qrows.resize(3 * B);
+      for (int64_t b = 0; b < B; ++b)
+        for (int c = 0; c < 3; ++c) qrows[3 * b + c] = q[(q0 + b) + ldq * c];
+      BatchOut bo;
+      run_batch(*h, L, B, qrows.data(), bo);
+      L.tm.collect();
+      const EigResult& R = bo.R;
+      std::vector<int32_t> infos(B);
+      std::vector<int64_t> nzs(B);
+      for (int64_t b = 0; b < B; ++b) {
+        const int64_t iq = q0 + b;
+        const int64_t m = R.M[b];
+        int32_t info = R.info[b];
+        int64_t nz = 0;
+        if (h->opts.jobz == 'V') {
+          nz = m;
+          if (out->evec || out->vel) {
+            if (nz > out->evec_cols) { info |= PPH_INFO_MEMORY; nz = out->evec_cols; }
+          }
+        }
+        infos[b] = info;
+        nzs[b] = nz;
+        if (m > 0)
+          PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + b * R.mcap,
+                                 m * sizeof(double), kind, s));
+        if (out->evec && nz > 0)
+          PPH_HIP(hipMemcpyAsync((char*)out->evec + (size_t)iq * n * out->evec_cols * 16,
+                                 R.Z64 + (size_t)b * n * R.mcap, (size_t)nz * n * 16, kind, s));
+        if (out->vel && nz > 0)
+          PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + b * R.mcap * 3,
+                                 (size_t)nz * 3 * sizeof(double), kind, s));
+      }
+      if (device_out) {
+        PPH_HIP(hipMemcpyAsync(out->m + q0, R.M.data(), B * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        PPH_HIP(hipMemcpyAsync(out->info + q0, infos.data(), B * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        if (out->nz) PPH_HIP(hipMemcpyAsync(out->nz + q0, nzs.data(), B * sizeof(int64_t), hipMemcpyHostToDevice, s));
+      } else {
+        for (int64_t b = 0; b < B; ++b) {
+          out->m[q0 + b] = R.M[b];
+          out->info[q0 + b] = infos[b];
+          if (out->nz) out->nz[q0 + b] = nzs[b];
         }
       }
-      if (m > 0)
-        PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + b * R.mcap,
-                               m * sizeof(double), kind, s));
-      if (out->evec && nz > 0)
-        PPH_HIP(hipMemcpyAsync((char*)out->evec + (size_t)iq * n * out->evec_cols * 16,
-                               R.Z64 + (size_t)b * n * R.mcap, (size_t)nz * n * 16, kind, s));
-      if (out->vel && nz > 0)
-        PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + b * R.mcap * 3,
-                               (size_t)nz * 3 * sizeof(double), kind, s));
-      if (device_out) {
-        PPH_HIP(hipMemcpyAsync(out->m + iq, &R.M[b], sizeof(int64_t), hipMemcpyHostToDevice, s));
-        PPH_HIP(hipMemcpyAsync(out->info + iq, &info, sizeof(int32_t), hipMemcpyHostToDevice, s));
-        if (out->nz) PPH_HIP(hipMemcpyAsync(out->nz + iq, &nz, sizeof(int64_t), hipMemcpyHostToDevice, s));
-        PPH_HIP(hipStreamSynchronize(s));
-      } else {
-        out->m[iq] = m;
-        out->info[iq] = info;
-        if (out->nz) out->nz[iq] = nz;
-      }
+      PPH_HIP(hipStreamSynchronize(s));      // the host vectors above outlive the copies
     }
-    PPH_HIP(hipStreamSynchronize(s));
+  };
+  if (nl == 1) {
+    work(0);
+  } else {
+    std::vector<std::exception_ptr> errs((size_t)nl);
+    std::vector<std::thread> th;
+    for (int64_t l = 1; l < nl; ++l)
+      th.emplace_back([&, l] {
+        try { work(l); } catch (...) { errs[(size_t)l] = std::current_exception(); }
+      });
+    try { work(0); } catch (...) { errs[0] = std::current_exception(); }
+    for (auto& t : th) t.join();
+    for (auto& e : errs)
+      if (e) std::rethrow_exception(e);
   }
   return PPH_OK;
   PPH_API_CATCH(h)
@@ -1003,10 +1057,10 @@ int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w,
     if (fp32) {
       h.A32.ensure((size_t)B * ldA * n * sizeof(cplx<float>));
       launch_convert<double, float>(h.A64.as<cplx<double>>(), h.A32.as<cplx<float>>(), B * ldA * n, s);
-      eig_batch<float>(h, B, h.A32.as<cplx<float>>(), h.A64.as<cplx<double>>(), Z != nullptr,
+      eig_batch<float>(h, h, B, h.A32.as<cplx<float>>(), h.A64.as<cplx<double>>(), Z != nullptr,
                        opts->vl, opts->vu, R);
     } else {
-      eig_batch<double>(h, B, h.A64.as<cplx<double>>(), nullptr, Z != nullptr, opts->vl, opts->vu, R);
+      eig_batch<double>(h, h, B, h.A64.as<cplx<double>>(), nullptr, Z != nullptr, opts->vl, opts->vu, R);
     }
     for (int64_t b = 0; b < B; ++b) {
       const int64_t mm = R.M[b];
@@ -1175,7 +1229,8 @@ int pph_destroy(pph_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return PPH_OK;
   (void)hipSetDevice(h->device);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (size_t l = 0; l < h->lanes(); ++l)
+    if (h->lane(l).stream) (void)hipStreamSynchronize(h->lane(l).stream);
   delete h;
   return PPH_OK;
 }
@@ -1189,14 +1244,21 @@ extern "C" int pph_profile(pph_handle* hh, int32_t enable, int32_t reset, double
   if (!h) return fail(nullptr, "pph_profile: handle is NULL", PPH_ERR_ARG);
   PPH_API_TRY
   PPH_HIP(hipSetDevice(h->device));
-  h->tm.init();
+  // summed over the lanes (phases of concurrent batches overlap in time)
   if (ms)
-    for (int p = 0; p < PH_N; ++p) ms[p] = h->tm.ms[p];
+    for (int p = 0; p < PH_N; ++p) ms[p] = 0.0;
   if (counts)
-    for (int p = 0; p < PH_N; ++p) counts[p] = h->tm.cnt[p];
-  if (reset)
-    for (int p = 0; p < PH_N; ++p) { h->tm.ms[p] = 0; h->tm.cnt[p] = 0; }
-  h->tm.on = enable != 0;
+    for (int p = 0; p < PH_N; ++p) counts[p] = 0;
+  for (size_t l = 0; l < h->lanes(); ++l) {
+    Timers& t = h->lane(l).tm;
+    t.init();
+    for (int p = 0; p < PH_N; ++p) {
+      if (ms) ms[p] += t.ms[p];
+      if (counts) counts[p] += t.cnt[p];
+      if (reset) { t.ms[p] = 0; t.cnt[p] = 0; }
+    }
+    t.on = enable != 0;
+  }
   return PPH_OK;
   PPH_API_CATCH(h)
 }

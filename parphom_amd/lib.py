@@ -29,7 +29,8 @@ class PphOpts(C.Structure):
     _fields_ = [("jobz", C.c_char), ("range", C.c_char), ("vel_method", C.c_char),
                 ("precision", C.c_char), ("vel", C.c_int32), ("device", C.c_int32),
                 ("vl", C.c_double), ("vu", C.c_double), ("il", C.c_int64), ("iu", C.c_int64),
-                ("abstol", C.c_double), ("orfac", C.c_double), ("batch", C.c_int64)]
+                ("abstol", C.c_double), ("orfac", C.c_double), ("batch", C.c_int64),
+                ("streams", C.c_int32)]
 
 
 class PphOut(C.Structure):
@@ -107,9 +108,9 @@ def _ptr(a, ctype=C.c_double):
 
 def make_opts(jobz="N", rng="A", vel=False, vel_method="A", precision="D", vl=0.0, vu=0.0,
               il=1, iu=1, abstol=float(np.float32(1e-15)), orfac=float(np.float32(1e-15)),
-              batch=0, device=0) -> PphOpts:
+              batch=0, device=0, streams=0) -> PphOpts:
     return PphOpts(jobz.encode(), rng.encode(), vel_method.encode(), precision.encode(),
-                   1 if vel else 0, device, vl, vu, il, iu, abstol, orfac, batch)
+                   1 if vel else 0, device, vl, vu, il, iu, abstol, orfac, batch, streams)
 
 
 class Handle:

@@ -158,3 +158,39 @@ def test_run_tbg_3_4_path(gpu_available, prec):
             vp_ref = U.vel_projected(ref["evec"], dD, ref["eval_cm"], groups)
             tol = 1e-6 * np.abs(vp_ref) + 1e-9 + noise[:, None]
             assert np.all(np.abs(vp - vp_ref) <= tol)
+
+
+@pytest.mark.parametrize("prec", ["S", "D"])
+def test_run_two_streams(gpu_available, prec):
+    """Several batches in flight (batch 2, two streams, a ragged last batch):
+    the same frequencies as one stream and as the oracle, host and device
+    outputs."""
+    lib = _lib()
+    import torch
+    st, fc = U.structure(1, 2, 11)
+    S = U.oracle_system(1, 2, 11)
+    from parphom_amd import synth
+    qs = synth.q_random(7, 21)
+    runs = {}
+    for streams in (1, 2):
+        with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                        precision=prec, batch=2, streams=streams) as h:
+            runs[streams] = h.run(qs)
+            n = 3 * st.natom
+            dev = torch.device("cuda:0")
+            out = {"eval_cm": torch.zeros((7, n), dtype=torch.float64, device=dev),
+                   "m": torch.zeros(7, dtype=torch.int64, device=dev),
+                   "nz": torch.zeros(7, dtype=torch.int64, device=dev),
+                   "info": torch.zeros(7, dtype=torch.int32, device=dev),
+                   "evec": torch.zeros((7, n, n), dtype=torch.complex128, device=dev),
+                   "vel": torch.zeros((7, n, 3), dtype=torch.float64, device=dev)}
+            h.run_device(qs, out)
+            torch.cuda.synchronize()
+            assert out["m"].cpu().tolist() == [n] * 7
+            assert np.abs(out["eval_cm"].cpu().numpy() - np.stack(runs[streams]["eval_cm"])).max() <= 1e-6
+    for iq, q in enumerate(qs):
+        ref = S.solve(q)["eval_cm"]
+        for streams in (1, 2):
+            assert runs[streams]["info"][iq] == 0
+            assert np.abs(runs[streams]["eval_cm"][iq] - ref).max() <= 1e-3
+        assert np.abs(runs[1]["eval_cm"][iq] - runs[2]["eval_cm"][iq]).max() <= 1e-6
