@@ -178,7 +178,7 @@ struct Handle : Lane {
   DevBuf phi, crys, pos, lat, rmass, mask, cd_dc;
   double cd_norm = 0.0;
   int nb_panel = 32;       // reflectors per reduction panel
-  int nb_back = 128;       // reflectors per back-transformation block
+  int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
   int nchunk = 64;
   // lanes 1.. (created on demand by pph_run)
   std::vector<std::unique_ptr<Lane>> more;
@@ -515,6 +515,8 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
 
     // ---- back-transformation, in blocks of nbt reflectors
     const int64_t nbt = h.nb_back;
+    // the Gram products read V columns up to npanel * nbt, nbt - 1 past the last
+    if (nbt > kVpad) throw std::runtime_error("back-transformation block wider than the V padding");
     L.Z.ensure((size_t)B * n * mcap * sizeof(cplx<T>));
     const int64_t strideWs = (n + nbt) * nbt;
     L.Tm.ensure((size_t)B * strideWs * sizeof(cplx<T>));
