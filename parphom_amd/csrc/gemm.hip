@@ -41,8 +41,12 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 template <typename T, bool CONJA, bool CONJB, bool DOT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_zgemm(GemmParams<T> p) {
   using acc_t = typename mfma<T>::acc;
-  __shared__ cplx<T> As[GBK][GBM];
-  __shared__ cplx<T> Bs[GBK][GBN];
+  // two LDS stages (the next K tile is stored while the current one is
+  // consumed: one barrier per K step); rows padded by 16 bytes so the
+  // k-major stores of 16 lanes fall in distinct banks
+  constexpr int PAD = 16 / (int)sizeof(cplx<T>);
+  __shared__ cplx<T> As[2][GBK][GBM + PAD];
+  __shared__ cplx<T> Bs[2][GBK][GBN + PAD];
 
   const int tid = threadIdx.x;
   const int lane = tid % WAVE, wave = tid / WAVE;
@@ -128,30 +132,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       }
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles = [&](int st) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (!CONJA) { As[tid / 64 + 4 * i][tid % 64] = ra[i]; }
-      else { As[tid % 16][tid / 16 + 16 * i] = ra[i]; }
-      if (!CONJB) { Bs[tid % 16][tid / 16 + 16 * i] = rb[i]; }
-      else { Bs[tid / 64 + 4 * i][tid % 64] = rb[i]; }
+      if (!CONJA) { As[st][tid / 64 + 4 * i][tid % 64] = ra[i]; }
+      else { As[st][tid % 16][tid / 16 + 16 * i] = ra[i]; }
+      if (!CONJB) { Bs[st][tid % 16][tid / 16 + 16 * i] = rb[i]; }
+      else { Bs[st][tid / 64 + 4 * i][tid % 64] = rb[i]; }
     }
   };
 
   load_tiles(kbeg);
+  store_tiles(0);
+  __syncthreads();
+  int st = 0;
   for (int64_t k0 = kbeg; k0 < K; k0 += GBK) {
-    __syncthreads();
-    store_tiles();
-    __syncthreads();
-    if (k0 + GBK < K) load_tiles(k0 + GBK);
+    const bool more = k0 + GBK < K;
+    if (more) load_tiles(k0 + GBK);
 #pragma unroll
     for (int ks = 0; ks < GBK / 4; ++ks) {
       const int kk = ks * 4 + lane / 16;
       cplx<T> a[2], bb[2];
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi) a[mi] = As[kk][wr0 + mi * 16 + lane % 16];
+      for (int mi = 0; mi < 2; ++mi) a[mi] = As[st][kk][wr0 + mi * 16 + lane % 16];
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) bb[ni] = Bs[kk][wc0 + ni * 16 + lane % 16];
+      for (int ni = 0; ni < 2; ++ni) bb[ni] = Bs[st][kk][wc0 + ni * 16 + lane % 16];
       T as_[2] = {}, bs_[2] = {};
       if constexpr (k3M) {
 #pragma unroll
@@ -175,6 +180,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
           }
         }
     }
+    if (more) {
+      // the other stage was last read before the previous barrier
+      store_tiles(st ^ 1);
+      st ^= 1;
+    }
+    __syncthreads();
   }
 
   if (!DOT) {
