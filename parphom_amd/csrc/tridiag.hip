@@ -201,6 +201,48 @@ template <typename T> constexpr int kHemvSeg = sizeof(T) == 4 ? 4 : 12;
 template <typename T>
 __device__ __forceinline__ T xsel(bool h, T a, T b) { return h ? a : b; }
 
+// Cross-lane steps of the butterfly without the LDS crossbar.
+// v_permlane32_swap exchanges lanes 32..63 of x with lanes 0..31 of y, so
+// afterwards x + y holds x(l) + x(l + 32) in lanes 0..31 and y(l - 32) + y(l)
+// in lanes 32..63; v_permlane16_swap does the same for the odd 16-lane rows of
+// x and the even rows of y.  64-bit values go through as two 32-bit halves.
+template <int W32>
+__device__ __forceinline__ void lane_swap(uint32_t& x, uint32_t& y) {
+  auto r = W32 ? __builtin_amdgcn_permlane32_swap(x, y, false, false)
+               : __builtin_amdgcn_permlane16_swap(x, y, false, false);
+  x = r[0];
+  y = r[1];
+}
+template <int W32>
+__device__ __forceinline__ float fold_swap(float x, float y) {
+  uint32_t a = __float_as_uint(x), b = __float_as_uint(y);
+  lane_swap<W32>(a, b);
+  return __uint_as_float(a) + __uint_as_float(b);
+}
+template <int W32>
+__device__ __forceinline__ double fold_swap(double x, double y) {
+  uint64_t a = __double_as_longlong(x), b = __double_as_longlong(y);
+  uint32_t alo = (uint32_t)a, ahi = (uint32_t)(a >> 32), blo = (uint32_t)b, bhi = (uint32_t)(b >> 32);
+  lane_swap<W32>(alo, blo);
+  lane_swap<W32>(ahi, bhi);
+  return __longlong_as_double((long long)(((uint64_t)ahi << 32) | alo)) +
+         __longlong_as_double((long long)(((uint64_t)bhi << 32) | blo));
+}
+// DPP move within 16-lane rows: 0x128 row_ror:8 (partner l ^ 8), 0x141
+// row_half_mirror (partner 7 - l within 8 lanes), 0x4E / 0xB1 quad
+// permutations (partners l ^ 2, l ^ 1)
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const uint64_t a = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)a, CTRL, 0xf, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(a >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
                                                   int seg, int sh) {
@@ -280,31 +322,34 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
       pr[cc] = q.x;
       pi[cc] = q.y;
     }
-    // butterfly reduce-scatter of the 16 column sums over the 64 lanes
+    // butterfly reduce-scatter of the 16 column sums over the 64 lanes: each
+    // step halves the live columns, a lane keeping the half its bit selects
     T r8[8], i8[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      r8[k] = xsel(h5, pr[8 + k], pr[k]) + __shfl_xor(xsel(h5, pr[k], pr[8 + k]), 32, WAVE);
-      i8[k] = xsel(h5, pi[8 + k], pi[k]) + __shfl_xor(xsel(h5, pi[k], pi[8 + k]), 32, WAVE);
+      r8[k] = fold_swap<1>(pr[k], pr[8 + k]);
+      i8[k] = fold_swap<1>(pi[k], pi[8 + k]);
     }
     T r4[4], i4[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      r4[k] = xsel(h4, r8[4 + k], r8[k]) + __shfl_xor(xsel(h4, r8[k], r8[4 + k]), 16, WAVE);
-      i4[k] = xsel(h4, i8[4 + k], i8[k]) + __shfl_xor(xsel(h4, i8[k], i8[4 + k]), 16, WAVE);
+      r4[k] = fold_swap<0>(r8[k], r8[4 + k]);
+      i4[k] = fold_swap<0>(i8[k], i8[4 + k]);
     }
     T r2[2], i2[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      r2[k] = xsel(h3, r4[2 + k], r4[k]) + __shfl_xor(xsel(h3, r4[k], r4[2 + k]), 8, WAVE);
-      i2[k] = xsel(h3, i4[2 + k], i4[k]) + __shfl_xor(xsel(h3, i4[k], i4[2 + k]), 8, WAVE);
+      r2[k] = xsel(h3, r4[2 + k], r4[k]) + dpp<0x128>(xsel(h3, r4[k], r4[2 + k]));
+      i2[k] = xsel(h3, i4[2 + k], i4[k]) + dpp<0x128>(xsel(h3, i4[k], i4[2 + k]));
     }
-    T r1 = xsel(h2, r2[1], r2[0]) + __shfl_xor(xsel(h2, r2[0], r2[1]), 4, WAVE);
-    T i1 = xsel(h2, i2[1], i2[0]) + __shfl_xor(xsel(h2, i2[0], i2[1]), 4, WAVE);
-    r1 += __shfl_xor(r1, 2, WAVE);
-    i1 += __shfl_xor(i1, 2, WAVE);
-    r1 += __shfl_xor(r1, 1, WAVE);
-    i1 += __shfl_xor(i1, 1, WAVE);
+    // the partner 7 - l has the other bit 2, so each quad of lanes still
+    // covers all eight lanes of its group once the quad sums are taken
+    T r1 = xsel(h2, r2[1], r2[0]) + dpp<0x141>(xsel(h2, r2[0], r2[1]));
+    T i1 = xsel(h2, i2[1], i2[0]) + dpp<0x141>(xsel(h2, i2[0], i2[1]));
+    r1 += dpp<0x4E>(r1);
+    i1 += dpp<0x4E>(i1);
+    r1 += dpp<0xB1>(r1);
+    i1 += dpp<0xB1>(i1);
     const int64_t cJ = (int64_t)J * 64 + mycol;
     if ((lane & 3) == 0 && cJ >= sh && cJ < mL) {
       atomicAdd(&y[cJ].x, r1);
