@@ -831,7 +831,9 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
   if (nl == 1 && h->opts.batch <= 0) bmax = auto_batch(*h, std::max<int64_t>(nq, 1), 1);
   while ((int64_t)h->lanes() < nl) {
     std::unique_ptr<Lane> ln(new Lane());
+    // a lane added while profiling is on times its batches like lane 0
     ln->tm.on = h->tm.on;
+    if (ln->tm.on) ln->tm.init();
     PPH_HIP(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
     h->more.push_back(std::move(ln));
   }

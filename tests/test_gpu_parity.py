@@ -194,3 +194,24 @@ def test_run_two_streams(gpu_available, prec):
             assert runs[streams]["info"][iq] == 0
             assert np.abs(runs[streams]["eval_cm"][iq] - ref).max() <= 1e-3
         assert np.abs(runs[1]["eval_cm"][iq] - runs[2]["eval_cm"][iq]).max() <= 1e-6
+
+
+def test_profile_before_first_two_lane_run(gpu_available):
+    """Phase timers switched on before any batch has run: the second lane,
+    created by pph_run, must time its batches too (every batch counted once,
+    every eighth hemv launch of each batch timed)."""
+    lib = _lib()
+    st, fc = U.structure(1, 2, 11)
+    from parphom_amd import synth
+    qs = synth.q_random(7, 22)
+    n = 3 * st.natom
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                    batch=2, streams=2) as h:
+        h.profile(enable=True, reset=True)
+        res = h.run(qs)
+        prof = h.profile(enable=False)
+    assert all(i == 0 for i in res["info"])
+    nbatch = 4                                        # 2 + 2 + 2 + 1 q-points
+    assert prof["total"][1] == nbatch and prof["tridiag"][1] == nbatch
+    assert prof["total"][0] > 0.0
+    assert prof["hemv_launches"][1] == nbatch * len(range(0, n - 1, 8))
