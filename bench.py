@@ -113,6 +113,41 @@ def hemv_traffic(workload, n, B):
     return float(kv["traffic_bytes_per_launch"])
 
 
+def gemm_peak(torch, dtype, size=8192, reps=5):
+    """Dense real GEMM rate of the device's library GEMM (torch.matmul) in
+    FLOP/s: the measured matrix peak the path roofline is also quoted on."""
+    a = torch.randn(size, size, dtype=dtype, device="cuda:0")
+    b = torch.randn(size, size, dtype=dtype, device="cuda:0")
+    torch.matmul(a, b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c = torch.matmul(a, b)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 2.0 * size ** 3 * reps / dt
+
+
+def path_roofline(N, n, B, qps, peak_spec, peak_meas):
+    """Whole-path roofline per q-point: t_ideal = useful flops / matrix peak +
+    assembly bytes / HBM peak, with the useful flops of the textbook path
+    (16/3 n^3 Hermitian tridiagonalisation + 8 n^3 back-transformation of all
+    vectors + 24 n^3 for the three velocity products) and the assembly bytes
+    72 N^2 / B (force constants once per batch) + 4 * 16 n^2 (D and three dD in
+    c128).  Refinement, the tridiagonal eigensolver and the work the Hermitian
+    products save are not counted."""
+    flops = (16.0 / 3.0 + 8.0 + 24.0) * float(n) ** 3
+    k1 = 72.0 * N * N / B + 4 * 16.0 * n * n
+    t_meas = 1.0 / qps
+    t_spec = flops / peak_spec + k1 / HBM_PEAK
+    t_meas_peak = flops / peak_meas + k1 / HBM_PEAK
+    return {"flops_per_q": flops, "assembly_bytes_per_q": k1,
+            "ideal_ms_per_q": 1e3 * t_spec, "measured_ms_per_q": 1e3 * t_meas,
+            "frac": t_spec / t_meas, "frac_vs_measured_peak": t_meas_peak / t_meas}
+
+
 def cpu_baseline(st, fc, q, seconds_cap=60.0):
     """The FP64 oracle (numpy assembly, LAPACK zheevx, numpy GEMMs) on one
     q-point of the same workload, on this host's cores."""
@@ -380,6 +415,14 @@ def main():
                             "phase_ms_per_batch": {k: round(v, 3) for k, v in ph64.items()}}
         res["speedup_vs_fp64"] = qps32 / qps64
         res["max_abs_domega_vs_fp64_cm"] = dw
+        # whole-path roofline of both paths, against the spec-sheet matrix peaks
+        # and against what the library GEMM reaches on this device
+        p32, p64 = gemm_peak(torch, torch.float32), gemm_peak(torch, torch.float64)
+        res["roofline_path"] = {
+            "S": path_roofline(st.natom, n, B, qps32, FP32_PEAK, p32),
+            "D": path_roofline(st.natom, n, B64, qps64, FP64_PEAK, p64),
+            "gemm_peak_measured_tflops": {"fp32": p32 / 1e12, "fp64": p64 / 1e12},
+            "gemm_peak_spec_tflops": {"fp32": FP32_PEAK / 1e12, "fp64": FP64_PEAK / 1e12}}
 
     if not a.no_cpu:
         cdt, r = cpu_baseline(st, fc, qs[0])

@@ -243,6 +243,41 @@ __device__ __forceinline__ double dpp(double v) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// Butterfly reduce-scatter of 16 per-lane complex values over the 64 lanes:
+// each step halves the live values, a lane keeping the half its bit selects.
+// Afterwards the lanes with (lane & 3) == 0 hold the sum of value
+// 8 h5 + 4 h4 + 2 h3 + h2 (hX = bit X of the lane index).
+template <typename T>
+__device__ __forceinline__ void reduce_scatter16(const T (&pr)[16], const T (&pi)[16], bool h3,
+                                                 bool h2, T& r1, T& i1) {
+  T r8[8], i8[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    r8[k] = fold_swap<1>(pr[k], pr[8 + k]);
+    i8[k] = fold_swap<1>(pi[k], pi[8 + k]);
+  }
+  T r4[4], i4[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    r4[k] = fold_swap<0>(r8[k], r8[4 + k]);
+    i4[k] = fold_swap<0>(i8[k], i8[4 + k]);
+  }
+  T r2[2], i2[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    r2[k] = xsel(h3, r4[2 + k], r4[k]) + dpp<0x128>(xsel(h3, r4[k], r4[2 + k]));
+    i2[k] = xsel(h3, i4[2 + k], i4[k]) + dpp<0x128>(xsel(h3, i4[k], i4[2 + k]));
+  }
+  // the partner 7 - l has the other bit 2, so each quad of lanes still
+  // covers all eight lanes of its group once the quad sums are taken
+  r1 = xsel(h2, r2[1], r2[0]) + dpp<0x141>(xsel(h2, r2[0], r2[1]));
+  i1 = xsel(h2, i2[1], i2[0]) + dpp<0x141>(xsel(h2, i2[0], i2[1]));
+  r1 += dpp<0x4E>(r1);
+  i1 += dpp<0x4E>(i1);
+  r1 += dpp<0xB1>(r1);
+  i1 += dpp<0xB1>(i1);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, int jj, int64_t p,
                                                   int seg, int sh) {
@@ -322,34 +357,9 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
       pr[cc] = q.x;
       pi[cc] = q.y;
     }
-    // butterfly reduce-scatter of the 16 column sums over the 64 lanes: each
-    // step halves the live columns, a lane keeping the half its bit selects
-    T r8[8], i8[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      r8[k] = fold_swap<1>(pr[k], pr[8 + k]);
-      i8[k] = fold_swap<1>(pi[k], pi[8 + k]);
-    }
-    T r4[4], i4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      r4[k] = fold_swap<0>(r8[k], r8[4 + k]);
-      i4[k] = fold_swap<0>(i8[k], i8[4 + k]);
-    }
-    T r2[2], i2[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      r2[k] = xsel(h3, r4[2 + k], r4[k]) + dpp<0x128>(xsel(h3, r4[k], r4[2 + k]));
-      i2[k] = xsel(h3, i4[2 + k], i4[k]) + dpp<0x128>(xsel(h3, i4[k], i4[2 + k]));
-    }
-    // the partner 7 - l has the other bit 2, so each quad of lanes still
-    // covers all eight lanes of its group once the quad sums are taken
-    T r1 = xsel(h2, r2[1], r2[0]) + dpp<0x141>(xsel(h2, r2[0], r2[1]));
-    T i1 = xsel(h2, i2[1], i2[0]) + dpp<0x141>(xsel(h2, i2[0], i2[1]));
-    r1 += dpp<0x4E>(r1);
-    i1 += dpp<0x4E>(i1);
-    r1 += dpp<0xB1>(r1);
-    i1 += dpp<0xB1>(i1);
+    // the 16 column sums of the wave, reduced across its lanes
+    T r1, i1;
+    reduce_scatter16(pr, pi, h3, h2, r1, i1);
     const int64_t cJ = (int64_t)J * 64 + mycol;
     if ((lane & 3) == 0 && cJ >= sh && cJ < mL) {
       atomicAdd(&y[cJ].x, r1);
@@ -363,23 +373,31 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     atomicAdd(&y[rI].x, s.x);
     atomicAdd(&y[rI].y, s.y);
   }
-  if (blockIdx.y == 0) {
-    // correction dots W^H x, V^H x over this tile row
+  if (blockIdx.y == 0 && jj > 0) {
+    // correction dots W_eff^H x (k < jj), V^H x (jj <= k < 2 jj) over this
+    // tile row: wave wv takes k = wv + 4 i, i < 16; every load is issued
+    // before the products are reduced together (these workgroups run longest,
+    // and the last batch's ones end the launch)
     const cplx<T> vr = rowok ? vI : cplx<T>{0, 0};
-    for (int k = wv; k < 2 * jj; k += 4) {
-      cplx<T> t{0, 0};
-      if (rowok) {
-        const cplx<T> x = (k < jj) ? weff(w, b, r0 + rI, k, p, scal)
-                                   : V[(r0 + rI) + (p + k - jj) * n];
-        t = cmulc(x, vr);
-      }
-      t.x = wave_sum(t.x);
-      t.y = wave_sum(t.y);
-      if (lane == 0) {
-        cplx<T>* d = w.dpart + b * w.strideDp + k;
-        atomicAdd(&d->x, t.x);
-        atomicAdd(&d->y, t.y);
-      }
+    const int64_t rr = r0 + (rowok ? rI : (int64_t)sh);     // a valid row when masked
+    T dr[16], di[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int k = wv + 4 * i;
+      const bool ok = k < 2 * jj;
+      const int kc = ok ? k : 0;
+      const cplx<T> x = (kc < jj) ? weff(w, b, rr, kc, p, scal) : V[rr + (p + kc - jj) * n];
+      const cplx<T> t = cmulc(x, vr);
+      dr[i] = ok ? t.x : T(0);
+      di[i] = ok ? t.y : T(0);
+    }
+    T r1, i1;
+    reduce_scatter16(dr, di, h3, h2, r1, i1);
+    const int kk = wv + 4 * ((h5 ? 8 : 0) + (h4 ? 4 : 0) + (h3 ? 2 : 0) + (h2 ? 1 : 0));
+    if ((lane & 3) == 0 && kk < 2 * jj) {
+      cplx<T>* d = w.dpart + b * w.strideDp + kk;
+      atomicAdd(&d->x, r1);
+      atomicAdd(&d->y, i1);
     }
   }
 }
