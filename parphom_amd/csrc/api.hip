@@ -507,9 +507,9 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   if (vectors && mmax > 0) {
     const int64_t ldzt = mcap;
     L.tzt.ensure((size_t)B * n * ldzt * sizeof(double));
-    L.tfac.ensure((size_t)B * 5 * n * ldzt * sizeof(double));
+    L.tfac.ensure((size_t)B * tri_fac_doubles(n, ldzt) * sizeof(double));
     te.zt = L.tzt.as<double>(); te.ldzt = ldzt; te.strideZt = n * ldzt;
-    te.fac = L.tfac.as<double>(); te.strideFac = 5 * n * ldzt;
+    te.fac = L.tfac.as<double>(); te.strideFac = tri_fac_doubles(n, ldzt);
     tri_vectors(te, s);
     L.tm.end(PH_TRIEIG, s);
 

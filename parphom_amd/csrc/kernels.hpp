@@ -89,6 +89,10 @@ constexpr int64_t kHemvTimeStride = 8;
 template <typename T> void tridiagonalise(const TridiagWork<T>& w, hipStream_t s);
 
 // ---------------------------------------------------------------- tridiagonal eigensolver (FP64)
+// doubles per matrix for the inverse-iteration factors: u0, u1, l (n x ld
+// each) and a pivot byte per element
+inline int64_t tri_fac_doubles(int64_t n, int64_t ld) { return 3 * n * ld + (n * ld + 7) / 8; }
+
 struct TriEigWork {
   int64_t n, batch;
   const void* d; const void* e; int64_t strideDE; int de_is_float;
@@ -100,7 +104,7 @@ struct TriEigWork {
   double* w; int64_t ldw;                  // eigenvalues: w[b*ldw + k]
   int want_vectors;
   double* zt; int64_t ldzt; int64_t strideZt;   // row-major n x ldzt eigenvectors of T
-  double* fac; int64_t strideFac;               // LU scratch: 4 n x ldzt
+  double* fac; int64_t strideFac;               // LU factors: tri_fac_doubles(n, ldzt)
   int64_t* clus; int64_t strideClus;            // cluster starts
   int* info;                                    // per matrix pzheevx-style bits
   int64_t mmax;                                 // max eigenvalues per matrix (threads launched)

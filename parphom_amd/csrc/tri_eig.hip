@@ -29,6 +29,31 @@ __device__ __forceinline__ int64_t sturm_count(const double* __restrict__ d,
   return c;
 }
 
+// Sturm counts at K shifts in one pass over T: K independent recurrences of
+// the same form as sturm_count, sharing the loads of d and e^2, so their
+// division latencies overlap
+template <int K>
+__device__ __forceinline__ void sturm_counts(const double* __restrict__ d,
+                                             const double* __restrict__ e2, int64_t n,
+                                             const double (&x)[K], double pivmin, int64_t (&c)[K]) {
+  double q[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    q[j] = d[0] - x[j];
+    if (fabs(q[j]) < pivmin) q[j] = -pivmin;
+    c[j] = (q[j] < 0.0);
+  }
+  for (int64_t i = 1; i < n; ++i) {
+    const double di = d[i], ei = e2[i - 1];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      q[j] = (di - x[j]) - ei / q[j];
+      if (fabs(q[j]) < pivmin) q[j] = -pivmin;
+      c[j] += (q[j] < 0.0);
+    }
+  }
+}
+
 // one workgroup per matrix
 __global__ void __launch_bounds__(256) k_tri_prepare(TriEigWork w) {
   __shared__ double sh[8];
@@ -100,6 +125,10 @@ __global__ void __launch_bounds__(256) k_tri_prepare(TriEigWork w) {
 
 // one thread per (matrix, selected eigenvalue); d and e^2 are staged in LDS
 // when the tridiagonal fits (use_lds), so the Sturm recurrences read LDS.
+// Each pass counts at kSects - 1 equally spaced interior points of the
+// current interval and keeps the subinterval holding eigenvalue kk, until its
+// width is within the tolerance of plain bisection.
+constexpr int kSects = 4;
 __global__ void __launch_bounds__(256) k_tri_bisect(TriEigWork w, int use_lds) {
   extern __shared__ double tsh[];
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -128,8 +157,35 @@ __global__ void __launch_bounds__(256) k_tri_bisect(TriEigWork w, int use_lds) {
     if (hi - lo <= tol) break;
     const double mid = 0.5 * (lo + hi);
     if (mid <= lo || mid >= hi) break;
-    if (sturm_count(d, e2, w.n, mid, pivmin) > kk) hi = mid;
-    else lo = mid;
+    double x[kSects - 1];
+    const double h = (hi - lo) / kSects;
+#pragma unroll
+    for (int j = 0; j < kSects - 1; ++j) x[j] = lo + h * (j + 1);
+    if (!(x[0] > lo && x[kSects - 2] < hi)) {
+      // too few representable points left: one bisection step
+      if (sturm_count(d, e2, w.n, mid, pivmin) > kk) hi = mid;
+      else lo = mid;
+      continue;
+    }
+    int64_t c[kSects - 1];
+    sturm_counts<kSects - 1>(d, e2, w.n, x, pivmin, c);
+    // counts grow with x: the new lower end is the last point counting <= kk
+    // eigenvalues below it, the new upper end the first counting more
+    double nlo = lo, nhi = hi;
+#pragma unroll
+    for (int j = 0; j < kSects - 1; ++j)
+      if (c[j] <= kk) nlo = x[j];
+#pragma unroll
+    for (int j = kSects - 2; j >= 0; --j)
+      if (c[j] > kk) nhi = x[j];
+    if (nlo < nhi) {
+      lo = nlo;
+      hi = nhi;
+    } else if (c[0] > kk) {             // counts out of order (rounding): bisect at x[0]
+      hi = x[0];
+    } else {
+      lo = x[0];
+    }
   }
   w.w[b * w.ldw + k] = 0.5 * (lo + hi);
 }
@@ -143,70 +199,115 @@ __device__ __forceinline__ double rand_pm1(uint64_t s) {
   return 2.0 * ((double)(s >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
 }
 
-// LU with partial pivoting of T - sigma I (the DLAGTF scheme); factors stored
-// interleaved: u0, u1, u2, l, piv at fac[(f * n + i) * ld + k].
-__device__ void tri_factor(const double* __restrict__ d, const double* __restrict__ e, int64_t n,
-                           double sigma, double tol, double* __restrict__ fac, int64_t ld,
-                           int64_t k) {
-  double alpha = d[0] - sigma, beta = (n > 1) ? e[0] : 0.0;
-  for (int64_t i = 0; i < n - 1; ++i) {
-    const double c = e[i];                  // subdiagonal (row i+1, col i)
-    const double dn = d[i + 1] - sigma;
-    const double en = (i + 2 < n) ? e[i + 1] : 0.0;
-    double u0, u1, u2, l, piv;
-    if (fabs(alpha) >= fabs(c)) {
-      u0 = alpha; u1 = beta; u2 = 0.0; piv = 0.0;
-      if (fabs(u0) < tol) u0 = (u0 >= 0.0) ? tol : -tol;
-      l = c / u0;
-      alpha = dn - l * u1;
-      beta = en;
-    } else {
-      u0 = c; u1 = dn; u2 = en; piv = 1.0;
-      l = alpha / c;
-      alpha = beta - l * dn;
-      beta = -l * en;
-    }
-    fac[(0 * n + i) * ld + k] = u0;
-    fac[(1 * n + i) * ld + k] = u1;
-    fac[(2 * n + i) * ld + k] = u2;
-    fac[(3 * n + i) * ld + k] = l;
-    fac[(4 * n + i) * ld + k] = piv;
-  }
-  if (fabs(alpha) < tol) alpha = (alpha >= 0.0) ? tol : -tol;
-  fac[(0 * n + (n - 1)) * ld + k] = alpha;
-  fac[(1 * n + (n - 1)) * ld + k] = 0.0;
-  fac[(2 * n + (n - 1)) * ld + k] = 0.0;
+// LU with partial pivoting of T - sigma I (the DLAGTF scheme).  Factors are
+// stored interleaved (row i of vector k at [i * ld + k]): u0 at U0 = fac,
+// u1 at U1 = fac + n ld, the multiplier l at Lm = fac + 2 n ld, and the
+// pivot flag as one byte at P = (uint8_t*)(fac + 3 n ld).  The second
+// superdiagonal of U is e[i + 1] where row i pivoted and 0 elsewhere, so it is
+// read from the tridiagonal instead of being stored.
+struct TriFac {
+  double* u0;
+  double* u1;
+  double* l;
+  uint8_t* piv;
+};
+__device__ __forceinline__ TriFac tri_fac(double* fac, int64_t n, int64_t ld) {
+  return TriFac{fac, fac + n * ld, fac + 2 * n * ld, reinterpret_cast<uint8_t*>(fac + 3 * n * ld)};
 }
 
-// y <- (T - sigma I)^{-1} y in place (y interleaved at y[i * ld + k]); returns max|y|
-__device__ double tri_solve(const double* __restrict__ fac, int64_t n, int64_t ld, int64_t k,
-                            double* __restrict__ y) {
-  // forward: apply row interchanges and multipliers
-  double cur = y[k];
-  for (int64_t i = 0; i < n - 1; ++i) {
-    const double nxt = y[(i + 1) * ld + k];
-    const double l = fac[(3 * n + i) * ld + k];
-    const double piv = fac[(4 * n + i) * ld + k];
-    double bi, bn;
-    if (piv == 0.0) { bi = cur; bn = nxt - l * cur; }
-    else { bi = nxt; bn = cur - l * nxt; }
-    y[i * ld + k] = bi;
-    cur = bn;
-  }
-  y[(n - 1) * ld + k] = cur;
-  // backward substitution with the upper factor (bandwidth 3)
+__device__ __forceinline__ double u2_of(const double* __restrict__ e, int64_t n, int64_t i, bool p) {
+  return (p && i + 2 < n) ? e[i + 1] : 0.0;
+}
+
+// Backward substitution with the upper factor (bandwidth 3) in place; returns
+// max|y|, and with `norm` also sum y^2 and the entry of largest magnitude.
+__device__ __forceinline__ double tri_back(const TriFac& F, const double* __restrict__ e, int64_t n,
+                                           int64_t ld, int64_t k, double* __restrict__ y,
+                                           bool norm, double& ssq, double& big) {
   double y1 = 0.0, y2 = 0.0, mx = 0.0;
+  ssq = 0.0;
+  big = 0.0;
   for (int64_t i = n - 1; i >= 0; --i) {
-    const double u0 = fac[(0 * n + i) * ld + k];
-    const double u1 = fac[(1 * n + i) * ld + k];
-    const double u2 = fac[(2 * n + i) * ld + k];
+    const double u0 = F.u0[i * ld + k];
+    const double u1 = F.u1[i * ld + k];
+    const double u2 = u2_of(e, n, i, F.piv[i * ld + k] != 0);
     const double v = (y[i * ld + k] - u1 * y1 - u2 * y2) / u0;
     y[i * ld + k] = v;
     y2 = y1;
     y1 = v;
     mx = fmax(mx, fabs(v));
+    if (norm) {
+      ssq += v * v;
+      if (fabs(v) > fabs(big)) big = v;
+    }
   }
   return mx;
+}
+
+// Forward elimination (row interchanges and multipliers) of sc * y in place.
+__device__ __forceinline__ void tri_forward(const TriFac& F, int64_t n, int64_t ld, int64_t k,
+                                            double* __restrict__ y, double sc) {
+  double cur = y[k] * sc;
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const double nxt = y[(i + 1) * ld + k] * sc;
+    const double l = F.l[i * ld + k];
+    double bi, bn;
+    if (F.piv[i * ld + k] == 0) { bi = cur; bn = nxt - l * cur; }
+    else { bi = nxt; bn = cur - l * nxt; }
+    y[i * ld + k] = bi;
+    cur = bn;
+  }
+  y[(n - 1) * ld + k] = cur;
+}
+
+// y <- (T - sigma I)^{-1} y in place; returns max|y|
+__device__ double tri_solve(const TriFac& F, const double* __restrict__ e, int64_t n, int64_t ld,
+                            int64_t k, double* __restrict__ y) {
+  tri_forward(F, n, ld, k, y, 1.0);
+  double ssq, big;
+  return tri_back(F, e, n, ld, k, y, false, ssq, big);
+}
+
+// Factor T - sigma I and, in the same pass, eliminate the random start vector
+// (element i = rand_pm1(seed + i)) forward, writing the result to y.
+__device__ void tri_factor_start(const double* __restrict__ d, const double* __restrict__ e,
+                                 int64_t n, double sigma, double tol, const TriFac& F, int64_t ld,
+                                 int64_t k, uint64_t seed, double* __restrict__ y) {
+  double alpha = d[0] - sigma, beta = (n > 1) ? e[0] : 0.0;
+  double cur = rand_pm1(seed);
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const double c = e[i];                  // subdiagonal (row i+1, col i)
+    const double dn = d[i + 1] - sigma;
+    const double en = (i + 2 < n) ? e[i + 1] : 0.0;
+    const double nxt = rand_pm1(seed + (uint64_t)(i + 1));
+    double u0, u1, l, bi, bn;
+    uint8_t piv;
+    if (fabs(alpha) >= fabs(c)) {
+      u0 = alpha; u1 = beta; piv = 0;
+      if (fabs(u0) < tol) u0 = (u0 >= 0.0) ? tol : -tol;
+      l = c / u0;
+      alpha = dn - l * u1;
+      beta = en;
+      bi = cur; bn = nxt - l * cur;
+    } else {
+      u0 = c; u1 = dn; piv = 1;
+      l = alpha / c;
+      alpha = beta - l * dn;
+      beta = -l * en;
+      bi = nxt; bn = cur - l * nxt;
+    }
+    F.u0[i * ld + k] = u0;
+    F.u1[i * ld + k] = u1;
+    F.l[i * ld + k] = l;
+    F.piv[i * ld + k] = piv;
+    y[i * ld + k] = bi;
+    cur = bn;
+  }
+  if (fabs(alpha) < tol) alpha = (alpha >= 0.0) ? tol : -tol;
+  F.u0[(n - 1) * ld + k] = alpha;
+  F.u1[(n - 1) * ld + k] = 0.0;
+  F.piv[(n - 1) * ld + k] = 0;
+  y[(n - 1) * ld + k] = cur;
 }
 
 __device__ double perturbed_shift(const double* __restrict__ wv, int64_t k, double tnorm) {
@@ -245,21 +346,19 @@ __global__ void __launch_bounds__(256) k_tri_inviter(TriEigWork w, int use_lds) 
   double* fac = w.fac + b * w.strideFac;
   const double sigma = perturbed_shift(wv, k, tnorm);
   const double tol = fmax(ULP * tnorm, SAFMIN);
-  tri_factor(d, e, n, sigma, tol, fac, ld, k);
+  const TriFac F = tri_fac(fac, n, ld);
+  // three inverse-iteration solves from a random start; each solve's scaling
+  // by 1 / max|y| is applied as the next forward pass reads y
   const uint64_t seed = (uint64_t)(w.ilo[b] + k) * 0x100000001B3ull;
-  for (int64_t i = 0; i < n; ++i) y[i * ld + k] = rand_pm1(seed + (uint64_t)i);
-  for (int it = 0; it < 3; ++it) {
-    const double mx = tri_solve(fac, n, ld, k, y);
-    const double sc = (mx > 0.0) ? 1.0 / mx : 1.0;
-    for (int64_t i = 0; i < n; ++i) y[i * ld + k] *= sc;
+  tri_factor_start(d, e, n, sigma, tol, F, ld, k, seed, y);
+  double ssq = 0.0, big = 0.0;
+  double mx = tri_back(F, e, n, ld, k, y, false, ssq, big);
+  for (int it = 1; it < 3; ++it) {
+    tri_forward(F, n, ld, k, y, (mx > 0.0) ? 1.0 / mx : 1.0);
+    mx = tri_back(F, e, n, ld, k, y, it == 2, ssq, big);
   }
-  double nrm = 0.0, big = 0.0;
-  for (int64_t i = 0; i < n; ++i) {
-    const double v = y[i * ld + k];
-    nrm += v * v;
-    if (fabs(v) > fabs(big)) big = v;
-  }
-  double sc = 1.0 / sqrt(nrm);
+  // unit norm, largest entry positive
+  double sc = 1.0 / sqrt(ssq);
   if (big < 0.0) sc = -sc;
   for (int64_t i = 0; i < n; ++i) y[i * ld + k] *= sc;
 }
@@ -282,7 +381,8 @@ __global__ void __launch_bounds__(256) k_tri_cluster(TriEigWork w, double ctol_r
   while (k1 + 1 < M && wv[k1 + 1] - wv[k1] <= ctol) ++k1;      // members k0..k1
   const int64_t n = w.n, ld = w.ldzt;
   double* y = w.zt + b * w.strideZt;
-  const double* fac = w.fac + b * w.strideFac;
+  const TriFac F = tri_fac(w.fac + b * w.strideFac, n, ld);
+  const double* e = w.ee + b * w.strideT;
   for (int64_t k = k0; k <= k1; ++k) {
     const uint64_t seed = (uint64_t)(w.ilo[b] + k) * 0x100000001B3ull + 0x5bd1e995ull;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i * ld + k] = rand_pm1(seed + (uint64_t)i);
@@ -298,7 +398,7 @@ __global__ void __launch_bounds__(256) k_tri_cluster(TriEigWork w, double ctol_r
         }
       if (it == 3) break;
       __syncthreads();
-      if (threadIdx.x == 0) smx = tri_solve(fac, n, ld, k, y);
+      if (threadIdx.x == 0) smx = tri_solve(F, e, n, ld, k, y);
       __syncthreads();
       const double sc = (smx > 0.0) ? 1.0 / smx : 1.0;
       for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i * ld + k] *= sc;
