@@ -5,6 +5,7 @@
 #include "../../include/parphom_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <complex>
@@ -155,10 +156,25 @@ struct Lane {
   DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
+  // rocBLAS handle for the plain GEMMs of the back-transformation (made on
+  // first use, bound to this lane's stream, atomics off so results repeat)
+  rocblas_handle blas = nullptr;
+  rocblas_handle blas_on(hipStream_t s) {
+    if (!blas) {
+      if (rocblas_create_handle(&blas) != rocblas_status_success)
+        throw std::runtime_error("rocblas_create_handle failed");
+      if (rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
+        throw std::runtime_error("rocblas_set_atomics_mode failed");
+    }
+    if (rocblas_set_stream(blas, s) != rocblas_status_success)
+      throw std::runtime_error("rocblas_set_stream failed");
+    return blas;
+  }
   Lane() = default;
   Lane(const Lane&) = delete;
   Lane& operator=(const Lane&) = delete;
   ~Lane() {
+    if (blas) (void)rocblas_destroy_handle(blas);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -533,6 +549,7 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     L.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     L.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     bw.Tm = L.Tm.as<cplx<T>>(); bw.VT = L.VT.as<cplx<T>>(); bw.Wt = L.Wt.as<cplx<T>>();
+    bw.blas = L.blas_on(s);
     L.tm.begin(PH_BACK, s);
     back_transform<T>(bw, s);
     L.tm.end(PH_BACK, s);

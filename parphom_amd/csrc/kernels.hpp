@@ -123,6 +123,9 @@ struct BackWork {
   const double* zt; int64_t ldzt; int64_t strideZt;   // row-major T eigenvectors
   cplx<T>* Z; int64_t ldz; int64_t strideZ;            // output n x m column-major
   cplx<T>* Tm; cplx<T>* VT; cplx<T>* Wt; int64_t strideWs;   // workspaces
+  // rocblas_handle bound to the stream: the two plain products of every
+  // panel go to rocBLAS's GEMM (null: the library's own GEMM)
+  void* blas;
 };
 
 template <typename T> void back_transform(const BackWork<T>& w, hipStream_t s);
