@@ -32,6 +32,12 @@ int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t
  * the accumulated milliseconds and counts; reset != 0 clears them; enable
  * turns recording on or off. */
 int pph_profile(void* h, int32_t enable, int32_t reset, double* ms, int64_t* counts);
+/* Blocking of the reduction and the back-transformation on this handle:
+ * nb_panel reflectors per reduction panel (16, 32, 48 or 64) and nb_back
+ * reflectors per back-transformation block (a multiple of 16 in [16, 128]).
+ * The defaults (32, 128) are the measured best at n = 4764; the others are
+ * supported so the kernels can be checked at every blocking. */
+int pph_debug_blocking(void* h, int32_t nb_panel, int32_t nb_back);
 
 #ifdef __cplusplus
 }

@@ -1259,6 +1259,18 @@ int pph_destroy(pph_handle* hh) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- measurement hook
+extern "C" int pph_debug_blocking(pph_handle* hh, int32_t nb_panel, int32_t nb_back) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_debug_blocking: handle is NULL", PPH_ERR_ARG);
+  if (nb_panel < 16 || nb_panel > 64 || nb_panel % 16 != 0)
+    return fail(h, "pph_debug_blocking: nb_panel must be 16, 32, 48 or 64", PPH_ERR_ARG);
+  if (nb_back < 16 || nb_back > kVpad || nb_back % 16 != 0)
+    return fail(h, "pph_debug_blocking: nb_back must be a multiple of 16 in [16, 128]", PPH_ERR_ARG);
+  h->nb_panel = nb_panel;
+  h->nb_back = nb_back;
+  return PPH_OK;
+}
+
 extern "C" int pph_profile(pph_handle* hh, int32_t enable, int32_t reset, double* ms,
                            int64_t* counts) {
   Handle* h = reinterpret_cast<Handle*>(hh);

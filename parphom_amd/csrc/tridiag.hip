@@ -375,29 +375,32 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
   }
   if (blockIdx.y == 0 && jj > 0) {
     // correction dots W_eff^H x (k < jj), V^H x (jj <= k < 2 jj) over this
-    // tile row: wave wv takes k = wv + 4 i, i < 16; every load is issued
-    // before the products are reduced together (these workgroups run longest,
-    // and the last batch's ones end the launch)
+    // tile row, 64 at a time (2 jj < 128): wave wv takes k = k0 + wv + 4 i,
+    // i < 16; every load of a chunk is issued before its products are reduced
+    // together (these workgroups run longest, and the last batch's ones end
+    // the launch)
     const cplx<T> vr = rowok ? vI : cplx<T>{0, 0};
     const int64_t rr = r0 + (rowok ? rI : (int64_t)sh);     // a valid row when masked
-    T dr[16], di[16];
+    for (int k0 = 0; k0 < 2 * jj; k0 += 64) {
+      T dr[16], di[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int k = wv + 4 * i;
-      const bool ok = k < 2 * jj;
-      const int kc = ok ? k : 0;
-      const cplx<T> x = (kc < jj) ? weff(w, b, rr, kc, p, scal) : V[rr + (p + kc - jj) * n];
-      const cplx<T> t = cmulc(x, vr);
-      dr[i] = ok ? t.x : T(0);
-      di[i] = ok ? t.y : T(0);
-    }
-    T r1, i1;
-    reduce_scatter16(dr, di, h3, h2, r1, i1);
-    const int kk = wv + 4 * ((h5 ? 8 : 0) + (h4 ? 4 : 0) + (h3 ? 2 : 0) + (h2 ? 1 : 0));
-    if ((lane & 3) == 0 && kk < 2 * jj) {
-      cplx<T>* d = w.dpart + b * w.strideDp + kk;
-      atomicAdd(&d->x, r1);
-      atomicAdd(&d->y, i1);
+      for (int i = 0; i < 16; ++i) {
+        const int k = k0 + wv + 4 * i;
+        const bool ok = k < 2 * jj;
+        const int kc = ok ? k : 0;
+        const cplx<T> x = (kc < jj) ? weff(w, b, rr, kc, p, scal) : V[rr + (p + kc - jj) * n];
+        const cplx<T> t = cmulc(x, vr);
+        dr[i] = ok ? t.x : T(0);
+        di[i] = ok ? t.y : T(0);
+      }
+      T r1, i1;
+      reduce_scatter16(dr, di, h3, h2, r1, i1);
+      const int kk = k0 + wv + 4 * ((h5 ? 8 : 0) + (h4 ? 4 : 0) + (h3 ? 2 : 0) + (h2 ? 1 : 0));
+      if ((lane & 3) == 0 && kk < 2 * jj) {
+        cplx<T>* d = w.dpart + b * w.strideDp + kk;
+        atomicAdd(&d->x, r1);
+        atomicAdd(&d->y, i1);
+      }
     }
   }
 }

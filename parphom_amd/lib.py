@@ -41,7 +41,7 @@ class PphOut(C.Structure):
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
             "pph_dos_triangulation", "pph_dos_smearing", "pph_fc_symmetrize", "pph_destroy",
             "pph_last_error",
-            "pph_version", "pph_debug_zgemm", "pph_profile")
+            "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
           "hemv_launches")
@@ -74,6 +74,8 @@ def load(path: str = LIB_PATH):
     lib.pph_expect.restype = C.c_int
     lib.pph_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _i64p]
     lib.pph_profile.restype = C.c_int
+    lib.pph_debug_blocking.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.pph_debug_blocking.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -201,6 +203,11 @@ class Handle:
                      C.c_void_p(t["evec"].data_ptr()) if t.get("evec") is not None else None,
                      cols, dp(t.get("vel")), dp(t["info"], C.c_int32))
         _check(lib.pph_run_device(self._h, nq, _ptr(q), nq, C.byref(out)), self._h)
+
+    def blocking(self, nb_panel=32, nb_back=128):
+        """Reflectors per reduction panel and per back-transformation block
+        (a test hook: the defaults are the measured best)."""
+        _check(load().pph_debug_blocking(self._h, nb_panel, nb_back), self._h)
 
     def profile(self, enable=True, reset=False):
         """Accumulated phase times (ms) and batch counts, then set recording."""

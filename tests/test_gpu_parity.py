@@ -160,6 +160,45 @@ def test_run_tbg_3_4_path(gpu_available, prec):
             assert np.all(np.abs(vp - vp_ref) <= tol)
 
 
+@pytest.mark.parametrize("prec", ["D", "S"])
+@pytest.mark.parametrize("nb_panel,nb_back", [(16, 16), (48, 64), (64, 128), (32, 48)])
+def test_run_every_blocking(gpu_available, prec, nb_panel, nb_back):
+    """The reduction and back-transformation at every supported blocking
+    (panels of 16..64 reflectors, back-transformation blocks of 16..128):
+    frequencies against the oracle and orthonormal eigenvectors with small
+    residuals, on the config-1 structure (n = 444: several panels, a ragged
+    last one)."""
+    lib = _lib()
+    st, fc = U.structure(3, 4, 1003)
+    S = U.oracle_system(3, 4, 1003)
+    from parphom_amd import synth
+    qs = synth.q_path_gkmg()[[0, 13, 30]]
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                    precision=prec) as h:
+        h.blocking(nb_panel, nb_back)
+        res = h.run(qs)
+    n = 3 * st.natom
+    for iq, q in enumerate(qs):
+        ref = S.solve(q)
+        assert res["info"][iq] == 0
+        assert np.abs(res["eval_cm"][iq] - ref["eval_cm"]).max() <= 1e-3
+        Z = np.asarray(res["evec"][iq])
+        D = S.D(q)
+        nrm = np.abs(ref["lam"]).max()
+        tol = 1e-10 if prec == "D" else 1e-5
+        assert np.linalg.norm(D @ Z - Z * ref["lam"], axis=0).max() <= tol * nrm
+        assert np.abs(Z.conj().T @ Z - np.eye(n)).max() <= (1e-10 if prec == "D" else 1e-5)
+
+
+def test_blocking_rejects_unsupported(gpu_available):
+    lib = _lib()
+    st, fc = U.structure(1, 2, 11)
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc) as h:
+        for bad in ((8, 128), (80, 128), (40, 128), (32, 256), (32, 24)):
+            with pytest.raises(lib.PphError):
+                h.blocking(*bad)
+
+
 @pytest.mark.parametrize("prec", ["S", "D"])
 def test_run_two_streams(gpu_available, prec):
     """Several batches in flight (batch 2, two streams, a ragged last batch):
