@@ -1,13 +1,14 @@
 // Back-transformation of tridiagonal eigenvectors, Z = Q Z_T with
 // Q = H_0 H_1 ... H_{n-2}, applied one panel of reflectors at a time in
 // compact-WY form Q_p = I - V_p T_p V_p^H (the PZUNMTR stage of pzheevx):
-//   Wt = V_p^H Z          (rocBLAS GEMM; or split-K on the library's own)
+//   Wt = V_p^H Z          (split-K on the library's GEMM)
 //   VT = V_p T_p
 //   Z  = Z - VT Wt        (rocBLAS GEMM; or the library's own)
 // T_p for every panel is formed up front from the Gram matrices V_p^H V_p.
-// The two large products are plain GEMMs with no fused epilogue, the shapes
-// where rocBLAS's tuned kernels beat the library's 64 x 64 tile (measured at
-// n = 4764, batch 32: 116 vs 96 and 126 vs 106 TFLOP/s).
+// The update of Z is a plain GEMM with K = 128 and no fused epilogue, the
+// one shape here where rocBLAS's tuned kernels beat the library's 64 x 64
+// tile (measured at n = 4764, batch 32: 114 vs 102 TFLOP/s; for Wt the
+// library's kernel is the faster, 140 vs 127).
 #include "common.hpp"
 #include "kernels.hpp"
 #include <rocblas/rocblas.h>
@@ -138,10 +139,7 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
     const int64_t kw = ceil_div(pw, 16) * 16;
     rocblas_handle blas = reinterpret_cast<rocblas_handle>(w.blas);
     // Wt = V_p^H Z(j0+1:, :)   (kw x M)
-    if (blas) {
-      blas_gemm(blas, true, kw, M, mr, T(1), w.V + (j0 + 1) + j0 * n, n, w.strideV, w.Z + (j0 + 1),
-                w.ldz, w.strideZ, T(0), w.Wt, nb, w.strideWs, B);
-    } else {
+    {
       launch_fill_zero(w.Wt, (int64_t)((B - 1) * w.strideWs + nb * M) * (int64_t)sizeof(cplx<T>), s);
       GemmParams<T> g{};
       g.opA = 'C'; g.opB = 'N';

@@ -93,52 +93,78 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
 
   cplx<T> ra[4], rb[4];
-  // Operand tiles are loaded unconditionally from a clamped address and then
-  // masked, so all eight loads of a stage are in flight together.
-  auto ld = [](const cplx<T>* p, int64_t idx, bool ok, bool conj) {
-    const cplx<T> t = p[ok ? idx : 0];
-    const T zr = T(0);
-    return cplx<T>{ok ? t.x : zr, ok ? (conj ? -t.y : t.y) : zr};
+  // Operand tiles are loaded unconditionally from clamped addresses into
+  // registers; masking, conjugation and the Hermitian-dot diagonal rule are
+  // applied only when the registers are stored to LDS, after the current K
+  // step's MFMAs.  No loaded value is used before then, so the next tile's
+  // loads stay in flight across the whole K step instead of being waited for
+  // at its start.
+  int64_t kpend = kbeg;                 // K offset of the tile held in ra / rb
+  struct Seg { const cplx<T>* A; const cplx<T>* B; int64_t k0, Kseg; };
+  auto seg = [&](int64_t kg) {
+    const bool second = two && (kg >= K1);
+    return Seg{second ? A2 : A1, second ? B2 : B1, second ? kg - K1 : kg, second ? K - K1 : K1};
   };
   auto load_tiles = [&](int64_t kg) {
-    const bool second = two && (kg >= K1);
-    const cplx<T>* A = second ? A2 : A1;
-    const cplx<T>* B = second ? B2 : B1;
-    const int64_t k0 = second ? kg - K1 : kg;
-    const int64_t Kseg = second ? K - K1 : K1;
+    kpend = kg;
+    const Seg g = seg(kg);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if (!CONJA) {
         const int r = tid % 64, kk = tid / 64 + 4 * i;
-        const int64_t gr = row0 + r, gk = k0 + kk;
-        ra[i] = ld(A, gr + gk * p.lda, gr < M && gk < Kseg, false);
-        if (herm && gk >= gr) ra[i] = (gk == gr) ? cplx<T>{T(0.5) * ra[i].x, T(0)} : cplx<T>{0, 0};
+        const int64_t gr = row0 + r, gk = g.k0 + kk;
+        const bool ok = gr < M && gk < g.Kseg;
+        ra[i] = g.A[ok ? gr + gk * p.lda : 0];
       } else {
         const int kk = tid % 16, r = tid / 16 + 16 * i;
-        const int64_t gr = row0 + r, gk = k0 + kk;
-        ra[i] = ld(A, gk + gr * p.lda, gr < M && gk < Kseg, true);
+        const int64_t gr = row0 + r, gk = g.k0 + kk;
+        const bool ok = gr < M && gk < g.Kseg;
+        ra[i] = g.A[ok ? gk + gr * p.lda : 0];
       }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if (!CONJB) {
         const int kk = tid % 16, c = tid / 16 + 16 * i;
-        const int64_t gk = k0 + kk, gc = col0 + c;
-        rb[i] = ld(B, gk + gc * p.ldb, gk < Kseg && gc < N, false);
+        const int64_t gk = g.k0 + kk, gc = col0 + c;
+        const bool ok = gk < g.Kseg && gc < N;
+        rb[i] = g.B[ok ? gk + gc * p.ldb : 0];
       } else {
         const int c = tid % 64, kk = tid / 64 + 4 * i;
-        const int64_t gk = k0 + kk, gc = col0 + c;
-        rb[i] = ld(B, gc + gk * p.ldb, gk < Kseg && gc < N, true);
+        const int64_t gk = g.k0 + kk, gc = col0 + c;
+        const bool ok = gk < g.Kseg && gc < N;
+        rb[i] = g.B[ok ? gc + gk * p.ldb : 0];
       }
     }
   };
+  auto mask = [](cplx<T> v, bool ok, bool conj) {
+    const T zr = T(0);
+    return cplx<T>{ok ? v.x : zr, ok ? (conj ? -v.y : v.y) : zr};
+  };
   auto store_tiles = [&](int st) {
+    const Seg g = seg(kpend);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (!CONJA) { As[st][tid / 64 + 4 * i][tid % 64] = ra[i]; }
-      else { As[st][tid % 16][tid / 16 + 16 * i] = ra[i]; }
-      if (!CONJB) { Bs[st][tid % 16][tid / 16 + 16 * i] = rb[i]; }
-      else { Bs[st][tid / 64 + 4 * i][tid % 64] = rb[i]; }
+      if (!CONJA) {
+        const int r = tid % 64, kk = tid / 64 + 4 * i;
+        const int64_t gr = row0 + r, gk = g.k0 + kk;
+        cplx<T> v = mask(ra[i], gr < M && gk < g.Kseg, false);
+        if (herm && gk >= gr) v = (gk == gr) ? cplx<T>{T(0.5) * v.x, T(0)} : cplx<T>{0, 0};
+        As[st][kk][r] = v;
+      } else {
+        const int kk = tid % 16, r = tid / 16 + 16 * i;
+        const int64_t gr = row0 + r, gk = g.k0 + kk;
+        As[st][kk][r] = mask(ra[i], gr < M && gk < g.Kseg, true);
+      }
+      if (!CONJB) {
+        const int kk = tid % 16, c = tid / 16 + 16 * i;
+        const int64_t gk = g.k0 + kk, gc = col0 + c;
+        Bs[st][kk][c] = mask(rb[i], gk < g.Kseg && gc < N, false);
+      } else {
+        const int c = tid % 64, kk = tid / 64 + 4 * i;
+        const int64_t gk = g.k0 + kk, gc = col0 + c;
+        Bs[st][kk][c] = mask(rb[i], gk < g.Kseg && gc < N, true);
+      }
     }
   };
 
