@@ -36,11 +36,15 @@ template <> struct mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBK = 16;
 
-template <typename T, bool CONJA, bool CONJB, bool DOT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_zgemm(GemmParams<T> p) {
+// WM x WN MFMA blocks of 16 x 16 per wave: the workgroup tile is
+// (32 WM) x (32 WN); the library dispatches 2 x 2 (64 x 64)
+template <typename T, bool CONJA, bool CONJB, bool DOT, int WM = 2, int WN = 2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WM * WN > 4 ? 1 : 2)))
+k_zgemm(GemmParams<T> p) {
   using acc_t = typename mfma<T>::acc;
+  constexpr int GBM = 32 * WM, GBN = 32 * WN;
   // two LDS stages (the next K tile is stored while the current one is
   // consumed: one barrier per K step); rows padded by 16 bytes so the
   // k-major stores of 16 lanes fall in distinct banks
@@ -50,7 +54,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 
   const int tid = threadIdx.x;
   const int lane = tid % WAVE, wave = tid / WAVE;
-  const int wr0 = (wave / 2) * 32, wc0 = (wave % 2) * 32;
+  const int wr0 = (wave / 2) * 16 * WM, wc0 = (wave % 2) * 16 * WN;
   const int64_t ksplit = p.ksplit > 1 ? p.ksplit : 1;
   const int64_t binner = p.binner > 1 ? p.binner : 1;
   int64_t z = blockIdx.z;
@@ -82,17 +86,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   auto product = [](T p1, T p2, T p3) {
     return k3M ? cplx<T>{p1 - p2, p3 - p1 - p2} : cplx<T>{p1, p2};
   };
-  acc_t acc1[2][2], acc2[2][2], acc3[2][2];
+  acc_t acc1[WM][WN], acc2[WM][WN], acc3[WM][WN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WN; ++j) {
       acc1[i][j] = acc_t{0, 0, 0, 0};
       acc2[i][j] = acc_t{0, 0, 0, 0};
       acc3[i][j] = acc_t{0, 0, 0, 0};
     }
 
-  cplx<T> ra[4], rb[4];
+  cplx<T> ra[2 * WM], rb[2 * WN];
   // Operand tiles are loaded unconditionally from clamped addresses into
   // registers; masking, conjugation and the Hermitian-dot diagonal rule are
   // applied only when the registers are stored to LDS, after the current K
@@ -109,9 +113,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     kpend = kg;
     const Seg g = seg(kg);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 2 * WM; ++i) {
       if (!CONJA) {
-        const int r = tid % 64, kk = tid / 64 + 4 * i;
+        const int r = tid % GBM, kk = tid / GBM + (256 / GBM) * i;
         const int64_t gr = row0 + r, gk = g.k0 + kk;
         const bool ok = gr < M && gk < g.Kseg;
         ra[i] = g.A[ok ? gr + gk * p.lda : 0];
@@ -123,14 +127,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       }
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 2 * WN; ++i) {
       if (!CONJB) {
         const int kk = tid % 16, c = tid / 16 + 16 * i;
         const int64_t gk = g.k0 + kk, gc = col0 + c;
         const bool ok = gk < g.Kseg && gc < N;
         rb[i] = g.B[ok ? gk + gc * p.ldb : 0];
       } else {
-        const int c = tid % 64, kk = tid / 64 + 4 * i;
+        const int c = tid % GBN, kk = tid / GBN + (256 / GBN) * i;
         const int64_t gk = g.k0 + kk, gc = col0 + c;
         const bool ok = gk < g.Kseg && gc < N;
         rb[i] = g.B[ok ? gc + gk * p.ldb : 0];
@@ -144,9 +148,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   auto store_tiles = [&](int st) {
     const Seg g = seg(kpend);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 2 * WM; ++i) {
       if (!CONJA) {
-        const int r = tid % 64, kk = tid / 64 + 4 * i;
+        const int r = tid % GBM, kk = tid / GBM + (256 / GBM) * i;
         const int64_t gr = row0 + r, gk = g.k0 + kk;
         cplx<T> v = mask(ra[i], gr < M && gk < g.Kseg, false);
         if (herm && gk >= gr) v = (gk == gr) ? cplx<T>{T(0.5) * v.x, T(0)} : cplx<T>{0, 0};
@@ -156,12 +160,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const int64_t gr = row0 + r, gk = g.k0 + kk;
         As[st][kk][r] = mask(ra[i], gr < M && gk < g.Kseg, true);
       }
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * WN; ++i) {
       if (!CONJB) {
         const int kk = tid % 16, c = tid / 16 + 16 * i;
         const int64_t gk = g.k0 + kk, gc = col0 + c;
         Bs[st][kk][c] = mask(rb[i], gk < g.Kseg && gc < N, false);
       } else {
-        const int c = tid % 64, kk = tid / 64 + 4 * i;
+        const int c = tid % GBN, kk = tid / GBN + (256 / GBN) * i;
         const int64_t gk = g.k0 + kk, gc = col0 + c;
         Bs[st][kk][c] = mask(rb[i], gk < g.Kseg && gc < N, true);
       }
@@ -178,22 +185,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 #pragma unroll
     for (int ks = 0; ks < GBK / 4; ++ks) {
       const int kk = ks * 4 + lane / 16;
-      cplx<T> a[2], bb[2];
+      cplx<T> a[WM], bb[WN];
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi) a[mi] = As[st][kk][wr0 + mi * 16 + lane % 16];
+      for (int mi = 0; mi < WM; ++mi) a[mi] = As[st][kk][wr0 + mi * 16 + lane % 16];
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) bb[ni] = Bs[st][kk][wc0 + ni * 16 + lane % 16];
-      T as_[2] = {}, bs_[2] = {};
+      for (int ni = 0; ni < WN; ++ni) bb[ni] = Bs[st][kk][wc0 + ni * 16 + lane % 16];
+      T as_[WM] = {}, bs_[WN] = {};
       if constexpr (k3M) {
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi) as_[mi] = a[mi].x + a[mi].y;
+        for (int mi = 0; mi < WM; ++mi) as_[mi] = a[mi].x + a[mi].y;
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) bs_[ni] = bb[ni].x + bb[ni].y;
+        for (int ni = 0; ni < WN; ++ni) bs_[ni] = bb[ni].x + bb[ni].y;
       }
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
+      for (int mi = 0; mi < WM; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
+        for (int ni = 0; ni < WN; ++ni) {
           if constexpr (k3M) {
             acc1[mi][ni] = mfma<T>::op(a[mi].x, bb[ni].x, acc1[mi][ni]);
             acc2[mi][ni] = mfma<T>::op(a[mi].y, bb[ni].y, acc2[mi][ni]);
@@ -218,9 +225,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     cplx<T>* C = p.C + b * p.strideC + bi * p.sCin;
     const bool beta0 = (p.beta.x == T(0) && p.beta.y == T(0));
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < WM; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) {
+      for (int ni = 0; ni < WN; ++ni) {
         const int64_t col = col0 + wc0 + ni * 16 + lane % 16;
         if (col >= N) continue;
 #pragma unroll
@@ -243,12 +250,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const cplx<T>* X = p.dotX + b * p.strideX + bi * p.sXin;
     double* dout = p.dot_out + b * p.dot_stride + bi * p.sDotin;
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni) {
+    for (int ni = 0; ni < WN; ++ni) {
       const int64_t col = col0 + wc0 + ni * 16 + lane % 16;
       double s = 0.0;
       if (col < N) {
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
+        for (int mi = 0; mi < WM; ++mi)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int64_t row = row0 + wr0 + mi * 16 + mfma<T>::row(lane, i);
@@ -276,7 +283,7 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
     throw std::runtime_error("launch_gemm: herm_dot needs the dot epilogue, opA = 'N', square A, one product");
   // a K step must not straddle the two products
   if (p.A2 != nullptr && p.k1 % GBK != 0) throw std::runtime_error("launch_gemm: k1 must be a multiple of 16");
-  dim3 grid((unsigned)ceil_div(p.m, GBM), (unsigned)ceil_div(p.n, GBN),
+  dim3 grid((unsigned)ceil_div(p.m, 64), (unsigned)ceil_div(p.n, 64),
             (unsigned)(p.batch * bin * ks));
   const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);
 #define PPH_GEMM_CASE(CA, CB, DT)                                                  \
