@@ -254,3 +254,57 @@ def test_profile_before_first_two_lane_run(gpu_available):
     assert prof["total"][1] == nbatch and prof["tridiag"][1] == nbatch
     assert prof["total"][0] > 0.0
     assert prof["hemv_launches"][1] == nbatch * len(range(0, n - 1, 8))
+
+
+@pytest.mark.parametrize("prec", ["D", "S"])
+@pytest.mark.parametrize("sel", ["I", "V", "N"])
+def test_run_selections_in_lanes(gpu_available, prec, sel):
+    """pph_run with several batches in flight (batch 2, two lanes, a ragged
+    last batch) for the other selections of pzheevx: RANGE='I' (modes 5..40),
+    RANGE='V' (the reference's vl/K32, vu/K32 scaling, a window through the
+    middle of the spectrum) and JOBZ='N' (values only).  The selected
+    frequencies and their count against the oracle; where vectors exist, their
+    residuals against the oracle's D(q), and each selected mode's velocity
+    against Re(z^H dD z) of the library's own vector, scaled as
+    write_output.F90:443-446 scales it (independent of the gauge of degenerate
+    modes)."""
+    lib = _lib()
+    from parphom_amd import synth
+    st, fc = U.structure(2, 3, 23)
+    S = U.oracle_system(2, 3, 23)
+    qs = synth.q_random(5, 31)
+    if sel == "I":
+        opt = dict(jobz="V", rng="I", il=5, iu=40, vel=True)
+    elif sel == "V":
+        opt = dict(jobz="V", rng="V", vl=300.0, vu=900.0, vel=True)
+    else:
+        opt = dict(jobz="N", rng="A")
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, precision=prec, batch=2, streams=2,
+                    **opt) as h:
+        res = h.run(qs)
+    const = O.C2 * O.C1SQ32 * 100.0 * 2.0 * O.PI32
+    for iq, q in enumerate(qs):
+        ref = S.solve(q, jobz=opt["jobz"], rng=opt["rng"], vl=opt.get("vl", 0.0),
+                      vu=opt.get("vu", 0.0), il=opt.get("il", 1), iu=opt.get("iu", 1))
+        assert res["info"][iq] == 0
+        assert res["m"][iq] == len(ref["lam"]) > 0, (res["m"][iq], len(ref["lam"]))
+        assert np.abs(res["eval_cm"][iq] - ref["eval_cm"]).max() <= 1e-3
+        if opt["jobz"] == "N":
+            assert res["evec"] is None
+            continue
+        Z = np.asarray(res["evec"][iq])
+        D = S.D(q)
+        nrm = np.abs(np.linalg.eigvalsh(D)).max()
+        tol = 1e-10 if prec == "D" else 1e-5
+        assert np.linalg.norm(D @ Z - Z * ref["lam"], axis=0).max() <= tol * nrm
+        om = np.asarray(res["eval_cm"][iq])
+        v = np.asarray(res["vel"][iq])
+        expect = np.zeros_like(v)
+        for a in range(3):
+            dD = S.D(q, a + 1)
+            e = np.einsum("ik,ik->k", Z.conj(), dD @ Z).real
+            expect[:, a] = e / np.maximum(2.0 * np.abs(om), 1e-3) * const
+        # relative to the largest velocity of the q-point (v_z of the flat
+        # layers is rounding noise on its own scale)
+        vtol = (1e-7 if prec == "D" else 1e-5) * np.abs(expect).max() + 1e-9
+        assert np.abs(v - expect).max() <= vtol, np.abs(v - expect).max()
