@@ -52,8 +52,8 @@ class InputConfig:
     output_name: str = "results"
     output_location: str = "./"
     messages: List[str] = field(default_factory=list)
-    # keys the reference does not know, kept by this build (they print
-    # "not recognised" in parphom.x): precision and batch
+    # keys parphom.x does not know (it prints "not recognised" for them),
+    # accepted here: precision and batch
     precision: str = "D"
     batch: int = 0
 
