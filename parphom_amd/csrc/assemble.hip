@@ -65,6 +65,17 @@ __global__ void k_image_table(const double* __restrict__ crys, const double* __r
   mask[i + j * N] = m;
 }
 
+// x / b correctly rounded, from r = RN(1 / b): q0 = x r is within one ulp of
+// x / b, e = x - q0 b is exact with an FMA, and q0 + e r rounds to RN(x / b)
+// (Markstein, IBM J. Res. Develop. 34 (1990) 111-119).  Three instructions
+// instead of a full division for each of the 144 quotients a thread forms per
+// q-point, with the same result bit for bit.
+__device__ __forceinline__ double div_rn(double x, double b, double r) {
+  const double q0 = x * r;
+  const double e = fma(-q0, b, x);
+  return fma(e, r, q0);
+}
+
 struct PhaseOut {
   double p0r, p0i;          // sum exp(i theta)
   double pdr[3], pdi[3];    // analytic derivative sums
@@ -136,7 +147,7 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
   // (dr, di) or the central-difference sums at q + dc/2 (dr, di) and q - dc/2 (er, ei)
   __shared__ double sp0r[256], sp0i[256];
   __shared__ double sdr[3][256], sdi[3][256], ser[MODE == 1 ? 3 : 1][256], sei[MODE == 1 ? 3 : 1][256];
-  __shared__ double smult[256], ssmm[256];
+  __shared__ double smult[256], ssmm[256], srmult[256], srsmm[256];
   const int64_t N = A.N, n = 3 * N;
   const int tid = threadIdx.x;
   const int64_t i0 = (int64_t)blockIdx.x * 256;
@@ -155,6 +166,8 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
     m = A.mask[i + j * N];
     smult[tid] = (double)__popc((unsigned)m);
     ssmm[tid] = sqrt(A.rmass[i] * A.rmass[j]);
+    srmult[tid] = 1.0 / smult[tid];
+    srsmm[tid] = 1.0 / ssmm[tid];
     ci0 = A.crys[i]; ci1 = A.crys[i + N]; ci2 = A.crys[i + 2 * N];
   }
   // Phi_sym(3 i0 + r, 3 j + b) for this thread's rows r = tid + 256 k
@@ -202,14 +215,17 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
       if (r >= nrow) continue;
       const int pr = (int)(r / 3);                   // pair within the workgroup
       const double p0r = sp0r[pr], p0i = sp0i[pr], mult = smult[pr], smm = ssmm[pr];
+      const double rmult = srmult[pr], rsmm = srsmm[pr];
+      // ((Phi p) / multiplicity) / sqrt(m_i m_j), each quotient correctly rounded
+      auto scale = [&](double fp) { return div_rn(div_rn(fp, mult, rmult), smm, rsmm); };
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         const size_t idx = base + (size_t)(3 * i0 + r) + (size_t)(3 * j + b) * n;
         const size_t idxD = baseD + (size_t)(3 * i0 + r) + (size_t)(3 * j + b) * A.ldD;
         const double fv = f[k][b];
         // dij = Phi * phase / multiplicity / sqrt(m_i m_j)
-        const double vr = ((fv * p0r) / mult) / smm;
-        const double vi = ((fv * p0i) / mult) / smm;
+        const double vr = scale(fv * p0r);
+        const double vi = scale(fv * p0i);
         if (A.D64) A.D64[idxD] = cplx<double>{vr, vi};
         if (A.D32) A.D32[idxD] = cplx<float>{(float)vr, (float)vi};
         if (want_d) {
@@ -218,14 +234,14 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
             double wr, wi;
             if (MODE == 1) {
               // (D(q+) - D(q-)) / |delta| elementwise
-              const double pr_ = ((fv * sdr[d][pr]) / mult) / smm, pi_ = ((fv * sdi[d][pr]) / mult) / smm;
-              const double mr = ((fv * ser[MODE == 1 ? d : 0][pr]) / mult) / smm;
-              const double mi = ((fv * sei[MODE == 1 ? d : 0][pr]) / mult) / smm;
+              const double pr_ = scale(fv * sdr[d][pr]), pi_ = scale(fv * sdi[d][pr]);
+              const double mr = scale(fv * ser[MODE == 1 ? d : 0][pr]);
+              const double mi = scale(fv * sei[MODE == 1 ? d : 0][pr]);
               wr = (pr_ - mr) / A.cd_norm;
               wi = (pi_ - mi) / A.cd_norm;
             } else {
-              wr = ((fv * sdr[d][pr]) / mult) / smm;
-              wi = ((fv * sdi[d][pr]) / mult) / smm;
+              wr = scale(fv * sdr[d][pr]);
+              wi = scale(fv * sdi[d][pr]);
             }
             const size_t didx = (size_t)(3 * iq + d) * mstride + (idx - base);
             if (A.dD64) A.dD64[didx] = cplx<double>{wr, wi};
