@@ -117,6 +117,54 @@ __device__ __forceinline__ cplx<T> panel_terms(int sub, int jj, FX&& term_x, FY&
   return acc;
 }
 
+// Raw V(r, p + k) and W(r, k) of a thread's first eight panel terms,
+// k = sub + 4 i (i < 8), loaded before the kernel's first barrier so that
+// their latency overlaps its reductions (a barrier keeps the compiler from
+// moving loads above it).  Entries with k >= jj or !ok are loaded from a
+// valid address and never used; terms from k = sub + 32 on (panels wider
+// than 32) are loaded where they are used.
+template <typename T>
+struct PanelRow {
+  cplx<T> v[8], w[8];
+};
+template <typename T>
+__device__ __forceinline__ void load_panel_row(const TridiagWork<T>& w, int64_t b, int64_t r,
+                                               int sub, int jj, int64_t p, bool ok,
+                                               PanelRow<T>& pr) {
+  const int64_t rc = ok ? r : 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int k = sub + 4 * i;
+    const int kc = (ok && k < jj) ? k : 0;
+    pr.v[i] = w.V[b * w.strideV + rc + (p + kc) * w.n];
+    pr.w[i] = w.W[b * w.strideW + rc + (int64_t)kc * w.n];
+  }
+}
+// sum over the thread's terms of V(r, k) x_k + W_eff(r, k) y_k, with
+// W_eff(r, k) = W(r, k) + a2(k) V(r, k): the first eight from pr, the rest
+// (k >= sub + 32) loaded here
+template <typename T, typename FA, typename FX, typename FY>
+__device__ __forceinline__ cplx<T> panel_row_terms(const TridiagWork<T>& w, int64_t b, int64_t r,
+                                                   int sub, int jj, int64_t p,
+                                                   const PanelRow<T>& pr, FA&& a2, FX&& xk,
+                                                   FY&& yk) {
+  cplx<T> acc{0, 0};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int k = sub + 4 * i;
+    if (k < jj) {
+      const cplx<T> we = cadd(pr.w[i], cmul(a2(k), pr.v[i]));
+      acc = cadd(acc, cadd(cmul(pr.v[i], xk(k)), cmul(we, yk(k))));
+    }
+  }
+  if (sub + 32 < jj) {
+    acc = cadd(acc, panel_terms<T>(
+        sub + 32, jj, [&](int k) { return cmul(w.V[b * w.strideV + r + (p + k) * w.n], xk(k)); },
+        [&](int k) { return cmul(weff_a(w, b, r, k, p, a2(k)), yk(k)); }));
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------- k_col
 // 256 threads = 64 rows x 4 waves; wave s handles the panel terms k = s, s+4, ...
 template <typename T>
@@ -132,6 +180,11 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   cplx<T>* A = w.A + b * w.strideA;
   const cplx<T>* V = w.V + b * w.strideV;
   const int lane = threadIdx.x % 64, sub = threadIdx.x / 64;
+  const int64_t r = j + blockIdx.x * (int64_t)64 + lane;
+  // loads that do not wait for alpha2: this row's panel entries and A(r, j)
+  PanelRow<T> prow;
+  load_panel_row(w, b, r, sub, jj, p, r < n && sub < jj, prow);
+  const cplx<T> a_rj = (sub == 0 && r < n) ? A[r + j * w.lda] : cplx<T>{0, 0};
 
   if (jj > 0) {
     // alpha2 of column j - 1, whose k_fin ran over m = n - j rows
@@ -154,7 +207,6 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   }
   __syncthreads();
 
-  const int64_t r = j + blockIdx.x * (int64_t)64 + lane;
   // clear the accumulators k_hemv_sym adds into (y over rows j+1.., dots 2 nb)
   if (sub == 0 && r < n) w.ypart[b * w.strideY + r] = cplx<T>{0, 0};
   if (blockIdx.x == 0)
@@ -162,15 +214,15 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   cplx<T> acc{0, 0};
   if (r < n && sub < jj) {
     // V(r,k) conj(W(j,k)) + W(r,k) conj(V(j,k))
-    acc = panel_terms<T>(
-        sub, jj, [&](int k) { return cmul(V[r + (p + k) * n], cconj(wj[k])); },
-        [&](int k) { return cmul(weff_a(w, b, r, k, p, a2s[k]), cconj(vj[k])); });
+    acc = panel_row_terms<T>(
+        w, b, r, sub, jj, p, prow, [&](int k) { return a2s[k]; },
+        [&](int k) { return cconj(wj[k]); }, [&](int k) { return cconj(vj[k]); });
   }
   red[sub][lane] = acc;
   __syncthreads();
   double xpart = 0.0;
   if (sub == 0 && r < n) {
-    cplx<T> a = A[r + j * w.lda];
+    cplx<T> a = a_rj;
     a = csub(a, cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane])));
     if (r == j) a.y = 0;
     A[r + j * w.lda] = a;
@@ -420,6 +472,18 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   const cplx<T> tau{(T)scal[S_TAU], (T)scal[S_TAU + 1]};
   const cplx<T> sc{(T)scal[S_SCALE], (T)scal[S_SCALE + 1]};
   const int lane = threadIdx.x % 64, sub = threadIdx.x / 64;
+  const int64_t rr = blockIdx.x * (int64_t)64 + lane;
+  const int64_t r = j + 1 + rr;
+  // loads that do not wait for the dots: this row's panel entries, S(r, 0),
+  // u(r) and x(r) = A(r, j)
+  PanelRow<T> prow;
+  load_panel_row(w, b, r, sub, jj, p, rr < m && sub < jj, prow);
+  cplx<T> s0{0, 0}, ur{0, 0}, xr{0, 0};
+  if (sub == 0 && rr < m) {
+    s0 = A[r + (j + 1) * w.lda];
+    ur = w.ypart[b * w.strideY + r];
+    xr = A[r + j * w.lda];
+  }
 
   // W^H v = conj(W_eff(j+1, k)) + scale (W^H x), V^H v likewise
   for (int k = threadIdx.x; k < 2 * jj; k += blockDim.x) {
@@ -429,26 +493,24 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   }
   __syncthreads();
 
-  const int64_t rr = blockIdx.x * (int64_t)64 + lane;
-  const int64_t r = j + 1 + rr;
   cplx<T> acc{0, 0};
   if (rr < m && sub < jj) {
-    acc = panel_terms<T>(
-        sub, jj, [&](int k) { return cmul(V[r + (p + k) * n], sW[k]); },
-        [&](int k) { return cmul(weff(w, b, r, k, p, scal), sV[k]); });
+    acc = panel_row_terms<T>(
+        w, b, r, sub, jj, p, prow,
+        [&](int k) { return cplx<T>{(T)scal[S_A2 + 2 * k], (T)scal[S_A2 + 2 * k + 1]}; },
+        [&](int k) { return sW[k]; }, [&](int k) { return sV[k]; });
   }
   red[sub][lane] = acc;
   __syncthreads();
   double pr = 0, pi = 0;
   if (sub == 0 && rr < m) {
     // y = S(:, 0) + scale u, S(:, 0) = A(j+1:, j+1) with a real diagonal
-    cplx<T> s0 = A[r + (j + 1) * w.lda];
     if (rr == 0) s0.y = 0;
-    cplx<T> y = cadd(s0, cmul(sc, w.ypart[b * w.strideY + r]));
+    cplx<T> y = cadd(s0, cmul(sc, ur));
     y = csub(y, cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane])));
     y = cmul(tau, y);
     w.W[b * w.strideW + r + (int64_t)jj * n] = y;
-    const cplx<T> v = (rr == 0) ? cplx<T>{1, 0} : cmul(sc, A[r + j * w.lda]);
+    const cplx<T> v = (rr == 0) ? cplx<T>{1, 0} : cmul(sc, xr);
     V[r + j * n] = v;
     const cplx<T> t = cmulc(y, v);
     pr = t.x; pi = t.y;
