@@ -292,6 +292,86 @@ def fcsym_bench(lib, torch, a, iterations=10):
     print(json.dumps(res))
 
 
+def large_bench(lib, torch, a, nvec=2000):
+    """The large single-GPU workload: (30,31) twisted bilayer graphene, N = 11164
+    atoms, n = 33492.  Range I (the lowest `nvec` eigenpairs) on its 8 q-points
+    (Gamma, K, M and 5 seeded random), FP32+refine and FP64, then range A
+    eigenvalues-only on Gamma, K and M in both precisions.  Every run is
+    pph_run with host output buffers, two q-points per call (one per lane, or
+    one lane for the range-A FP32 run, whose all-vector refinement workspace
+    is ~100 GB per q-point); handle creation (force-constant upload and
+    symmetrisation) is not timed.  Reports q-pts/s per run, max |dw| of FP32
+    against FP64, and the FP32 eigenvector residual |D z - lam z| / |D|_2 at K
+    with D from the library's FP64 assembly."""
+    from parphom_amd import synth
+    t0 = time.perf_counter()
+    st, fc, qs = synth.workload("tbg_30_31")
+    n = 3 * st.natom
+    log(f"[large] N={st.natom} n={n} nq={len(qs)} (inputs built in {time.perf_counter() - t0:.1f}s)")
+
+    def run(prec, rng, q, streams):
+        jobz = "V" if rng == "I" else "N"
+        h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz=jobz, rng=rng, il=1,
+                       iu=nvec, vel=False, precision=prec, batch=1, streams=streams)
+        ev, info, zk, t = [], [], None, 0.0
+        for i in range(0, len(q), 2):
+            t1 = time.perf_counter()
+            r = h.run(q[i:i + 2])
+            t += time.perf_counter() - t1
+            ev += r["eval_cm"]
+            info += [int(x) for x in r["info"]]
+            if r["evec"] is not None and i <= 1 < i + 2:
+                zk = r["evec"][1 - i]                      # vectors at K (q index 1)
+            log(f"[large] {prec} range {rng}: q {i + 1}-{min(i + 2, len(q))} of {len(q)} "
+                f"done ({t:.1f}s)")
+        h.close()
+        return len(q) / t, ev, info, zk
+
+    qps_i32, ev_i32, inf_i32, zk = run("S", "I", qs, 2)
+    qps_i64, ev_i64, inf_i64, _ = run("D", "I", qs, 2)
+    q3 = qs[:3]
+    qps_a32, ev_a32, inf_a32, _ = run("S", "A", q3, 1)
+    qps_a64, ev_a64, inf_a64, _ = run("D", "A", q3, 2)
+    dw_i = max(float(np.abs(x - y).max()) for x, y in zip(ev_i32, ev_i64))
+    dw_a = max(float(np.abs(x - y).max()) for x, y in zip(ev_a32, ev_a64))
+    # residual of the FP32 vectors at K against the FP64 D(K); |D|_2 = max |lambda|
+    # from the FP64 range-A eigenvalues at K
+    K = float(np.float32(15.633302)) * float(np.float32(33.35641))
+    lam_all = np.sign(ev_a64[1]) * (ev_a64[1] / K) ** 2
+    dnorm = float(np.abs(lam_all).max())
+    h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, precision="D", batch=1)
+    Dk = h.assemble(qs[1:2])[0]                        # column-major buffer, strided view
+    h.close()
+    dev = torch.device("cuda:0")
+    Dg = torch.from_numpy(Dk).to(dev)
+    del Dk
+    Zg = torch.from_numpy(np.ascontiguousarray(zk)).to(dev)
+    lam_k = torch.from_numpy(np.sign(ev_i32[1]) * (ev_i32[1] / K) ** 2).to(dev)
+    R = Dg @ Zg - Zg * lam_k[None, :]
+    resid = float((torch.linalg.vector_norm(R, dim=0) /
+                   torch.linalg.vector_norm(Zg, dim=0)).max().item()) / dnorm
+    del Dg, Zg, R
+    torch.cuda.empty_cache()
+    res = {"metric": f"q-pts/s at 3N={n}: lowest {nvec} eigenpairs, FP32+refine vs FP64",
+           "value": qps_i32, "unit": "q-pts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": 1e3 * len(qs) / qps_i32, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c64 MFMA (FP32 acc) + f64 refinement",
+           "data": "synthetic (seeded spring-model force constants; Gamma, K, M + 5 random q)",
+           "config": {"workload": "tbg_30_31", "natom": st.natom, "n": n, "nq": len(qs),
+                      "range": "I", "il": 1, "iu": nvec, "eigenvectors": True,
+                      "group_velocity": False, "precision": "S", "batch": 1, "streams": 2,
+                      "host_buffers": True},
+           "fp64_path": {"value": qps_i64, "unit": "q-pts/s"},
+           "speedup_vs_fp64": qps_i32 / qps_i64,
+           "max_abs_domega_vs_fp64_cm": dw_i,
+           "max_eigvec_residual_rel_at_K": resid,
+           "range_A_values_only": {"nq": 3, "fp32_qps": qps_a32, "fp64_qps": qps_a64,
+                                   "speedup_vs_fp64": qps_a32 / qps_a64,
+                                   "max_abs_domega_vs_fp64_cm": dw_a},
+           "info": {"I_S": inf_i32, "I_D": inf_i64, "A_S": inf_a32, "A_D": inf_a64}}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +387,8 @@ def main():
                     help="measure the force-constant symmetrisation instead (see fcsym_bench)")
     ap.add_argument("--dos", action="store_true",
                     help="measure the linear-triangulation DOS kernel instead (see dos_bench)")
+    ap.add_argument("--large", action="store_true",
+                    help="run the large single-GPU workload instead (see large_bench)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device 3 times and exit (a copy of known "
                          "size, to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -332,6 +414,9 @@ def main():
         return
     if a.fcsym:
         fcsym_bench(lib, torch, a)
+        return
+    if a.large:
+        large_bench(lib, torch, a)
         return
     t0 = time.perf_counter()
     st, fc, qs = synth.workload(a.workload)

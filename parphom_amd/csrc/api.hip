@@ -155,7 +155,7 @@ struct Lane {
   Timers tm;
   DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
-  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12;
+  DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   // rocBLAS handle for the plain GEMMs of the back-transformation (made on
   // first use, bound to this lane's stream, atomics off so results repeat)
   rocblas_handle blas = nullptr;
@@ -329,9 +329,16 @@ static int64_t range_mcap(const pph_opts& o, int64_t n) {
 // -sum_j |x_j^H r|^2 / ((rho_j - theta) |x_j|^2) from every other computed
 // mode j, all in FP64: W = D X_T, C1 = X^H W, C2 = X^H X_T on the GPU, the
 // k x k algebra on the host.
-static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* D64, const cplx<double>* X,
+static void complement_correction(Handle& h, Lane& L, int64_t B, const cplx<double>* D64,
+                                  cplx<double>* X, int64_t mcap, int64_t K, int64_t mmax,
+                                  const std::vector<int64_t>& Mv,
+                                  const std::vector<std::complex<double>>& Uh,
+                                  const std::vector<double>& thh, const std::vector<double>& invnb,
+                                  std::vector<double>& corr, hipStream_t s);
+
+static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* D64, cplx<double>* X,
                              double* lam_dev, const double* nrm2_dev, const std::vector<int64_t>& Mv,
-                             int64_t mcap, hipStream_t s) {
+                             const std::vector<int64_t>& ilo, int64_t mcap, hipStream_t s) {
   const int64_t n = h.n;
   const double K32 = (double)(float)(15.633302f * 33.35641f);
   const double lam_low = (50.0 / K32) * (50.0 / K32);
@@ -389,6 +396,10 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
     return m;
   }();
   const double ctol = 1e-7 * std::max(tnorm_proxy, 1e-300);
+  // Ritz pairs of the low block, kept for the complement correction below
+  std::vector<cd> Uh((size_t)B * Kp * Kp, 0.0);
+  std::vector<double> thh((size_t)B * Kp, 0.0);
+  std::vector<std::vector<double>> newls(B);
   for (int64_t b = 0; b < B; ++b) {
     const int64_t kt = kb[b], M = Mv[b];
     if (kt == 0) continue;
@@ -430,13 +441,136 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
           if (std::fabs(dl) > 0.0) corr -= std::norm(sjm) / (dl * nb[j]);
         }
         newl[c0 + m] = th[m] + corr;
+        thh[b * Kp + m] = th[m];
+        for (int c = 0; c < kc; ++c) Uh[(size_t)b * Kp * Kp + c + (size_t)m * Kp] = U[c + (size_t)m * kc];
       }
       c0 = c1e;
     }
-    std::sort(newl.begin(), newl.end());
-    PPH_HIP(hipMemcpyAsync(lam_dev + b * mcap, newl.data(), kt * sizeof(double), hipMemcpyHostToDevice, s));
+    newls[b].swap(newl);
+  }
+  // The sums above run over the computed modes only.  When those start at
+  // the bottom of the spectrum but stop short of its top (a range I or V
+  // selection), the modes never computed contribute
+  // -r_perp^H (D - theta)^-1 r_perp, r_perp the part of the residual outside
+  // the computed span, on which D - theta is positive definite.  At the
+  // acoustic triplet at Gamma (omega = K sqrt(lambda) near 0) that share
+  // matters at the 1e-3 cm^-1 level for large n.
+  std::vector<double> invnb((size_t)B * mcap, 0.0);
+  bool any = false;
+  for (int64_t b = 0; b < B; ++b) {
+    const bool part = kb[b] > 0 && ilo[b] == 0 && Mv[b] < n;
+    if (!part) {
+      for (int64_t i = 0; i < Kp * Kp; ++i) Uh[(size_t)b * Kp * Kp + i] = 0.0;
+      for (int64_t m = 0; m < Kp; ++m) thh[b * Kp + m] = 0.0;
+      continue;
+    }
+    any = true;
+    for (int64_t j = 0; j < Mv[b]; ++j)
+      invnb[b * mcap + j] = nrm[b * mcap + j] > 0.0 ? 1.0 / nrm[b * mcap + j] : 0.0;
+  }
+  if (any) {
+    std::vector<double> corr;
+    complement_correction(h, L, B, D64, X, mcap, K, mmax, Mv, Uh, thh, invnb, corr, s);
+    for (int64_t b = 0; b < B; ++b)
+      if (kb[b] > 0 && ilo[b] == 0 && Mv[b] < n)
+        for (int64_t m = 0; m < kb[b]; ++m) newls[b][m] += corr[b * Kp + m];
+  }
+  for (int64_t b = 0; b < B; ++b) {
+    if (kb[b] == 0) continue;
+    std::sort(newls[b].begin(), newls[b].end());
+    PPH_HIP(hipMemcpyAsync(lam_dev + b * mcap, newls[b].data(), kb[b] * sizeof(double),
+                           hipMemcpyHostToDevice, s));
   }
   PPH_HIP(hipStreamSynchronize(s));
+}
+
+// The complement share of the second-order correction for the Ritz pairs
+// (theta_m, y_m = X(:, 0:K) U(:, m)) of each matrix, by conjugate gradients
+// in FP64: r = D y - theta y, r_perp = P r with P = I - X diag(1/|x_j|^2) X^H
+// over the computed modes (applied twice), then kCgIters steps on
+// P (D - theta) P z = r_perp from z = 0; corr = -Re(r_perp^H z).  Matrices
+// whose U and theta are zero get corr = 0.  Per step: one n x n x 16 product
+// with D and two with X.  Columns M..mmax-1 of X (no mode) are cleared first:
+// the batched products read them.
+constexpr int kCgIters = 48;
+
+static void complement_correction(Handle& h, Lane& L, int64_t B, const cplx<double>* D64,
+                                  cplx<double>* X, int64_t mcap, int64_t K, int64_t mmax,
+                                  const std::vector<int64_t>& Mv,
+                                  const std::vector<std::complex<double>>& Uh,
+                                  const std::vector<double>& thh, const std::vector<double>& invnb,
+                                  std::vector<double>& corr, hipStream_t s) {
+  using cd = std::complex<double>;
+  const int64_t n = h.n, Kp = 16;
+  const int64_t sv = n * Kp;                      // stride of the n x Kp blocks
+  L.CG.ensure((size_t)B * (6 * sv + mcap * Kp + Kp * Kp) * sizeof(cd) +
+              (size_t)B * (4 * Kp + mcap) * sizeof(double));
+  cplx<double>* base = L.CG.as<cplx<double>>();
+  cplx<double>* Y = base;
+  cplx<double>* R = Y + B * sv;        // r_perp (kept)
+  cplx<double>* Z = R + B * sv;
+  cplx<double>* Rs = Z + B * sv;       // CG residual
+  cplx<double>* P = Rs + B * sv;
+  cplx<double>* Q = P + B * sv;
+  cplx<double>* T = Q + B * sv;        // mcap x Kp
+  cplx<double>* Ud = T + B * mcap * Kp;
+  double* thd = reinterpret_cast<double*>(Ud + B * Kp * Kp);
+  double* rho = thd + B * Kp;          // two buffers of B x Kp
+  double* pq = rho + 2 * B * Kp;
+  double* wn = pq + B * Kp;            // 1 / |x_j|^2, B x mcap
+  PPH_HIP(hipMemcpyAsync(Ud, Uh.data(), Uh.size() * sizeof(cd), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(thd, thh.data(), thh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(wn, invnb.data(), invnb.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  for (int64_t b = 0; b < B; ++b)
+    if (Mv[b] < mmax)
+      launch_fill_zero(X + b * n * mcap + Mv[b] * n, (mmax - Mv[b]) * n * (int64_t)sizeof(cd), s);
+  auto gemm = [&](char opA, int64_t m, int64_t k, const cplx<double>* A, int64_t lda, int64_t sA,
+                  const cplx<double>* Bm, int64_t ldb, int64_t sB, cplx<double>* C, int64_t ldc,
+                  int64_t sC, double alpha, double beta) {
+    GemmParams<double> g{};
+    g.opA = opA; g.opB = 'N'; g.m = m; g.n = Kp; g.k = k;
+    g.alpha = cplx<double>{alpha, 0}; g.beta = cplx<double>{beta, 0};
+    g.A = A; g.lda = lda; g.strideA = sA;
+    g.B = Bm; g.ldb = ldb; g.strideB = sB;
+    g.C = C; g.ldc = ldc; g.strideC = sC;
+    g.batch = B;
+    launch_gemm<double>(g, s);
+  };
+  // V <- V - X diag(1/|x|^2) X^H V, twice
+  auto project = [&](cplx<double>* V) {
+    for (int pass = 0; pass < 2; ++pass) {
+      gemm('C', mmax, n, X, n, n * mcap, V, n, sv, T, mcap, mcap * Kp, 1.0, 0.0);
+      launch_scale_rows(T, wn, mmax, Kp, mcap, mcap * Kp, mcap, B, s);
+      gemm('N', n, mmax, X, n, n * mcap, T, mcap, mcap * Kp, V, n, sv, -1.0, 1.0);
+    }
+  };
+  // y = X(:, 0:K) U, r = (D X(:, 0:K)) U - theta y with D X(:, 0:K) in L.WT
+  gemm('N', n, K, X, n, n * mcap, Ud, Kp, Kp * Kp, Y, n, sv, 1.0, 0.0);
+  gemm('N', n, K, L.WT.as<cplx<double>>(), n, n * Kp, Ud, Kp, Kp * Kp, R, n, sv, 1.0, 0.0);
+  launch_axpy_cols(R, Y, thd, -1.0, n, Kp, n, sv, Kp, B, s);
+  project(R);
+  const size_t vbytes = (size_t)B * sv * sizeof(cd);
+  launch_fill_zero(Z, (int64_t)vbytes, s);
+  PPH_HIP(hipMemcpyAsync(Rs, R, vbytes, hipMemcpyDeviceToDevice, s));
+  PPH_HIP(hipMemcpyAsync(P, R, vbytes, hipMemcpyDeviceToDevice, s));
+  launch_coldot_re(Rs, Rs, n, Kp, n, sv, B, rho, Kp, s);
+  for (int it = 0; it < kCgIters; ++it) {
+    double* rho_old = rho + (it % 2) * B * Kp;
+    double* rho_new = rho + ((it + 1) % 2) * B * Kp;
+    // q = P (D - theta) p
+    gemm('N', n, n, D64, h.ldA, h.ldA * n, P, n, sv, Q, n, sv, 1.0, 0.0);
+    launch_axpy_cols(Q, P, thd, -1.0, n, Kp, n, sv, Kp, B, s);
+    project(Q);
+    launch_coldot_re(P, Q, n, Kp, n, sv, B, pq, Kp, s);
+    launch_cg_update(Z, Rs, P, Q, rho_old, pq, n, Kp, n, sv, Kp, B, s);
+    launch_coldot_re(Rs, Rs, n, Kp, n, sv, B, rho_new, Kp, s);
+    launch_cg_direction(P, Rs, rho_new, rho_old, n, Kp, n, sv, Kp, B, s);
+  }
+  launch_coldot_re(R, Z, n, Kp, n, sv, B, pq, Kp, s);
+  corr.assign((size_t)B * Kp, 0.0);
+  PPH_HIP(hipMemcpyAsync(corr.data(), pq, corr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
+  for (auto& c : corr) c = -c;
 }
 
 // A: B matrices (n x n, working precision, destroyed); D64: FP64 matrices for
@@ -577,7 +711,8 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
                               mcap, s);
       launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
-      refine_low_modes(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), R.M, mcap, s);
+      refine_low_modes(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), R.M, ilo,
+                       mcap, s);
       L.tm.end(PH_REFINE, s);
       R.Z64 = L.Z64.as<cplx<double>>();
     } else {

@@ -140,6 +140,28 @@ void launch_colnorm2(const cplx<T>* X, int64_t n, int64_t m, int64_t ldx, int64_
                      int64_t batch, double* out, int64_t ldo, hipStream_t s);
 void launch_rq_finish(const double* num, const double* den, double* lam, int64_t nb, int64_t m,
                       int64_t ld, hipStream_t s);
+// Column-wise FP64 helpers (misc.hip).  Element (i, c) of matrix b sits at
+// b * stride + c * ld + i, columns c < k; per-column scalars at b * lds + c.
+// out = Re sum_i conj(A(i, c)) B(i, c)
+void launch_coldot_re(const cplx<double>* A, const cplx<double>* Bm, int64_t n, int64_t k,
+                      int64_t ld, int64_t stride, int64_t batch, double* out, int64_t ldo,
+                      hipStream_t s);
+// Y(:, c) += a sc[c] X(:, c)
+void launch_axpy_cols(cplx<double>* Y, const cplx<double>* X, const double* sc, double a, int64_t n,
+                      int64_t k, int64_t ld, int64_t stride, int64_t lds, int64_t batch,
+                      hipStream_t s);
+// T(j, c) *= w[b * ldw + j] for rows j < m (rows of weight 0 are set to 0)
+void launch_scale_rows(cplx<double>* T, const double* w, int64_t m, int64_t k, int64_t ld,
+                       int64_t stride, int64_t ldw, int64_t batch, hipStream_t s);
+// conjugate gradients per column: z += al p, r -= al q with al = rho / pq; then
+// p = r + (rho_new / rho_old) p
+void launch_cg_update(cplx<double>* z, cplx<double>* r, const cplx<double>* p,
+                      const cplx<double>* q, const double* rho, const double* pq, int64_t n,
+                      int64_t k, int64_t ld, int64_t stride, int64_t lds, int64_t batch,
+                      hipStream_t s);
+void launch_cg_direction(cplx<double>* p, const cplx<double>* r, const double* rho_new,
+                         const double* rho_old, int64_t n, int64_t k, int64_t ld, int64_t stride,
+                         int64_t lds, int64_t batch, hipStream_t s);
 template <typename Ti, typename To>
 void launch_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count, hipStream_t s);
 void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);

@@ -125,6 +125,116 @@ void launch_rq_finish(const double* num, const double* den, double* lam, int64_t
   PPH_CHECK_LAUNCH();
 }
 
+// ---------------------------------------------------------------- column helpers
+// Element (i, c) of matrix b sits at b * stride + c * ld + i; columns c < k.
+
+__global__ void __launch_bounds__(256) k_coldot_re(const cplx<double>* A, const cplx<double>* Bm,
+                                                   int64_t n, int64_t ld, int64_t stride,
+                                                   double* out, int64_t ldo) {
+  __shared__ double sh[8];
+  const int64_t c = blockIdx.x, b = blockIdx.y;
+  const cplx<double>* a = A + b * stride + c * ld;
+  const cplx<double>* x = Bm + b * stride + c * ld;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += a[i].x * x[i].x + a[i].y * x[i].y;
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[b * ldo + c] = s;
+}
+
+void launch_coldot_re(const cplx<double>* A, const cplx<double>* Bm, int64_t n, int64_t k,
+                      int64_t ld, int64_t stride, int64_t batch, double* out, int64_t ldo,
+                      hipStream_t s) {
+  if (k <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_coldot_re, dim3((unsigned)k, (unsigned)batch), dim3(256), 0, s, A, Bm, n, ld,
+                     stride, out, ldo);
+  PPH_CHECK_LAUNCH();
+}
+
+__global__ void k_axpy_cols(cplx<double>* Y, const cplx<double>* X, const double* sc, double a,
+                            int64_t n, int64_t ld, int64_t stride, int64_t lds) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y, b = blockIdx.z;
+  if (i >= n) return;
+  const double f = a * sc[b * lds + c];
+  const int64_t o = b * stride + c * ld + i;
+  Y[o] = cplx<double>{Y[o].x + f * X[o].x, Y[o].y + f * X[o].y};
+}
+
+void launch_axpy_cols(cplx<double>* Y, const cplx<double>* X, const double* sc, double a, int64_t n,
+                      int64_t k, int64_t ld, int64_t stride, int64_t lds, int64_t batch,
+                      hipStream_t s) {
+  if (n <= 0 || k <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)ceil_div(n, 256), (unsigned)k, (unsigned)batch),
+                     dim3(256), 0, s, Y, X, sc, a, n, ld, stride, lds);
+  PPH_CHECK_LAUNCH();
+}
+
+__global__ void k_scale_rows(cplx<double>* T, const double* w, int64_t m, int64_t ld, int64_t stride,
+                             int64_t ldw) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y, b = blockIdx.z;
+  if (j >= m) return;
+  const double f = w[b * ldw + j];
+  const int64_t o = b * stride + c * ld + j;
+  // rows with weight 0 are cleared, whatever they hold
+  T[o] = (f == 0.0) ? cplx<double>{0.0, 0.0} : cplx<double>{f * T[o].x, f * T[o].y};
+}
+
+void launch_scale_rows(cplx<double>* T, const double* w, int64_t m, int64_t k, int64_t ld,
+                       int64_t stride, int64_t ldw, int64_t batch, hipStream_t s) {
+  if (m <= 0 || k <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)ceil_div(m, 256), (unsigned)k, (unsigned)batch),
+                     dim3(256), 0, s, T, w, m, ld, stride, ldw);
+  PPH_CHECK_LAUNCH();
+}
+
+// conjugate-gradient step per column: al = rho / pq, z += al p, r -= al q;
+// a column whose rho or pq is not positive (converged, or empty) stands still
+__global__ void k_cg_update(cplx<double>* z, cplx<double>* r, const cplx<double>* p,
+                            const cplx<double>* q, const double* rho, const double* pq, int64_t n,
+                            int64_t ld, int64_t stride, int64_t lds) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y, b = blockIdx.z;
+  if (i >= n) return;
+  const double rh = rho[b * lds + c], d = pq[b * lds + c];
+  const double al = (rh > 0.0 && d > 0.0) ? rh / d : 0.0;
+  const int64_t o = b * stride + c * ld + i;
+  z[o] = cplx<double>{z[o].x + al * p[o].x, z[o].y + al * p[o].y};
+  r[o] = cplx<double>{r[o].x - al * q[o].x, r[o].y - al * q[o].y};
+}
+
+// new direction per column: p = r + (rho_new / rho_old) p
+__global__ void k_cg_direction(cplx<double>* p, const cplx<double>* r, const double* rho_new,
+                               const double* rho_old, int64_t n, int64_t ld, int64_t stride,
+                               int64_t lds) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y, b = blockIdx.z;
+  if (i >= n) return;
+  const double ro = rho_old[b * lds + c];
+  const double be = (ro > 0.0) ? rho_new[b * lds + c] / ro : 0.0;
+  const int64_t o = b * stride + c * ld + i;
+  p[o] = cplx<double>{r[o].x + be * p[o].x, r[o].y + be * p[o].y};
+}
+
+void launch_cg_update(cplx<double>* z, cplx<double>* r, const cplx<double>* p,
+                      const cplx<double>* q, const double* rho, const double* pq, int64_t n,
+                      int64_t k, int64_t ld, int64_t stride, int64_t lds, int64_t batch,
+                      hipStream_t s) {
+  if (n <= 0 || k <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_cg_update, dim3((unsigned)ceil_div(n, 256), (unsigned)k, (unsigned)batch),
+                     dim3(256), 0, s, z, r, p, q, rho, pq, n, ld, stride, lds);
+  PPH_CHECK_LAUNCH();
+}
+
+void launch_cg_direction(cplx<double>* p, const cplx<double>* r, const double* rho_new,
+                         const double* rho_old, int64_t n, int64_t k, int64_t ld, int64_t stride,
+                         int64_t lds, int64_t batch, hipStream_t s) {
+  if (n <= 0 || k <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_cg_direction, dim3((unsigned)ceil_div(n, 256), (unsigned)k, (unsigned)batch),
+                     dim3(256), 0, s, p, r, rho_new, rho_old, n, ld, stride, lds);
+  PPH_CHECK_LAUNCH();
+}
+
 template <typename Ti, typename To>
 __global__ void k_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;

@@ -308,3 +308,62 @@ def test_run_selections_in_lanes(gpu_available, prec, sel):
         # layers is rounding noise on its own scale)
         vtol = (1e-7 if prec == "D" else 1e-5) * np.abs(expect).max() + 1e-9
         assert np.abs(v - expect).max() <= vtol, np.abs(v - expect).max()
+
+
+def _reflect(B, rng):
+    """B <- H B H for a random complex Householder H = I - 2 u u^H, in place
+    (O(n^2): the spectrum of B is unchanged)."""
+    n = B.shape[0]
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u /= np.linalg.norm(u)
+    x = B @ u
+    c = np.vdot(u, x).real
+    y = 2.0 * x - 2.0 * c * u
+    B -= np.outer(u, y.conj())
+    B -= np.outer(y, u.conj())
+
+
+@pytest.mark.parametrize("prec", ["D", "S"])
+def test_heevx_large_n_known_spectrum(gpu_available, prec):
+    """n above the size whose tridiagonal fits the eigensolver's LDS (9216):
+    bisection and inverse iteration read it from global memory.  A = H2 H1
+    diag(lam) H1 H2 has the spectrum lam exactly, with a planted triple and a
+    pair 1e-10 apart among the selected lowest modes."""
+    lib = _lib()
+    rng = np.random.default_rng(9300)
+    n = 9300
+    lam = np.sort(rng.uniform(-2.0, 8.0, n))
+    lam[:3] = -2.5
+    lam[3], lam[4] = -2.4, -2.4 + 1e-10
+    A = np.diag(lam).astype(np.complex128)
+    _reflect(A, rng)
+    _reflect(A, rng)
+    A = 0.5 * (A + A.conj().T)
+    il, iu = 1, 40
+    w, Z, info = lib.heevx(A, jobz="V", rng="I", il=il, iu=iu, precision=prec)
+    nrm = np.abs(lam).max()
+    assert info == 0 and len(w) == iu - il + 1
+    tol = 1e-12 if prec == "D" else 1e-6
+    assert np.abs(w - lam[il - 1:iu]).max() <= tol * nrm
+    R = A @ Z - Z * w
+    assert np.linalg.norm(R, axis=0).max() <= (1e-11 if prec == "D" else 1e-5) * nrm
+    assert np.abs(Z.conj().T @ Z - np.eye(len(w))).max() <= (1e-10 if prec == "D" else 1e-5)
+
+
+def test_range_i_fp32_low_modes_at_gamma(gpu_available):
+    """FP32 + refinement with a range-I selection from the bottom: the acoustic
+    triplet at Gamma (omega = K sqrt(lambda) near 0, where eigenvalue errors are
+    magnified most) and the rest of the selection agree with the FP64 path to
+    1e-3 cm^-1, with the modes above the selection never computed."""
+    lib = _lib()
+    from parphom_amd import synth
+    st = synth.tbg_structure(8, 9)
+    fc = synth.force_constants(st, 2008)
+    q = np.zeros((1, 3))
+    out = {}
+    for prec, rng in (("S", "I"), ("D", "A")):
+        with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="N", rng=rng, il=1, iu=40,
+                        precision=prec, batch=1) as h:
+            out[prec] = h.run(q)["eval_cm"][0]
+    assert len(out["S"]) == 40
+    assert np.abs(out["S"] - out["D"][:40]).max() <= 1e-3
