@@ -38,6 +38,14 @@ int pph_profile(void* h, int32_t enable, int32_t reset, double* ms, int64_t* cou
  * The defaults (32, 128) are the measured best at n = 4764; the others are
  * supported so the kernels can be checked at every blocking. */
 int pph_debug_blocking(void* h, int32_t nb_panel, int32_t nb_back);
+/* Products with D(q) and dD/dq in the Rayleigh quotients and velocities:
+ * mode 0 dense (matrix-core GEMMs with the Hermitian-dot epilogue), 1
+ * automatic (block-sparse when at most a quarter of the force-constant
+ * blocks are nonzero; the default), 2 block-sparse whenever the handle built
+ * block lists (N <= 2048 or sparse).  nblk (may be NULL) receives the number
+ * of lower blocks the products now use, 0 for dense.  No counterpart in the
+ * reference, whose velocity products are dense (write_output.F90:415-421). */
+int pph_debug_sparse(void* h, int32_t mode, int64_t* nblk);
 
 #ifdef __cplusplus
 }

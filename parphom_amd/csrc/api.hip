@@ -156,6 +156,7 @@ struct Lane {
   DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
+  DevBuf bv, Zt;           // block values of D / dD, mode-contiguous vectors (block-sparse forms)
   // rocBLAS handle for the plain GEMMs of the back-transformation (made on
   // first use, bound to this lane's stream, atomics off so results repeat)
   rocblas_handle blas = nullptr;
@@ -196,6 +197,15 @@ struct Handle : Lane {
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
   int nchunk = 64;
+  // Nonzero 3x3 blocks (a, b), a >= b, of Phi_sym listed by row a (sp_rowptr,
+  // with sp_row / sp_col per block).  The quadratic forms of the refinement
+  // and the velocities use them instead of dense products when at most a
+  // quarter of all blocks are nonzero (a cutoff model in a large cell).
+  DevBuf sp_rowptr, sp_row, sp_col;
+  int64_t sp_nblk = 0;
+  bool sp_ok = false;
+  int sp_mode = 1;         // 0 dense products, 1 automatic, 2 block lists whenever built
+  bool use_sparse() const { return sp_nblk > 0 && (sp_mode == 2 || (sp_mode == 1 && sp_ok)); }
   // lanes 1.. (created on demand by pph_run)
   std::vector<std::unique_ptr<Lane>> more;
   Lane& lane(size_t i) { return i == 0 ? static_cast<Lane&>(*this) : *more[i - 1]; }
@@ -697,17 +707,28 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       L.num.ensure((size_t)B * mcap * sizeof(double));
       L.den.ensure((size_t)B * mcap * sizeof(double));
       launch_fill_zero(L.num.p, B * mcap * sizeof(double), s);
-      GemmParams<double> g{};
-      g.opA = 'N'; g.opB = 'N';
-      g.m = n; g.n = mmax; g.k = n;
-      g.alpha = cplx<double>{1, 0};
-      g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
-      g.B = L.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
-      g.batch = B;
-      g.dot_out = L.num.as<double>(); g.dot_stride = mcap;
-      g.herm_dot = 1;            // D is Hermitian: lower triangle, half the work
-      g.dotX = L.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
-      launch_gemm<double>(g, s);
+      if (h.use_sparse()) {
+        // from the nonzero blocks of D, with the FP32 vectors in a
+        // mode-contiguous copy (exact in FP64)
+        L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<float>));
+        launch_transpose<float>(L.Z.as<cplx<float>>(), n, mmax, n, n * mcap, L.Zt.as<cplx<float>>(),
+                                mcap, n * mcap, B, s);
+        launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                                  L.bv.as<cplx<double>>(), 0, 1, L.Zt.as<cplx<float>>(), mcap,
+                                  n * mcap, h.N, mmax, B, L.num.as<double>(), 0, mcap, s);
+      } else {
+        GemmParams<double> g{};
+        g.opA = 'N'; g.opB = 'N';
+        g.m = n; g.n = mmax; g.k = n;
+        g.alpha = cplx<double>{1, 0};
+        g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
+        g.B = L.Z64.as<cplx<double>>(); g.ldb = n; g.strideB = n * mcap;
+        g.batch = B;
+        g.dot_out = L.num.as<double>(); g.dot_stride = mcap;
+        g.herm_dot = 1;            // D is Hermitian: lower triangle, half the work
+        g.dotX = L.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
+        launch_gemm<double>(g, s);
+      }
       launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
                               mcap, s);
       launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
@@ -736,12 +757,14 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   const bool fp32 = (o.precision == 'S');
   const bool vel = o.vel != 0;
   const bool want_vec = vel || o.jobz == 'V';
+  const bool sparse = h.use_sparse();
 
   L.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
   L.A64.ensure((size_t)B * h.ldA * n * sizeof(cplx<double>));
   if (fp32) L.A32.ensure((size_t)B * h.ldA * n * sizeof(cplx<float>));
-  if (vel) L.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
+  if (vel && !sparse)
+    L.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
 
   AsmParams ap{};
   ap.phi = h.phi.as<double>(); ap.crys = h.crys.as<double>(); ap.lat = h.lat.as<double>();
@@ -751,13 +774,19 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   ap.D64 = L.A64.as<cplx<double>>();
   ap.D32 = fp32 ? L.A32.as<cplx<float>>() : nullptr;
   ap.ldD = h.ldA;
-  if (vel) {
+  if (vel && !sparse) {
     if (fp32) ap.dD32 = L.dD.as<cplx<float>>();
     else ap.dD64 = L.dD.as<cplx<double>>();
   }
   L.tm.begin(PH_TOTAL, s);
   L.tm.begin(PH_ASM, s);
   launch_assemble(ap, s);
+  if (sparse) {
+    // D (for the FP32 path's Rayleigh quotients) and dD of the nonzero blocks
+    L.bv.ensure((size_t)B * h.sp_nblk * 36 * sizeof(cplx<double>));
+    launch_sparse_blocks(ap, h.sp_row.as<int32_t>(), h.sp_col.as<int32_t>(), h.sp_nblk, vel ? 4 : 1,
+                         L.bv.as<cplx<double>>(), s);
+  }
   L.tm.end(PH_ASM, s);
 
   const double K32 = (double)(float)(15.633302f * 33.35641f);
@@ -777,7 +806,21 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
     L.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
     launch_fill_zero(L.vacc.p, B * mcap * 3 * sizeof(double), s);
     L.tm.begin(PH_VEL, s);
-    if (fp32) {
+    if (sparse) {
+      // the FP32 path transposed its vectors for the Rayleigh quotients
+      if (!fp32) {
+        L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
+        launch_transpose<double>(L.Z.as<cplx<double>>(), n, mmax, n, n * mcap,
+                                 L.Zt.as<cplx<double>>(), mcap, n * mcap, B, s);
+        launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                                   L.bv.as<cplx<double>>(), 1, 3, L.Zt.as<cplx<double>>(), mcap,
+                                   n * mcap, N, mmax, B, L.vacc.as<double>(), mcap, 3 * mcap, s);
+      } else {
+        launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                                  L.bv.as<cplx<double>>(), 1, 3, L.Zt.as<cplx<float>>(), mcap,
+                                  n * mcap, N, mmax, B, L.vacc.as<double>(), mcap, 3 * mcap, s);
+      }
+    } else if (fp32) {
       GemmParams<float> g{};
       g.opA = 'N'; g.opB = 'N';
       g.m = n; g.n = mmax; g.k = n;
@@ -815,13 +858,16 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
 // per-column launches of the reduction busy), at most 64.  Per matrix: D in
 // FP64 (16 n^2) and FP32 (8 n^2), dD (48 or 96 n^2), Householder vectors,
 // eigenvectors (FP32 + FP64 copies, the FP64 tridiagonal vectors and their
-// LU factors), about 6 n^2 x 16 bytes more.
+// LU factors), about 6 n^2 x 16 bytes more; with block-sparse forms no dD but
+// a mode-contiguous copy of the vectors.
 static int64_t auto_batch(const Handle& h, int64_t nq, int64_t lanes) {
   if (h.opts.batch > 0) return std::min(nq, h.opts.batch);
   const int64_t n = h.n;
   const bool s = h.opts.precision == 'S';
-  const double per = (double)n * n * (16.0 + (s ? 8.0 : 0.0) + (h.opts.vel ? (s ? 48.0 : 96.0) : 0.0) +
-                                      96.0);
+  const bool sp = h.use_sparse();
+  const double per = (double)n * n * (16.0 + (s ? 8.0 : 0.0) +
+                                      (h.opts.vel && !sp ? (s ? 48.0 : 96.0) : 0.0) +
+                                      (sp ? (s ? 8.0 : 16.0) : 0.0) + 96.0);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)24e9;
   int64_t b = (int64_t)std::floor(0.5 * (double)free_b / std::max<int64_t>(1, lanes) /
@@ -873,6 +919,47 @@ const char* pph_version(void) { return "parphom_hip 0.1 (gfx950)"; }
 const char* pph_last_error(const pph_handle* h) {
   if (h) return reinterpret_cast<const Handle*>(h)->err.c_str();
   return g_last_error.c_str();
+}
+
+// Nonzero 3x3 blocks of Phi_sym, from the device copy.  The lower lists
+// (a >= b, the diagonal always) are kept when at most a quarter of all N^2
+// blocks are nonzero, or for N <= 2048 (so that the block-sparse path can be
+// selected for testing).
+static void build_block_lists(Handle& h) {
+  const int64_t N = h.N;
+  hipStream_t s = h.stream;
+  std::vector<uint8_t> flags((size_t)N * N);
+  {
+    DevBuf fl;
+    fl.ensure((size_t)N * N);
+    launch_block_nz(h.phi.as<double>(), fl.as<uint8_t>(), N, s);
+    PPH_HIP(hipMemcpyAsync(flags.data(), fl.p, (size_t)N * N, hipMemcpyDeviceToHost, s));
+    PPH_HIP(hipStreamSynchronize(s));
+  }
+  int64_t full = 0;                 // nonzero blocks of the whole matrix
+  for (size_t i = 0; i < flags.size(); ++i) full += flags[i];
+  const int64_t cap = N * N / 4;
+  if (full > cap && N > 2048) return;
+  std::vector<int64_t> rowptr(N + 1, 0);
+  std::vector<int32_t> row, col;
+  for (int64_t a = 0; a < N; ++a) {
+    const uint8_t* fa = flags.data() + (size_t)a * N;      // flags(b, a) = flags(a, b)
+    for (int64_t b = 0; b <= a; ++b)
+      if (b == a || fa[b]) {
+        row.push_back((int32_t)a);
+        col.push_back((int32_t)b);
+      }
+    rowptr[a + 1] = (int64_t)row.size();
+  }
+  h.sp_ok = full <= cap;
+  h.sp_nblk = (int64_t)row.size();
+  h.sp_rowptr.ensure((N + 1) * sizeof(int64_t));
+  h.sp_row.ensure(h.sp_nblk * sizeof(int32_t));
+  h.sp_col.ensure(h.sp_nblk * sizeof(int32_t));
+  PPH_HIP(hipMemcpyAsync(h.sp_rowptr.p, rowptr.data(), (N + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(h.sp_row.p, row.data(), h.sp_nblk * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipMemcpyAsync(h.sp_col.p, col.data(), h.sp_nblk * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PPH_HIP(hipStreamSynchronize(s));
 }
 
 int pph_create(const pph_system* sys, const double* fc, const pph_opts* opts, pph_handle** out) {
@@ -951,6 +1038,7 @@ int pph_create(const pph_system* sys, const double* fc, const pph_opts* opts, pp
   PPH_HIP(hipMemcpyAsync(h->cd_dc.p, dc, 9 * sizeof(double), hipMemcpyHostToDevice, s));
   launch_image_table(h->crys.as<double>(), h->pos.as<double>(), h->lat.as<double>(),
                      h->mask.as<uint16_t>(), N, s);
+  build_block_lists(*h);
   PPH_HIP(hipStreamSynchronize(s));
   *out = reinterpret_cast<pph_handle*>(h.release());
   return PPH_OK;
@@ -1394,6 +1482,18 @@ int pph_destroy(pph_handle* hh) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- measurement hook
+extern "C" int pph_debug_sparse(pph_handle* hh, int32_t mode, int64_t* nblk) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_debug_sparse: handle is NULL", PPH_ERR_ARG);
+  if (mode < 0 || mode > 2) return fail(h, "pph_debug_sparse: mode must be 0, 1 or 2", PPH_ERR_ARG);
+  if (mode == 2 && h->sp_nblk == 0)
+    return fail(h, "pph_debug_sparse: no block lists (more than a quarter of the blocks nonzero)",
+                PPH_ERR_ARG);
+  h->sp_mode = mode;
+  if (nblk) *nblk = h->use_sparse() ? h->sp_nblk : 0;
+  return PPH_OK;
+}
+
 extern "C" int pph_debug_blocking(pph_handle* hh, int32_t nb_panel, int32_t nb_back) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return fail(nullptr, "pph_debug_blocking: handle is NULL", PPH_ERR_ARG);

@@ -254,6 +254,118 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
   }
 }
 
+// Block values of the nonzero 3x3 blocks (a, b), a >= b, of D(q) and, with
+// nops = 4, of dD/dq_{x,y,z}: one thread per (block, q).  Each element is
+// formed by the same operations as k_assemble forms it (pair phase sums,
+// then ((Phi p) / multiplicity) / sqrt(m_a m_b) with correctly rounded
+// quotients), so the values equal the dense matrices' elements bit for bit.
+// Output bv[((q * nblk + k) * 4 + op) * 9 + 3 alpha + beta] =
+// op(3a + alpha, 3b + beta), op 0 = D, 1..3 = dD/dq_{x,y,z}.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_sparse_blocks(AsmArgs A, const int32_t* __restrict__ brow,
+                                                       const int32_t* __restrict__ bcol,
+                                                       int64_t nblk, int nops,
+                                                       cplx<double>* __restrict__ bv) {
+#pragma clang fp contract(off)
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t iq = blockIdx.y;
+  if (k >= nblk) return;
+  const int64_t N = A.N, n = 3 * N;
+  const int64_t i = brow[k], j = bcol[k];
+  double L[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L[r][c] = A.lat[r + 3 * c];
+  const uint16_t m = A.mask[i + j * N];
+  const double mult = (double)__popc((unsigned)m);
+  const double smm = sqrt(A.rmass[i] * A.rmass[j]);
+  const double rmult = 1.0 / mult, rsmm = 1.0 / smm;
+  auto scale = [&](double fp) { return div_rn(div_rn(fp, mult, rmult), smm, rsmm); };
+  const double ci0 = A.crys[i], ci1 = A.crys[i + N], ci2 = A.crys[i + 2 * N];
+  const double cj0 = A.crys[j], cj1 = A.crys[j + N], cj2 = A.crys[j + 2 * N];
+  const double* q = A.q + 3 * iq;
+  const bool deriv = nops > 1;
+  PhaseOut po;
+  pair_phase(q, ci0, ci1, ci2, cj0, cj1, cj2, m, L, deriv && MODE == 0, po);
+  double dr[3][2] = {}, er[3][2] = {};      // derivative sums (MODE 0) or q+ / q- sums (MODE 1)
+  if (deriv) {
+    for (int a = 0; a < 3; ++a) {
+      if (MODE == 1) {
+        double qp[3], qm[3];
+        for (int c = 0; c < 3; ++c) {
+          qp[c] = q[c] + A.cd_dc[3 * a + c] / 2.0;
+          qm[c] = q[c] - A.cd_dc[3 * a + c] / 2.0;
+        }
+        PhaseOut pp, pm;
+        pair_phase(qp, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pp);
+        pair_phase(qm, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pm);
+        dr[a][0] = pp.p0r; dr[a][1] = pp.p0i;
+        er[a][0] = pm.p0r; er[a][1] = pm.p0i;
+      } else {
+        dr[a][0] = po.pdr[a]; dr[a][1] = po.pdi[a];
+      }
+    }
+  }
+  cplx<double>* out = bv + (iq * nblk + k) * 36;
+#pragma unroll
+  for (int al = 0; al < 3; ++al)
+#pragma unroll
+    for (int be = 0; be < 3; ++be) {
+      const double fv = A.phi[(3 * i + al) + (3 * j + be) * n];
+      out[3 * al + be] = cplx<double>{scale(fv * po.p0r), scale(fv * po.p0i)};
+      if (!deriv) continue;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double wr, wi;
+        if (MODE == 1) {
+          wr = (scale(fv * dr[d][0]) - scale(fv * er[d][0])) / A.cd_norm;
+          wi = (scale(fv * dr[d][1]) - scale(fv * er[d][1])) / A.cd_norm;
+        } else {
+          wr = scale(fv * dr[d][0]);
+          wi = scale(fv * dr[d][1]);
+        }
+        out[9 * (d + 1) + 3 * al + be] = cplx<double>{wr, wi};
+      }
+    }
+}
+
+void launch_sparse_blocks(const AsmParams& P, const int32_t* brow, const int32_t* bcol, int64_t nblk,
+                          int nops, cplx<double>* bv, hipStream_t s) {
+  if (nblk <= 0 || P.nb <= 0) return;
+  AsmArgs A{};
+  A.phi = P.phi; A.crys = P.crys; A.lat = P.lat; A.rmass = P.rmass; A.mask = P.mask;
+  A.q = P.q; A.cd_dc = P.cd_dc; A.cd_norm = P.cd_norm;
+  A.N = P.N; A.nb = P.nb; A.mode = P.mode;
+  dim3 grid((unsigned)ceil_div(nblk, 256), (unsigned)P.nb);
+  if (P.mode == 1)
+    hipLaunchKernelGGL(k_sparse_blocks<1>, grid, dim3(256), 0, s, A, brow, bcol, nblk, nops, bv);
+  else
+    hipLaunchKernelGGL(k_sparse_blocks<0>, grid, dim3(256), 0, s, A, brow, bcol, nblk, nops, bv);
+  PPH_CHECK_LAUNCH();
+}
+
+// flags[a + b N] = 1 when the 3x3 block (a, b) of Phi_sym has a nonzero entry
+// (Phi_sym is symmetric bit for bit, so the flags are too)
+__global__ void k_block_nz(const double* __restrict__ phi, uint8_t* __restrict__ flags, int64_t N) {
+  const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = blockIdx.y;
+  if (a >= N) return;
+  const int64_t n = 3 * N;
+  bool nz = false;
+#pragma unroll
+  for (int be = 0; be < 3; ++be)
+#pragma unroll
+    for (int al = 0; al < 3; ++al) nz |= phi[(3 * a + al) + (3 * b + be) * n] != 0.0;
+  flags[a + b * N] = nz ? 1 : 0;
+}
+
+void launch_block_nz(const double* phi, uint8_t* flags, int64_t N, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(N, 256), (unsigned)N);
+  hipLaunchKernelGGL(k_block_nz, grid, dim3(256), 0, s, phi, flags, N);
+  PPH_CHECK_LAUNCH();
+}
+
 void launch_symmetrise_fc(const double* fc_dev, double* phi, int64_t N, hipStream_t s) {
   const int64_t n = 3 * N;
   dim3 grid((unsigned)ceil_div(n, 256), (unsigned)n);

@@ -41,7 +41,8 @@ class PphOut(C.Structure):
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
             "pph_dos_triangulation", "pph_dos_smearing", "pph_fc_symmetrize", "pph_destroy",
             "pph_last_error",
-            "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking")
+            "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
+            "pph_debug_sparse")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
           "hemv_launches")
@@ -76,6 +77,8 @@ def load(path: str = LIB_PATH):
     lib.pph_profile.restype = C.c_int
     lib.pph_debug_blocking.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.pph_debug_blocking.restype = C.c_int
+    lib.pph_debug_sparse.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
+    lib.pph_debug_sparse.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -208,6 +211,14 @@ class Handle:
         """Reflectors per reduction panel and per back-transformation block
         (a test hook: the defaults are the measured best)."""
         _check(load().pph_debug_blocking(self._h, nb_panel, nb_back), self._h)
+
+    def sparse(self, mode=1):
+        """Products with D and dD in the Rayleigh quotients and velocities:
+        0 dense, 1 automatic (the default), 2 block-sparse whenever block lists
+        exist.  Returns the number of lower blocks now used (0 = dense)."""
+        nblk = C.c_int64(0)
+        _check(load().pph_debug_sparse(self._h, mode, C.byref(nblk)), self._h)
+        return int(nblk.value)
 
     def profile(self, enable=True, reset=False):
         """Accumulated phase times (ms) and batch counts, then set recording."""

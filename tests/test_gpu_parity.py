@@ -134,8 +134,11 @@ def test_heevx_ranges(gpu_available):
 
 
 @pytest.mark.parametrize("prec", ["D", "S"])
-def test_run_tbg_3_4_path(gpu_available, prec):
-    """Config-1 structure: eigenvalues (<= 1e-3 cm^-1) and velocities."""
+@pytest.mark.parametrize("sparse", [1, 2])
+def test_run_tbg_3_4_path(gpu_available, prec, sparse):
+    """Config-1 structure: eigenvalues (<= 1e-3 cm^-1) and velocities, with the
+    dense products (automatic: 30% of its blocks are nonzero) and with the
+    block-sparse quadratic forms forced."""
     lib = _lib()
     st, fc = U.structure(3, 4, 1003)
     S = U.oracle_system(3, 4, 1003)
@@ -143,6 +146,7 @@ def test_run_tbg_3_4_path(gpu_available, prec):
     qs = synth.q_path_gkmg()[[0, 5, 26, 30, 39, 50]]
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
                     precision=prec) as h:
+        assert (h.sparse(sparse) > 0) == (sparse == 2)
         res = h.run(qs)
     for iq, q in enumerate(qs):
         ref = S.solve(q, vel=True)
@@ -367,3 +371,47 @@ def test_range_i_fp32_low_modes_at_gamma(gpu_available):
             out[prec] = h.run(q)["eval_cm"][0]
     assert len(out["S"]) == 40
     assert np.abs(out["S"] - out["D"][:40]).max() <= 1e-3
+
+
+@pytest.mark.parametrize("prec", ["D", "S"])
+def test_block_sparse_products_match_dense(gpu_available, prec):
+    """The (8,9) cell's cutoff model has 5% of its force-constant blocks
+    nonzero, so the Rayleigh quotients and velocities take the block-sparse
+    path by default.  FP64 path: against the dense products (a second run,
+    same vectors), frequencies to 1e-6 cm^-1 and velocities to 1e-8 of the
+    largest.  FP32 path: its vectors change from run to run inside
+    near-degenerate pairs (float atomics in the reduction), so its velocities
+    are checked against FP64 forms Re(z^H dD z) formed on the host from the
+    same run's vectors (the oracle's dD), to 1e-8 of the largest, for the 30
+    lowest and 200 sampled modes; its frequencies against the dense run's to
+    1e-5 cm^-1 (they move by ~1e-6 between runs with the vectors)."""
+    lib = _lib()
+    from parphom_amd import synth
+    st = synth.tbg_structure(8, 9)
+    fc = synth.force_constants(st, 2008)
+    qs = synth.q_grid_shifted(16, 2)[[3, 77]]
+    with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                    precision=prec, batch=2, streams=1) as h:
+        assert h.sparse(1) > 0
+        r1 = h.run(qs)
+        assert h.sparse(0) == 0
+        r0 = h.run(qs)
+    for iq in range(len(qs)):
+        assert r1["info"][iq] == 0 and r0["info"][iq] == 0
+        # FP32: the refined frequencies move by ~1e-6 cm^-1 between runs with the vectors
+        assert np.abs(r1["eval_cm"][iq] - r0["eval_cm"][iq]).max() <= (1e-6 if prec == "D" else 1e-5)
+        if prec == "D":
+            v1, v0 = r1["vel"][iq], r0["vel"][iq]
+            assert np.abs(v1 - v0).max() <= 1e-8 * np.abs(v0).max()
+    if prec == "S":
+        Sys = O.System(st.lat, st.pos, st.types, st.masses, fc)
+        q = qs[0]
+        dD = [Sys.D(q, a) for a in (1, 2, 3)]
+        n = 3 * st.natom
+        pick = np.concatenate([np.arange(30),
+                               np.random.default_rng(1).choice(np.arange(30, n), 200, replace=False)])
+        Z = r1["evec"][0][:, pick]
+        om = r1["eval_cm"][0][pick]
+        v_host = U.vel_projected(Z, dD, om, [[k] for k in range(len(pick))])
+        v1 = r1["vel"][0][pick]
+        assert np.abs(v1 - v_host).max() <= 1e-8 * np.abs(v_host).max()
