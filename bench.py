@@ -50,10 +50,12 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams):
     library keeps `streams` batches of B in flight (one stream and host thread
     each).  Returns the timed seconds, the phase profile of the timed steps
     (concurrent batches), the profile of one batch run alone, and the
-    frequencies of the first B q-points."""
+    frequencies of the first B q-points, and the number of force-constant
+    blocks the Rayleigh quotients and velocities use (0: dense products)."""
     n = 3 * st.natom
     h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision=prec,
                    batch=B, streams=streams)
+    nblk = h.sparse(1)               # blocks the quadratic forms use (0 = dense products)
     per_step = B * streams
     out = device_outputs(torch, per_step, n)
     nq = len(qs)
@@ -82,7 +84,7 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams):
     h.close()
     del out
     torch.cuda.empty_cache()
-    return dt, prof, prof_iso, ev, z0
+    return dt, prof, prof_iso, ev, z0, nblk
 
 
 def split_profile(prof, n):
@@ -137,15 +139,21 @@ def path_roofline(N, n, B, qps, peak_spec, peak_meas):
     vectors + 24 n^3 for the three velocity products) and the assembly bytes
     72 N^2 / B (force constants once per batch) + 4 * 16 n^2 (D and three dD in
     c128).  Refinement, the tridiagonal eigensolver and the work the Hermitian
-    products save are not counted."""
+    products save are not counted.  When the velocity forms use only the
+    nonzero force-constant blocks, the 24 n^3 of dense velocity products are
+    not performed at all; `frac_eigensolve_only` then leaves them out of the
+    ideal (16/3 n^3 + 8 n^3 and the assembly bytes)."""
     flops = (16.0 / 3.0 + 8.0 + 24.0) * float(n) ** 3
+    flops_eig = (16.0 / 3.0 + 8.0) * float(n) ** 3
     k1 = 72.0 * N * N / B + 4 * 16.0 * n * n
     t_meas = 1.0 / qps
     t_spec = flops / peak_spec + k1 / HBM_PEAK
     t_meas_peak = flops / peak_meas + k1 / HBM_PEAK
+    t_eig = flops_eig / peak_spec + k1 / HBM_PEAK
     return {"flops_per_q": flops, "assembly_bytes_per_q": k1,
             "ideal_ms_per_q": 1e3 * t_spec, "measured_ms_per_q": 1e3 * t_meas,
-            "frac": t_spec / t_meas, "frac_vs_measured_peak": t_meas_peak / t_meas}
+            "frac": t_spec / t_meas, "frac_vs_measured_peak": t_meas_peak / t_meas,
+            "frac_eigensolve_only": t_eig / t_meas}
 
 
 def cpu_baseline(st, fc, q, seconds_cap=60.0):
@@ -425,8 +433,8 @@ def main():
         f"(inputs built in {time.perf_counter() - t0:.1f}s)")
 
     B, S = a.batch, a.streams
-    dt32, prof32, iso32, ev32, z32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup,
-                                               S)
+    dt32, prof32, iso32, ev32, z32, nblk = time_path(lib, torch, st, fc, qs, "S", B, a.steps,
+                                                     a.warmup, S)
     qps32 = a.steps * B * S / dt32
     res = {"metric": "q-pts/s at 3N=4764 eigpairs+v_g: FP32+refine vs FP64, max|dw|<=1e-3 cm^-1",
            "value": qps32, "unit": "q-pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -442,21 +450,42 @@ def main():
     iso, iso_launch_ms = split_profile(iso32, n)
     res["phase_ms_one_batch_alone"] = {k: round(v, 3) for k, v in iso.items()}
 
-    # roofline of the assembly kernel (one launch per batch, HBM-bound):
-    # algorithmic bytes = Phi (8 n^2) + B * (D in c128 and c64, dD in c64)
-    asm_bytes = 8 * n * n + B * (16 + 8 + 3 * 8) * n * n
-    asm_s = phases["assemble"] / 1e3
-    # roofline of the velocity products (one launch per batch, 3 directions):
-    # diag(Z^H dD Z) with dD Hermitian is formed from its lower triangle,
-    # 2 Re(Z^H A' Z), a triangular n x n times n x n complex product: n^3 / 2
-    # complex multiply-adds = 4 n^3 flops per direction, B * 12 n^3 per launch
-    vel_flops = B * 12.0 * n ** 3
+    # roofline of the assembly phase (HBM-bound): algorithmic bytes = Phi
+    # (8 n^2) + B * (D in c128 and c64) and either dD in c64 (dense products)
+    # or the four phase sums of D and dD on the nblk nonzero blocks (4 c128
+    # each, k_sparse_blocks)
+    asm_bytes = 8 * n * n + B * (16 + 8) * n * n + (B * 4 * 16.0 * nblk if nblk > 0
+                                                    else B * 3 * 8 * n * n)
+    # the assembly is one launch per batch; in the timed steps it shares the
+    # GPU with the other lane's reduction, so it is rated on the batch run alone
+    asm_s = iso["assemble"] / 1e3
+    # roofline of the velocity forms (one launch per batch, 3 directions).
+    # Dense products: diag(Z^H dD Z) with dD Hermitian is formed from its lower
+    # triangle, 2 Re(Z^H A' Z), a triangular n x n times n x n complex product:
+    # n^3 / 2 complex multiply-adds = 4 n^3 flops per direction, B * 12 n^3 per
+    # launch, on the FP32 matrix cores.  Block-sparse forms (nblk lower 3x3
+    # blocks, each a phase sum times a real block Phi~): per block and mode
+    # c = z_a^H (Phi~ z_b), a real 3x3 block times a complex 3-vector (36
+    # flops) and the dot with the row's 3-vector (24), then per direction
+    # Re(p c) weighted and accumulated (4): 72 flops for the three directions,
+    # in FP64 on the vector units.
     vel_s = phases["velocity"] / 1e3
+    if nblk > 0:
+        vel_flops = B * n * 72.0 * nblk
+        vel_rl = {"bound": "fp64 vector", "achieved": vel_flops / vel_s / 1e12,
+                  "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                  "frac": vel_flops / vel_s / FP64_PEAK, "traffic": None,
+                  "kernel": "k_sparse_dots<float, 3> (velocity forms on %d nonzero "
+                            "force-constant blocks)" % nblk}
+    else:
+        vel_flops = B * 12.0 * n ** 3
+        vel_rl = {"bound": "mfma", "achieved": vel_flops / vel_s / 1e12,
+                  "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                  "frac": vel_flops / vel_s / FP32_PEAK, "traffic": None,
+                  "kernel": "velocity products (k_zgemm dot epilogue, FP32)"}
     rl = {"assemble": {"bound": "hbm", "achieved": asm_bytes / asm_s / 1e9, "peak": HBM_PEAK / 1e9,
                        "unit": "GB/s", "frac": asm_bytes / asm_s / HBM_PEAK, "traffic": None},
-          "velocity": {"bound": "mfma", "achieved": vel_flops / vel_s / 1e12,
-                       "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                       "frac": vel_flops / vel_s / FP32_PEAK, "traffic": None}}
+          "velocity": vel_rl}
     # roofline of the dominant kernel, the hemv of the reduction (k_hemv_sym,
     # one launch per column j, HBM-bound): algorithmic bytes per launch = the
     # lower triangle of the trailing matrix, m (m + 1) / 2 complex64 elements
@@ -482,14 +511,15 @@ def main():
         "bound": "hbm", "achieved": hemv_bytes / iso_s / 1e9, "peak": HBM_PEAK / 1e9,
         "unit": "GB/s", "frac": hemv_bytes / iso_s / HBM_PEAK, "kernel": "k_hemv_sym<float>",
         "avg_launch_us": iso_launch_ms * 1e3}
-    res["roofline_velocity"] = dict(rl["velocity"],
-                                    kernel="velocity products (k_zgemm dot epilogue, FP32)")
-    res["roofline_assembly"] = dict(rl["assemble"], kernel="k_assemble")
+    res["roofline_velocity"] = rl["velocity"]
+    res["config"]["fc_blocks_used"] = nblk
+    res["roofline_assembly"] = dict(rl["assemble"], kernel="k_assemble" + (
+        " + k_sparse_blocks" if nblk > 0 else ""))
 
     if not a.no_fp64:
         # FP64 workspaces are larger: half the batch per stream keeps both in HBM
         B64 = max(1, B // 2) if S > 1 else B
-        dt64, prof64, _, ev64, _ = time_path(lib, torch, st, fc, qs, "D", B64,
+        dt64, prof64, _, ev64, _, _ = time_path(lib, torch, st, fc, qs, "D", B64,
                                              max(1, a.steps - 1), 0 if a.warmup == 0 else 1, S)
         qps64 = max(1, a.steps - 1) * B64 * S / dt64
         ev64 = ev64[:min(B, B64)]

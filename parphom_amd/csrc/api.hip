@@ -156,7 +156,7 @@ struct Lane {
   DevBuf qdev, A64, A32, dD, V, W, tau, dgl, egl, ypart, dpart, scal, part;
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
-  DevBuf bv, Zt;           // block values of D / dD, mode-contiguous vectors (block-sparse forms)
+  DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
   // rocBLAS handle for the plain GEMMs of the back-transformation (made on
   // first use, bound to this lane's stream, atomics off so results repeat)
   rocblas_handle blas = nullptr;
@@ -201,7 +201,7 @@ struct Handle : Lane {
   // with sp_row / sp_col per block).  The quadratic forms of the refinement
   // and the velocities use them instead of dense products when at most a
   // quarter of all blocks are nonzero (a cutoff model in a large cell).
-  DevBuf sp_rowptr, sp_row, sp_col;
+  DevBuf sp_rowptr, sp_row, sp_col, sp_phit;    // + the real block Phi~ of each
   int64_t sp_nblk = 0;
   bool sp_ok = false;
   int sp_mode = 1;         // 0 dense products, 1 automatic, 2 block lists whenever built
@@ -714,8 +714,9 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
         launch_transpose<float>(L.Z.as<cplx<float>>(), n, mmax, n, n * mcap, L.Zt.as<cplx<float>>(),
                                 mcap, n * mcap, B, s);
         launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                  L.bv.as<cplx<double>>(), 0, 1, L.Zt.as<cplx<float>>(), mcap,
-                                  n * mcap, h.N, mmax, B, L.num.as<double>(), 0, mcap, s);
+                                  h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, 1,
+                                  L.Zt.as<cplx<float>>(), mcap, n * mcap, h.N, mmax, B,
+                                  L.num.as<double>(), 0, mcap, s);
       } else {
         GemmParams<double> g{};
         g.opA = 'N'; g.opB = 'N';
@@ -782,8 +783,9 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   L.tm.begin(PH_ASM, s);
   launch_assemble(ap, s);
   if (sparse) {
-    // D (for the FP32 path's Rayleigh quotients) and dD of the nonzero blocks
-    L.bv.ensure((size_t)B * h.sp_nblk * 36 * sizeof(cplx<double>));
+    // phase sums of D (for the FP32 path's Rayleigh quotients) and dD on the
+    // nonzero blocks
+    L.bv.ensure((size_t)B * h.sp_nblk * 4 * sizeof(cplx<double>));
     launch_sparse_blocks(ap, h.sp_row.as<int32_t>(), h.sp_col.as<int32_t>(), h.sp_nblk, vel ? 4 : 1,
                          L.bv.as<cplx<double>>(), s);
   }
@@ -813,12 +815,14 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
         launch_transpose<double>(L.Z.as<cplx<double>>(), n, mmax, n, n * mcap,
                                  L.Zt.as<cplx<double>>(), mcap, n * mcap, B, s);
         launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                   L.bv.as<cplx<double>>(), 1, 3, L.Zt.as<cplx<double>>(), mcap,
-                                   n * mcap, N, mmax, B, L.vacc.as<double>(), mcap, 3 * mcap, s);
+                                   h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 1, 3,
+                                   L.Zt.as<cplx<double>>(), mcap, n * mcap, N, mmax, B,
+                                   L.vacc.as<double>(), mcap, 3 * mcap, s);
       } else {
         launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                  L.bv.as<cplx<double>>(), 1, 3, L.Zt.as<cplx<float>>(), mcap,
-                                  n * mcap, N, mmax, B, L.vacc.as<double>(), mcap, 3 * mcap, s);
+                                  h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 1, 3,
+                                  L.Zt.as<cplx<float>>(), mcap, n * mcap, N, mmax, B,
+                                  L.vacc.as<double>(), mcap, 3 * mcap, s);
       }
     } else if (fp32) {
       GemmParams<float> g{};
@@ -959,6 +963,10 @@ static void build_block_lists(Handle& h) {
   PPH_HIP(hipMemcpyAsync(h.sp_rowptr.p, rowptr.data(), (N + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   PPH_HIP(hipMemcpyAsync(h.sp_row.p, row.data(), h.sp_nblk * sizeof(int32_t), hipMemcpyHostToDevice, s));
   PPH_HIP(hipMemcpyAsync(h.sp_col.p, col.data(), h.sp_nblk * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  h.sp_phit.ensure(h.sp_nblk * 9 * sizeof(double));
+  launch_block_phi(h.phi.as<double>(), h.mask.as<uint16_t>(), h.rmass.as<double>(),
+                   h.sp_row.as<int32_t>(), h.sp_col.as<int32_t>(), h.sp_nblk, N,
+                   h.sp_phit.as<double>(), s);
   PPH_HIP(hipStreamSynchronize(s));
 }
 

@@ -254,23 +254,58 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
   }
 }
 
-// Block values of the nonzero 3x3 blocks (a, b), a >= b, of D(q) and, with
-// nops = 4, of dD/dq_{x,y,z}: one thread per (block, q).  Each element is
-// formed by the same operations as k_assemble forms it (pair phase sums,
-// then ((Phi p) / multiplicity) / sqrt(m_a m_b) with correctly rounded
-// quotients), so the values equal the dense matrices' elements bit for bit.
-// Output bv[((q * nblk + k) * 4 + op) * 9 + 3 alpha + beta] =
-// op(3a + alpha, 3b + beta), op 0 = D, 1..3 = dD/dq_{x,y,z}.
+// Nonzero 3x3 blocks (a, b), a >= b, of D(q) and dD/dq: each is a complex
+// phase sum times the q-independent real block
+//   Phi~_ab = (Phi_sym(a, b) / multiplicity) / sqrt(m_a m_b)
+// (dynamical_matrix.F90:241-258: the phase depends on the atom pair and the
+// tie set only), so a block is stored as Phi~ once per handle plus, per q, the
+// phase sums of D and of dD/dq_{x,y,z} (analytic, or the central difference
+// (p(q+) - p(q-)) / |delta|).  The quotients are correctly rounded as in
+// k_assemble; Phi~ p differs from k_assemble's ((Phi p) / mult) / sqrt(..)
+// only by the rounding of the product order.
+
+// phit[k * 9 + 3 alpha + beta] = Phi~(3a + alpha, 3b + beta) for block k
+__global__ void __launch_bounds__(256) k_block_phi(const double* __restrict__ phi,
+                                                   const uint16_t* __restrict__ mask,
+                                                   const double* __restrict__ rmass,
+                                                   const int32_t* __restrict__ brow,
+                                                   const int32_t* __restrict__ bcol, int64_t nblk,
+                                                   int64_t N, double* __restrict__ phit) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nblk) return;
+  const int64_t n = 3 * N, i = brow[k], j = bcol[k];
+  const double mult = (double)__popc((unsigned)mask[i + j * N]);
+  const double smm = sqrt(rmass[i] * rmass[j]);
+  const double rmult = 1.0 / mult, rsmm = 1.0 / smm;
+#pragma unroll
+  for (int al = 0; al < 3; ++al)
+#pragma unroll
+    for (int be = 0; be < 3; ++be)
+      phit[k * 9 + 3 * al + be] =
+          div_rn(div_rn(phi[(3 * i + al) + (3 * j + be) * n], mult, rmult), smm, rsmm);
+}
+
+void launch_block_phi(const double* phi, const uint16_t* mask, const double* rmass,
+                      const int32_t* brow, const int32_t* bcol, int64_t nblk, int64_t N,
+                      double* phit, hipStream_t s) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_block_phi, dim3((unsigned)ceil_div(nblk, 256)), dim3(256), 0, s, phi, mask,
+                     rmass, brow, bcol, nblk, N, phit);
+  PPH_CHECK_LAUNCH();
+}
+
+// bph[(q * nblk + k) * 4 + op]: phase sum of D (op 0) and, with nops = 4, of
+// dD/dq_{x,y,z} (ops 1..3) for block k; one thread per (block, q)
 template <int MODE>
 __global__ void __launch_bounds__(256) k_sparse_blocks(AsmArgs A, const int32_t* __restrict__ brow,
                                                        const int32_t* __restrict__ bcol,
                                                        int64_t nblk, int nops,
-                                                       cplx<double>* __restrict__ bv) {
+                                                       cplx<double>* __restrict__ bph) {
 #pragma clang fp contract(off)
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t iq = blockIdx.y;
   if (k >= nblk) return;
-  const int64_t N = A.N, n = 3 * N;
+  const int64_t N = A.N;
   const int64_t i = brow[k], j = bcol[k];
   double L[3][3];
 #pragma unroll
@@ -278,60 +313,34 @@ __global__ void __launch_bounds__(256) k_sparse_blocks(AsmArgs A, const int32_t*
 #pragma unroll
     for (int c = 0; c < 3; ++c) L[r][c] = A.lat[r + 3 * c];
   const uint16_t m = A.mask[i + j * N];
-  const double mult = (double)__popc((unsigned)m);
-  const double smm = sqrt(A.rmass[i] * A.rmass[j]);
-  const double rmult = 1.0 / mult, rsmm = 1.0 / smm;
-  auto scale = [&](double fp) { return div_rn(div_rn(fp, mult, rmult), smm, rsmm); };
   const double ci0 = A.crys[i], ci1 = A.crys[i + N], ci2 = A.crys[i + 2 * N];
   const double cj0 = A.crys[j], cj1 = A.crys[j + N], cj2 = A.crys[j + 2 * N];
   const double* q = A.q + 3 * iq;
   const bool deriv = nops > 1;
   PhaseOut po;
   pair_phase(q, ci0, ci1, ci2, cj0, cj1, cj2, m, L, deriv && MODE == 0, po);
-  double dr[3][2] = {}, er[3][2] = {};      // derivative sums (MODE 0) or q+ / q- sums (MODE 1)
-  if (deriv) {
-    for (int a = 0; a < 3; ++a) {
-      if (MODE == 1) {
-        double qp[3], qm[3];
-        for (int c = 0; c < 3; ++c) {
-          qp[c] = q[c] + A.cd_dc[3 * a + c] / 2.0;
-          qm[c] = q[c] - A.cd_dc[3 * a + c] / 2.0;
-        }
-        PhaseOut pp, pm;
-        pair_phase(qp, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pp);
-        pair_phase(qm, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pm);
-        dr[a][0] = pp.p0r; dr[a][1] = pp.p0i;
-        er[a][0] = pm.p0r; er[a][1] = pm.p0i;
-      } else {
-        dr[a][0] = po.pdr[a]; dr[a][1] = po.pdi[a];
+  cplx<double>* out = bph + (iq * nblk + k) * 4;
+  out[0] = cplx<double>{po.p0r, po.p0i};
+  if (!deriv) return;
+  for (int a = 0; a < 3; ++a) {
+    if (MODE == 1) {
+      double qp[3], qm[3];
+      for (int c = 0; c < 3; ++c) {
+        qp[c] = q[c] + A.cd_dc[3 * a + c] / 2.0;
+        qm[c] = q[c] - A.cd_dc[3 * a + c] / 2.0;
       }
+      PhaseOut pp, pm;
+      pair_phase(qp, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pp);
+      pair_phase(qm, ci0, ci1, ci2, cj0, cj1, cj2, m, L, false, pm);
+      out[1 + a] = cplx<double>{(pp.p0r - pm.p0r) / A.cd_norm, (pp.p0i - pm.p0i) / A.cd_norm};
+    } else {
+      out[1 + a] = cplx<double>{po.pdr[a], po.pdi[a]};
     }
   }
-  cplx<double>* out = bv + (iq * nblk + k) * 36;
-#pragma unroll
-  for (int al = 0; al < 3; ++al)
-#pragma unroll
-    for (int be = 0; be < 3; ++be) {
-      const double fv = A.phi[(3 * i + al) + (3 * j + be) * n];
-      out[3 * al + be] = cplx<double>{scale(fv * po.p0r), scale(fv * po.p0i)};
-      if (!deriv) continue;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double wr, wi;
-        if (MODE == 1) {
-          wr = (scale(fv * dr[d][0]) - scale(fv * er[d][0])) / A.cd_norm;
-          wi = (scale(fv * dr[d][1]) - scale(fv * er[d][1])) / A.cd_norm;
-        } else {
-          wr = scale(fv * dr[d][0]);
-          wi = scale(fv * dr[d][1]);
-        }
-        out[9 * (d + 1) + 3 * al + be] = cplx<double>{wr, wi};
-      }
-    }
 }
 
 void launch_sparse_blocks(const AsmParams& P, const int32_t* brow, const int32_t* bcol, int64_t nblk,
-                          int nops, cplx<double>* bv, hipStream_t s) {
+                          int nops, cplx<double>* bph, hipStream_t s) {
   if (nblk <= 0 || P.nb <= 0) return;
   AsmArgs A{};
   A.phi = P.phi; A.crys = P.crys; A.lat = P.lat; A.rmass = P.rmass; A.mask = P.mask;
@@ -339,9 +348,9 @@ void launch_sparse_blocks(const AsmParams& P, const int32_t* brow, const int32_t
   A.N = P.N; A.nb = P.nb; A.mode = P.mode;
   dim3 grid((unsigned)ceil_div(nblk, 256), (unsigned)P.nb);
   if (P.mode == 1)
-    hipLaunchKernelGGL(k_sparse_blocks<1>, grid, dim3(256), 0, s, A, brow, bcol, nblk, nops, bv);
+    hipLaunchKernelGGL(k_sparse_blocks<1>, grid, dim3(256), 0, s, A, brow, bcol, nblk, nops, bph);
   else
-    hipLaunchKernelGGL(k_sparse_blocks<0>, grid, dim3(256), 0, s, A, brow, bcol, nblk, nops, bv);
+    hipLaunchKernelGGL(k_sparse_blocks<0>, grid, dim3(256), 0, s, A, brow, bcol, nblk, nops, bph);
   PPH_CHECK_LAUNCH();
 }
 

@@ -29,21 +29,26 @@ void launch_assemble(const AsmParams& P, hipStream_t s);
 // flags[a + b N] = 1 when block (a, b) of phi (n x n column-major) is nonzero
 void launch_block_nz(const double* phi, uint8_t* flags, int64_t N, hipStream_t s);
 
-// Block values of D(q) (op 0) and, with nops = 4, dD/dq_{x,y,z} (ops 1..3) for
-// the listed nonzero 3x3 blocks (brow[k], bcol[k]), brow >= bcol, of every q of
-// P: bv[((q * nblk + k) * 4 + op) * 9 + 3 alpha + beta] (assemble.hip).
+// Nonzero 3x3 blocks (brow[k], bcol[k]), brow >= bcol, as a real block times a
+// phase sum (assemble.hip): phit[k * 9 + 3 alpha + beta] = Phi~ (per handle);
+// bph[(q * nblk + k) * 4 + op] = phase sum of D (op 0) and, with nops = 4, of
+// dD/dq_{x,y,z} (ops 1..3) for every q of P.
+void launch_block_phi(const double* phi, const uint16_t* mask, const double* rmass,
+                      const int32_t* brow, const int32_t* bcol, int64_t nblk, int64_t N,
+                      double* phit, hipStream_t s);
 void launch_sparse_blocks(const AsmParams& P, const int32_t* brow, const int32_t* bcol, int64_t nblk,
-                          int nops, cplx<double>* bv, hipStream_t s);
+                          int nops, cplx<double>* bph, hipStream_t s);
 
 // ---------------------------------------------------------------- block-sparse quadratic forms (sparse.hip)
 // out[b * out_b + o * out_op + i] += Re(z_i^H Op_{op0 + o} z_i), o < nops (1 or 3),
-// for modes i < M, from the blocks of rows a (rowptr[a] .. rowptr[a + 1]) in
-// bv; z_i from the mode-contiguous Zt (row r of matrix b at b * strideT + r * ldt).
+// for modes i < M, from the blocks of rows a (rowptr[a] .. rowptr[a + 1]):
+// Op = phase bph[...op] times phit; z_i from the mode-contiguous Zt (row r of
+// matrix b at b * strideT + r * ldt).
 template <typename Tz>
 void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk,
-                        const cplx<double>* bv, int op0, int nops, const cplx<Tz>* Zt, int64_t ldt,
-                        int64_t strideT, int64_t N, int64_t M, int64_t batch, double* out,
-                        int64_t out_op, int64_t out_b, hipStream_t s);
+                        const double* phit, const cplx<double>* bph, int op0, int nops,
+                        const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
+                        int64_t batch, double* out, int64_t out_op, int64_t out_b, hipStream_t s);
 // Zt(r, i) = Z(r, i), Z column-major n x m (ldz), Zt with rows ldt apart
 template <typename T>
 void launch_transpose(const cplx<T>* Z, int64_t n, int64_t m, int64_t ldz, int64_t strideZ,

@@ -6,10 +6,14 @@
 // summation order; the dense path (gemm.hip, Hermitian-dot epilogue) is used
 // when the blocks are not sparse.
 //
-// Like the dense Hermitian dot, only the lower triangle is read:
-//   z^H Op z = 2 Re(z^H Op' z), Op' = strict lower triangle + Re(diag) / 2,
-// here per block (a, b), a >= b: weight 2 for a > b; inside a diagonal block
-// weight 2 below its diagonal and Re(Op) |z|^2 on it.
+// A block is a phase sum p times a real block Phi~ (per atom pair), so for a
+// block (a, b) and mode z the forms of all operators share
+//   c = z_a^H (Phi~ z_b)          (18 + 12 real FMAs)
+// and each operator adds Re(p c).  Only blocks with a >= b are listed: an
+// off-diagonal block stands for itself and its Hermitian partner (weight 2),
+// a diagonal block (real symmetric Phi~, p real at q-independent zero
+// separation) is taken whole, which equals the dense Hermitian dot's lower
+// triangle plus Re(diagonal).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -19,11 +23,12 @@ constexpr int kDotAtoms = 16;      // atom rows per workgroup
 
 // grid: x = 256 modes, y = kDotAtoms atom rows, z = q-point.  Thread = mode i,
 // vectors from the mode-contiguous copy Zt (row r of matrix b at
-// b * strideT + r * ldt).  Block values bv as written by k_sparse_blocks.
+// b * strideT + r * ldt).
 template <typename Tz, int NOPS>
 __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__ rowptr,
                                                      const int32_t* __restrict__ bcol, int64_t nblk,
-                                                     const cplx<double>* __restrict__ bv, int op0,
+                                                     const double* __restrict__ phit,
+                                                     const cplx<double>* __restrict__ bph, int op0,
                                                      const cplx<Tz>* __restrict__ Zt, int64_t ldt,
                                                      int64_t strideT, int64_t N, int64_t M,
                                                      double* __restrict__ out, int64_t out_op,
@@ -32,6 +37,7 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
   const int64_t b = blockIdx.z;
   if (i >= M) return;
   const cplx<Tz>* Z = Zt + b * strideT + i;
+  const cplx<double>* P = bph + b * nblk * 4 + op0;
   double acc[NOPS];
 #pragma unroll
   for (int o = 0; o < NOPS; ++o) acc[o] = 0.0;
@@ -48,48 +54,32 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
     const int64_t k1 = rowptr[a + 1];
     for (int64_t k = rowptr[a]; k < k1; ++k) {
       const int64_t c = bcol[k];
-      const cplx<double>* V = bv + (b * nblk + k) * 36 + 9 * op0;
+      double wr[3], wi[3];
       if (c < a) {
-        double wr[3], wi[3];
 #pragma unroll
         for (int be = 0; be < 3; ++be) {
           const cplx<Tz> v = Z[(3 * c + be) * ldt];
           wr[be] = (double)v.x;
           wi[be] = (double)v.y;
         }
+      } else {
 #pragma unroll
-        for (int o = 0; o < NOPS; ++o) {
-          double s = 0.0;
+        for (int be = 0; be < 3; ++be) { wr[be] = zr[be]; wi[be] = zi[be]; }
+      }
+      const double* F = phit + k * 9;
+      double cr = 0.0, ci = 0.0;           // z_a^H (Phi~ w)
 #pragma unroll
-          for (int al = 0; al < 3; ++al) {
-            double tr = 0.0, ti = 0.0;       // (Op w)_alpha
+      for (int al = 0; al < 3; ++al) {
+        const double ur = F[3 * al] * wr[0] + F[3 * al + 1] * wr[1] + F[3 * al + 2] * wr[2];
+        const double ui = F[3 * al] * wi[0] + F[3 * al + 1] * wi[1] + F[3 * al + 2] * wi[2];
+        cr += zr[al] * ur + zi[al] * ui;
+        ci += zr[al] * ui - zi[al] * ur;
+      }
+      const double w = (c < a) ? 2.0 : 1.0;
 #pragma unroll
-            for (int be = 0; be < 3; ++be) {
-              const cplx<double> e = V[9 * o + 3 * al + be];
-              tr += e.x * wr[be] - e.y * wi[be];
-              ti += e.x * wi[be] + e.y * wr[be];
-            }
-            s += zr[al] * tr + zi[al] * ti;   // Re(conj(z_alpha) (Op w)_alpha)
-          }
-          acc[o] += 2.0 * s;
-        }
-      } else {                               // diagonal block (c == a)
-#pragma unroll
-        for (int o = 0; o < NOPS; ++o) {
-          double s = 0.0;
-#pragma unroll
-          for (int al = 0; al < 3; ++al) {
-            s += V[9 * o + 4 * al].x * (zr[al] * zr[al] + zi[al] * zi[al]);
-#pragma unroll
-            for (int be = 0; be < al; ++be) {
-              const cplx<double> e = V[9 * o + 3 * al + be];
-              const double tr = e.x * zr[be] - e.y * zi[be];
-              const double ti = e.x * zi[be] + e.y * zr[be];
-              s += 2.0 * (zr[al] * tr + zi[al] * ti);
-            }
-          }
-          acc[o] += s;
-        }
+      for (int o = 0; o < NOPS; ++o) {
+        const cplx<double> p = P[k * 4 + o];
+        acc[o] += w * (p.x * cr - p.y * ci);
       }
     }
   }
@@ -99,29 +89,31 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
 
 template <typename Tz>
 void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk,
-                        const cplx<double>* bv, int op0, int nops, const cplx<Tz>* Zt, int64_t ldt,
-                        int64_t strideT, int64_t N, int64_t M, int64_t batch, double* out,
-                        int64_t out_op, int64_t out_b, hipStream_t s) {
+                        const double* phit, const cplx<double>* bph, int op0, int nops,
+                        const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
+                        int64_t batch, double* out, int64_t out_op, int64_t out_b, hipStream_t s) {
   if (M <= 0 || batch <= 0 || N <= 0) return;
+  if (op0 < 0 || op0 + nops > 4) throw std::runtime_error("launch_sparse_dots: operators out of range");
   dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, kDotAtoms), (unsigned)batch);
   if (nops == 1)
-    hipLaunchKernelGGL((k_sparse_dots<Tz, 1>), grid, dim3(256), 0, s, rowptr, bcol, nblk, bv, op0,
-                       Zt, ldt, strideT, N, M, out, out_op, out_b);
+    hipLaunchKernelGGL((k_sparse_dots<Tz, 1>), grid, dim3(256), 0, s, rowptr, bcol, nblk, phit, bph,
+                       op0, Zt, ldt, strideT, N, M, out, out_op, out_b);
   else if (nops == 3)
-    hipLaunchKernelGGL((k_sparse_dots<Tz, 3>), grid, dim3(256), 0, s, rowptr, bcol, nblk, bv, op0,
-                       Zt, ldt, strideT, N, M, out, out_op, out_b);
+    hipLaunchKernelGGL((k_sparse_dots<Tz, 3>), grid, dim3(256), 0, s, rowptr, bcol, nblk, phit, bph,
+                       op0, Zt, ldt, strideT, N, M, out, out_op, out_b);
   else
     throw std::runtime_error("launch_sparse_dots: nops must be 1 or 3");
   PPH_CHECK_LAUNCH();
 }
 
-template void launch_sparse_dots<float>(const int64_t*, const int32_t*, int64_t, const cplx<double>*,
-                                        int, int, const cplx<float>*, int64_t, int64_t, int64_t,
-                                        int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
-template void launch_sparse_dots<double>(const int64_t*, const int32_t*, int64_t,
-                                         const cplx<double>*, int, int, const cplx<double>*, int64_t,
-                                         int64_t, int64_t, int64_t, int64_t, double*, int64_t,
-                                         int64_t, hipStream_t);
+template void launch_sparse_dots<float>(const int64_t*, const int32_t*, int64_t, const double*,
+                                        const cplx<double>*, int, int, const cplx<float>*, int64_t,
+                                        int64_t, int64_t, int64_t, int64_t, double*, int64_t,
+                                        int64_t, hipStream_t);
+template void launch_sparse_dots<double>(const int64_t*, const int32_t*, int64_t, const double*,
+                                         const cplx<double>*, int, int, const cplx<double>*,
+                                         int64_t, int64_t, int64_t, int64_t, int64_t, double*,
+                                         int64_t, int64_t, hipStream_t);
 
 // Zt(r, i) = Z(r, i) for rows r < n and modes i < m: Z column-major (ldz),
 // Zt mode-contiguous (row r at r * ldt).  32 x 32 tiles through LDS.
