@@ -485,12 +485,14 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
       if (kb[b] > 0 && ilo[b] == 0 && Mv[b] < n)
         for (int64_t m = 0; m < kb[b]; ++m) newls[b][m] += corr[b * Kp + m];
   }
+  // the refined values replace the first kb[b] of each matrix in the host copy,
+  // which then goes back in one transfer
   for (int64_t b = 0; b < B; ++b) {
     if (kb[b] == 0) continue;
     std::sort(newls[b].begin(), newls[b].end());
-    PPH_HIP(hipMemcpyAsync(lam_dev + b * mcap, newls[b].data(), kb[b] * sizeof(double),
-                           hipMemcpyHostToDevice, s));
+    std::copy(newls[b].begin(), newls[b].end(), lam.begin() + b * mcap);
   }
+  PPH_HIP(hipMemcpyAsync(lam_dev, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, s));
   PPH_HIP(hipStreamSynchronize(s));
 }
 
@@ -641,6 +643,9 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   te.strideT = n;
   te.range = o.range; te.vl = vl_l; te.vu = vu_l; te.il = o.il; te.iu = o.iu;
   te.abstol = o.abstol; te.orfac = o.orfac;
+  // the FP32 path's eigenvalues come from the FP64 refinement; bisection on
+  // its FP32 tridiagonal only supplies the inverse-iteration shifts
+  te.rtol = fp32 ? 1e-13 : 0.0;
   te.ilo = L.tilo.as<int64_t>(); te.ihi = L.tihi.as<int64_t>();
   te.tinfo = L.tinfo.as<double>();
   te.w = L.tw.as<double>(); te.ldw = mcap;

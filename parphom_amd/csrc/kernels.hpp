@@ -126,6 +126,7 @@ struct TriEigWork {
   double* dd; double* ee; double* e2;     // FP64 copies (n each per matrix)
   int64_t strideT;
   char range; double vl, vu; int64_t il, iu; double abstol; double orfac;
+  double rtol;                             // bisection stops at rtol * |T| too (0: abstol only)
   int64_t* ilo; int64_t* ihi;              // per matrix index range found [ilo, ihi)
   double* tinfo;                           // per matrix: gl, gu, tnorm, pivmin
   double* w; int64_t ldw;                  // eigenvalues: w[b*ldw + k]

@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256) k_tri_bisect(TriEigWork w, int use_lds) {
   const double* ti = w.tinfo + b * 4;
   const double pivmin = ti[3];
   double lo = ti[0], hi = ti[1];
-  const double atol = fmax(w.abstol, pivmin);
+  const double atol = fmax(fmax(w.abstol, pivmin), w.rtol * ti[2]);
   for (int it = 0; it < 200; ++it) {
     const double tol = fmax(atol, 2.0 * ULP * fmax(fabs(lo), fabs(hi)));
     if (hi - lo <= tol) break;
