@@ -1,0 +1,124 @@
+"""Autotuner for the blocking and batching of the phonon pipeline.
+
+The library's tunables on a given structure and device are the reflectors per
+reduction panel (``nb_panel``: 16, 32, 48, 64), the reflectors per
+back-transformation block (``nb_back``: multiples of 16 up to 128), the
+q-points per device batch (``batch``) and the batches in flight (``streams``,
+one stream and host thread each).  ``tune`` times every combination of the
+given candidates on the caller's own structure and q-points with the HIP path
+and returns the fastest, with the whole table.  The measured defaults of the
+library (32, 128, 32, 2) are the best found this way for twisted bilayer
+graphene at n = 4764; other sizes can prefer other values (smaller cells want
+larger batches).
+
+    from parphom_amd import tune, synth
+    st, fc, q = synth.workload("tbg_8_9")
+    best = tune.tune(st.lat, st.pos, st.types, st.masses, fc, q)
+    # best["config"] -> {"batch": ..., "streams": ..., "nb_panel": ..., "nb_back": ...}
+
+or ``python -m parphom_amd.tune --workload tbg_8_9``.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import time
+
+import numpy as np
+
+
+def _default_handle(lat, pos, types, masses, fc, **opts):
+    from parphom_amd import lib
+    return lib.Handle(lat, pos, types, masses, fc, **opts)
+
+
+def _device_outputs(nq, n):
+    import torch
+    dev = torch.device("cuda:0")
+    return {"eval_cm": torch.zeros((nq, n), dtype=torch.float64, device=dev),
+            "m": torch.zeros(nq, dtype=torch.int64, device=dev),
+            "nz": torch.zeros(nq, dtype=torch.int64, device=dev),
+            "info": torch.zeros(nq, dtype=torch.int32, device=dev),
+            "evec": torch.zeros((nq, n, n), dtype=torch.complex128, device=dev),
+            "vel": torch.zeros((nq, n, 3), dtype=torch.float64, device=dev)}
+
+
+def tune(lat, pos, types, masses, fc, q, precision="S", jobz="V", vel=True,
+         batches=(16, 32), streams=(1, 2), panels=(32, 48), backs=(64, 128), reps=1,
+         device_outputs=False, handle_factory=None, clock=time.perf_counter, log=None):
+    """Time every (batch, streams, nb_panel, nb_back) candidate and return
+    {"config": best, "qps": best rate, "table": [rows]}.  Each candidate runs
+    one untimed warm-up call and then `reps` timed calls of pph_run over
+    batch * streams q-points (taken cyclically from q): with host outputs, or
+    with outputs left in device buffers (pph_run_device, torch tensors) when
+    device_outputs is set.  Candidates the library rejects are recorded with
+    qps None."""
+    make = handle_factory or _default_handle
+    q = np.atleast_2d(np.asarray(q, np.float64))
+    table = []
+    best = None
+    for B, S in itertools.product(batches, streams):
+        per_call = B * S
+        idx = np.arange(per_call) % len(q)
+        qs = q[idx]
+        out = _device_outputs(per_call, 3 * len(pos)) if device_outputs else None
+        call = (lambda h: h.run_device(qs, out)) if device_outputs else (lambda h: h.run(qs))
+        with make(lat, pos, types, masses, fc, jobz=jobz, vel=vel, precision=precision,
+                  batch=B, streams=S) as h:
+            for nbp, nbb in itertools.product(panels, backs):
+                row = {"batch": B, "streams": S, "nb_panel": nbp, "nb_back": nbb, "qps": None}
+                try:
+                    h.blocking(nbp, nbb)
+                except Exception as e:          # unsupported blocking: keep the row, skip it
+                    row["error"] = str(e)
+                    table.append(row)
+                    continue
+                call(h)                        # warm-up (allocations, first-use set-up)
+                if device_outputs:
+                    import torch
+                    torch.cuda.synchronize()
+                t0 = clock()
+                for _ in range(reps):
+                    call(h)
+                if device_outputs:
+                    torch.cuda.synchronize()
+                dt = clock() - t0
+                row["qps"] = reps * per_call / dt if dt > 0 else float("inf")
+                table.append(row)
+                if log:
+                    log(f"[tune] batch {B} streams {S} panel {nbp} back {nbb}: "
+                        f"{row['qps']:.3f} q-pts/s")
+                if best is None or row["qps"] > best["qps"]:
+                    best = row
+    if best is None:
+        raise RuntimeError("tune: no candidate ran")
+    cfg = {k: best[k] for k in ("batch", "streams", "nb_panel", "nb_back")}
+    return {"config": cfg, "qps": best["qps"], "table": table}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", default="tbg_8_9")
+    ap.add_argument("--precision", default="S")
+    ap.add_argument("--batches", default="16,32")
+    ap.add_argument("--streams", default="1,2")
+    ap.add_argument("--panels", default="32,48")
+    ap.add_argument("--backs", default="64,128")
+    ap.add_argument("--reps", type=int, default=1)
+    ap.add_argument("--device-outputs", action="store_true",
+                    help="leave outputs in device buffers (pph_run_device) instead of host arrays")
+    a = ap.parse_args(argv)
+    from parphom_amd import synth
+    st, fc, q = synth.workload(a.workload)
+    ints = lambda s: tuple(int(x) for x in s.split(","))   # noqa: E731
+    res = tune(st.lat, st.pos, st.types, st.masses, fc, q, precision=a.precision,
+               batches=ints(a.batches), streams=ints(a.streams), panels=ints(a.panels),
+               backs=ints(a.backs), reps=a.reps, device_outputs=a.device_outputs,
+               log=lambda m: print(m, flush=True))
+    res["workload"] = a.workload
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -192,3 +192,46 @@ def test_dos_grid_and_triangles(tmp_path):
     tri = dos.triangles(lat, (m, m, 1), grid, mapping)
     assert tri.shape == (2 * (m - 1) ** 2, 3)
     assert tri.min() >= 0 and tri.max() < 5
+
+
+def test_tuner_picks_fastest_candidate_and_records_rejections():
+    """The autotuner's selection on a stub handle whose run time depends on
+    the configuration (no GPU): the fastest candidate wins, rejected
+    blockings stay in the table without a rate, every candidate is warmed up
+    once before its timed calls."""
+    from parphom_amd import tune
+
+    cost = {(16, 1, 32, 64): 4.0, (16, 1, 32, 128): 3.0, (32, 1, 32, 64): 2.5,
+            (32, 1, 32, 128): 2.0, (16, 2, 32, 64): 5.0, (16, 2, 32, 128): 1.0,
+            (32, 2, 32, 64): 6.0, (32, 2, 32, 128): 4.5}
+    now = [0.0]
+    calls = []
+
+    class StubHandle:
+        def __init__(self, *a, batch, streams, **k):
+            self.cfg = [batch, streams, None, None]
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            return False
+
+        def blocking(self, nbp, nbb):
+            if nbp == 40:
+                raise ValueError("unsupported panel")
+            self.cfg[2:] = [nbp, nbb]
+
+        def run(self, q):
+            calls.append(tuple(self.cfg))
+            now[0] += cost[tuple(self.cfg)] * len(q) / 100.0
+
+    res = tune.tune(None, None, None, None, None, np.zeros((5, 3)), batches=(16, 32),
+                    streams=(1, 2), panels=(32, 40), backs=(64, 128), reps=2,
+                    handle_factory=StubHandle, clock=lambda: now[0])
+    assert res["config"] == {"batch": 16, "streams": 2, "nb_panel": 32, "nb_back": 128}
+    assert abs(res["qps"] - 100.0) < 1e-9
+    rejected = [r for r in res["table"] if r["nb_panel"] == 40]
+    assert len(rejected) == 8 and all(r["qps"] is None and "error" in r for r in rejected)
+    # one warm-up and two timed calls per accepted candidate
+    assert len(calls) == 3 * 8
