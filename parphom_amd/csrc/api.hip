@@ -202,6 +202,7 @@ struct Handle : Lane {
   // and the velocities use them instead of dense products when at most a
   // quarter of all blocks are nonzero (a cutoff model in a large cell).
   DevBuf sp_rowptr, sp_row, sp_col, sp_phit;    // + the real block Phi~ of each
+  DevBuf sp_tptr, sp_trow, sp_tblk;             // per atom a: the blocks (c, a), c > a
   int64_t sp_nblk = 0;
   bool sp_ok = false;
   int sp_mode = 1;         // 0 dense products, 1 automatic, 2 block lists whenever built
@@ -374,7 +375,12 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
   cplx<double>* C2 = C1 + (size_t)B * mcap * Kp;
   int64_t mmax = 0;
   for (auto m : Mv) mmax = std::max(mmax, m);
-  {
+  if (h.use_sparse()) {                      // W_T = D X(:, 0:K) from the listed blocks
+    launch_sparse_spmm(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_tptr.as<int64_t>(),
+                       h.sp_trow.as<int32_t>(), h.sp_tblk.as<int64_t>(), h.sp_nblk,
+                       h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), X, n * mcap, WT, n * Kp,
+                       h.N, K, B, s);
+  } else {
     GemmParams<double> g{};                 // W_T = D X(:, 0:K)
     g.opA = 'N'; g.opB = 'N'; g.m = n; g.n = K; g.k = n;
     g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
@@ -570,7 +576,13 @@ static void complement_correction(Handle& h, Lane& L, int64_t B, const cplx<doub
     double* rho_old = rho + (it % 2) * B * Kp;
     double* rho_new = rho + ((it + 1) % 2) * B * Kp;
     // q = P (D - theta) p
-    gemm('N', n, n, D64, h.ldA, h.ldA * n, P, n, sv, Q, n, sv, 1.0, 0.0);
+    if (h.use_sparse())
+      launch_sparse_spmm(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_tptr.as<int64_t>(),
+                         h.sp_trow.as<int32_t>(), h.sp_tblk.as<int64_t>(), h.sp_nblk,
+                         h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), P, sv, Q, sv, h.N, Kp,
+                         B, s);
+    else
+      gemm('N', n, n, D64, h.ldA, h.ldA * n, P, n, sv, Q, n, sv, 1.0, 0.0);
     launch_axpy_cols(Q, P, thd, -1.0, n, Kp, n, sv, Kp, B, s);
     project(Q);
     launch_coldot_re(P, Q, n, Kp, n, sv, B, pq, Kp, s);
@@ -767,7 +779,10 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
 
   L.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
-  L.A64.ensure((size_t)B * h.ldA * n * sizeof(cplx<double>));
+  // D in FP64: the FP64 path's working matrix; on the FP32 path the
+  // refinement's dense products read it, which the block-sparse forms replace
+  const bool d64 = !fp32 || !sparse;
+  if (d64) L.A64.ensure((size_t)B * h.ldA * n * sizeof(cplx<double>));
   if (fp32) L.A32.ensure((size_t)B * h.ldA * n * sizeof(cplx<float>));
   if (vel && !sparse)
     L.dD.ensure((size_t)B * 3 * n * n * (fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>)));
@@ -777,7 +792,7 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   ap.rmass = h.rmass.as<double>(); ap.mask = h.mask.as<uint16_t>(); ap.q = L.qdev.as<double>();
   ap.cd_dc = h.cd_dc.as<double>(); ap.cd_norm = h.cd_norm;
   ap.N = N; ap.nb = B; ap.mode = (o.vel_method == 'C') ? 1 : 0;
-  ap.D64 = L.A64.as<cplx<double>>();
+  ap.D64 = d64 ? L.A64.as<cplx<double>>() : nullptr;
   ap.D32 = fp32 ? L.A32.as<cplx<float>>() : nullptr;
   ap.ldD = h.ldA;
   if (vel && !sparse) {
@@ -799,7 +814,9 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   const double K32 = (double)(float)(15.633302f * 33.35641f);
   const double vl_l = o.vl / K32, vu_l = o.vu / K32;
   EigResult& R = out.R;
-  if (fp32) eig_batch<float>(h, L, B, L.A32.as<cplx<float>>(), L.A64.as<cplx<double>>(), want_vec, vl_l, vu_l, R);
+  if (fp32)
+    eig_batch<float>(h, L, B, L.A32.as<cplx<float>>(), d64 ? L.A64.as<cplx<double>>() : nullptr,
+                     want_vec, vl_l, vu_l, R);
   else eig_batch<double>(h, L, B, L.A64.as<cplx<double>>(), nullptr, want_vec, vl_l, vu_l, R);
 
   const int64_t mcap = R.mcap;
@@ -874,7 +891,7 @@ static int64_t auto_batch(const Handle& h, int64_t nq, int64_t lanes) {
   const int64_t n = h.n;
   const bool s = h.opts.precision == 'S';
   const bool sp = h.use_sparse();
-  const double per = (double)n * n * (16.0 + (s ? 8.0 : 0.0) +
+  const double per = (double)n * n * ((s && sp ? 0.0 : 16.0) + (s ? 8.0 : 0.0) +
                                       (h.opts.vel && !sp ? (s ? 48.0 : 96.0) : 0.0) +
                                       (sp ? (s ? 8.0 : 16.0) : 0.0) + 96.0);
   size_t free_b = 0, total_b = 0;
@@ -960,8 +977,34 @@ static void build_block_lists(Handle& h) {
       }
     rowptr[a + 1] = (int64_t)row.size();
   }
+  // transposed lists: for atom b, the blocks (a, b) with a > b
+  std::vector<int64_t> tptr(N + 1, 0), tblk;
+  std::vector<int32_t> trow;
+  for (size_t k = 0; k < row.size(); ++k)
+    if (col[k] < row[k]) ++tptr[col[k] + 1];
+  for (int64_t a = 0; a < N; ++a) tptr[a + 1] += tptr[a];
+  trow.resize(tptr[N]);
+  tblk.resize(tptr[N]);
+  {
+    std::vector<int64_t> fill(tptr.begin(), tptr.end() - 1);
+    for (size_t k = 0; k < row.size(); ++k)
+      if (col[k] < row[k]) {
+        const int64_t t = fill[col[k]]++;
+        trow[t] = row[k];
+        tblk[t] = (int64_t)k;
+      }
+  }
   h.sp_ok = full <= cap;
   h.sp_nblk = (int64_t)row.size();
+  const int64_t nt = tptr[N];
+  h.sp_tptr.ensure((N + 1) * sizeof(int64_t));
+  h.sp_trow.ensure(std::max<int64_t>(nt, 1) * sizeof(int32_t));
+  h.sp_tblk.ensure(std::max<int64_t>(nt, 1) * sizeof(int64_t));
+  PPH_HIP(hipMemcpyAsync(h.sp_tptr.p, tptr.data(), (N + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  if (nt > 0) {
+    PPH_HIP(hipMemcpyAsync(h.sp_trow.p, trow.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PPH_HIP(hipMemcpyAsync(h.sp_tblk.p, tblk.data(), nt * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  }
   h.sp_rowptr.ensure((N + 1) * sizeof(int64_t));
   h.sp_row.ensure(h.sp_nblk * sizeof(int32_t));
   h.sp_col.ensure(h.sp_nblk * sizeof(int32_t));

@@ -49,6 +49,14 @@ void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk
                         const double* phit, const cplx<double>* bph, int op0, int nops,
                         const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
                         int64_t batch, double* out, int64_t out_op, int64_t out_b, hipStream_t s);
+// Y = D X for k columns (n x k per matrix, ld n) from the listed blocks: row
+// lists (rowptr, bcol) and their transposes (tptr, trow, tblk: for atom a the
+// blocks (c, a) with c > a), phases bph (op 0) and real blocks phit
+void launch_sparse_spmm(const int64_t* rowptr, const int32_t* bcol, const int64_t* tptr,
+                        const int32_t* trow, const int64_t* tblk, int64_t nblk, const double* phit,
+                        const cplx<double>* bph, const cplx<double>* X, int64_t strideX,
+                        cplx<double>* Y, int64_t strideY, int64_t N, int64_t k, int64_t batch,
+                        hipStream_t s);
 // Zt(r, i) = Z(r, i), Z column-major n x m (ldz), Zt with rows ldt apart
 template <typename T>
 void launch_transpose(const cplx<T>* Z, int64_t n, int64_t m, int64_t ldz, int64_t strideZ,

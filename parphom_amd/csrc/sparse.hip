@@ -115,6 +115,71 @@ template void launch_sparse_dots<double>(const int64_t*, const int32_t*, int64_t
                                          int64_t, int64_t, int64_t, int64_t, int64_t, double*,
                                          int64_t, int64_t, hipStream_t);
 
+// Y = D X for k columns of X (n x k per matrix, column-major, ld n): thread =
+// (atom row a, column c).  Row a of D is the listed blocks (a, b), b <= a,
+// and the Hermitian partners of the blocks (c', a), c' > a, found through the
+// transposed lists (tptr, trow, tblk): D_{a c'} = conj(p) Phi~^T.
+__global__ void __launch_bounds__(256) k_sparse_spmm(const int64_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ bcol,
+                                                     const int64_t* __restrict__ tptr,
+                                                     const int32_t* __restrict__ trow,
+                                                     const int64_t* __restrict__ tblk, int64_t nblk,
+                                                     const double* __restrict__ phit,
+                                                     const cplx<double>* __restrict__ bph,
+                                                     const cplx<double>* __restrict__ X,
+                                                     int64_t strideX, cplx<double>* __restrict__ Y,
+                                                     int64_t strideY, int64_t N) {
+  const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y, b = blockIdx.z;
+  if (a >= N) return;
+  const int64_t n = 3 * N;
+  const cplx<double>* x = X + b * strideX + c * n;
+  const cplx<double>* P = bph + b * nblk * 4;
+  double yr[3] = {0, 0, 0}, yi[3] = {0, 0, 0};
+  // accumulate p (F x_j) (transposed F when tr) into y
+  auto add = [&](const double* F, cplx<double> p, int64_t j, bool tr) {
+    double ur[3], ui[3];
+#pragma unroll
+    for (int al = 0; al < 3; ++al) {
+      ur[al] = 0.0;
+      ui[al] = 0.0;
+#pragma unroll
+      for (int be = 0; be < 3; ++be) {
+        const double f = tr ? F[3 * be + al] : F[3 * al + be];
+        const cplx<double> v = x[3 * j + be];
+        ur[al] += f * v.x;
+        ui[al] += f * v.y;
+      }
+    }
+#pragma unroll
+    for (int al = 0; al < 3; ++al) {
+      yr[al] += p.x * ur[al] - p.y * ui[al];
+      yi[al] += p.x * ui[al] + p.y * ur[al];
+    }
+  };
+  for (int64_t k = rowptr[a]; k < rowptr[a + 1]; ++k) add(phit + k * 9, P[k * 4], bcol[k], false);
+  for (int64_t t = tptr[a]; t < tptr[a + 1]; ++t) {
+    const int64_t k = tblk[t];
+    const cplx<double> p = P[k * 4];
+    add(phit + k * 9, cplx<double>{p.x, -p.y}, trow[t], true);
+  }
+  cplx<double>* y = Y + b * strideY + c * n + 3 * a;
+#pragma unroll
+  for (int al = 0; al < 3; ++al) y[al] = cplx<double>{yr[al], yi[al]};
+}
+
+void launch_sparse_spmm(const int64_t* rowptr, const int32_t* bcol, const int64_t* tptr,
+                        const int32_t* trow, const int64_t* tblk, int64_t nblk, const double* phit,
+                        const cplx<double>* bph, const cplx<double>* X, int64_t strideX,
+                        cplx<double>* Y, int64_t strideY, int64_t N, int64_t k, int64_t batch,
+                        hipStream_t s) {
+  if (N <= 0 || k <= 0 || batch <= 0) return;
+  dim3 grid((unsigned)ceil_div(N, 256), (unsigned)k, (unsigned)batch);
+  hipLaunchKernelGGL(k_sparse_spmm, grid, dim3(256), 0, s, rowptr, bcol, tptr, trow, tblk, nblk,
+                     phit, bph, X, strideX, Y, strideY, N);
+  PPH_CHECK_LAUNCH();
+}
+
 // Zt(r, i) = Z(r, i) for rows r < n and modes i < m: Z column-major (ldz),
 // Zt mode-contiguous (row r at r * ldt).  32 x 32 tiles through LDS.
 template <typename T>
