@@ -451,11 +451,11 @@ def main():
     res["phase_ms_one_batch_alone"] = {k: round(v, 3) for k, v in iso.items()}
 
     # roofline of the assembly phase (HBM-bound): algorithmic bytes = Phi
-    # (8 n^2) + B * (D in c128 and c64) and either dD in c64 (dense products)
-    # or the four phase sums of D and dD on the nblk nonzero blocks (4 c128
-    # each, k_sparse_blocks)
-    asm_bytes = 8 * n * n + B * (16 + 8) * n * n + (B * 4 * 16.0 * nblk if nblk > 0
-                                                    else B * 3 * 8 * n * n)
+    # (8 n^2) + B * D in c64 for the reduction, and either D in c128 and dD in
+    # c64 (dense products) or the four phase sums of D and dD on the nblk
+    # nonzero blocks (4 c128 each, k_sparse_blocks)
+    asm_bytes = 8 * n * n + B * 8 * n * n + (B * 4 * 16.0 * nblk if nblk > 0
+                                             else B * (16 + 3 * 8) * n * n)
     # the assembly is one launch per batch; in the timed steps it shares the
     # GPU with the other lane's reduction, so it is rated on the batch run alone
     asm_s = iso["assemble"] / 1e3

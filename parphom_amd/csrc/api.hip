@@ -203,6 +203,7 @@ struct Handle : Lane {
   // quarter of all blocks are nonzero (a cutoff model in a large cell).
   DevBuf sp_rowptr, sp_row, sp_col, sp_phit;    // + the real block Phi~ of each
   DevBuf sp_tptr, sp_trow, sp_tblk;             // per atom a: the blocks (c, a), c > a
+  DevBuf sp_flags;                              // N x N bytes: block (a, b) of Phi_sym nonzero
   int64_t sp_nblk = 0;
   bool sp_ok = false;
   int sp_mode = 1;         // 0 dense products, 1 automatic, 2 block lists whenever built
@@ -795,6 +796,7 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   ap.D64 = d64 ? L.A64.as<cplx<double>>() : nullptr;
   ap.D32 = fp32 ? L.A32.as<cplx<float>>() : nullptr;
   ap.ldD = h.ldA;
+  ap.nzflag = h.sp_flags.as<uint8_t>();
   if (vel && !sparse) {
     if (fp32) ap.dD32 = L.dD.as<cplx<float>>();
     else ap.dD64 = L.dD.as<cplx<double>>();
@@ -955,13 +957,12 @@ static void build_block_lists(Handle& h) {
   const int64_t N = h.N;
   hipStream_t s = h.stream;
   std::vector<uint8_t> flags((size_t)N * N);
-  {
-    DevBuf fl;
-    fl.ensure((size_t)N * N);
-    launch_block_nz(h.phi.as<double>(), fl.as<uint8_t>(), N, s);
-    PPH_HIP(hipMemcpyAsync(flags.data(), fl.p, (size_t)N * N, hipMemcpyDeviceToHost, s));
-    PPH_HIP(hipStreamSynchronize(s));
-  }
+  // the device flags stay with the handle: the assembly skips the phase sums
+  // of zero blocks
+  h.sp_flags.ensure((size_t)N * N);
+  launch_block_nz(h.phi.as<double>(), h.sp_flags.as<uint8_t>(), N, s);
+  PPH_HIP(hipMemcpyAsync(flags.data(), h.sp_flags.p, (size_t)N * N, hipMemcpyDeviceToHost, s));
+  PPH_HIP(hipStreamSynchronize(s));
   int64_t full = 0;                 // nonzero blocks of the whole matrix
   for (size_t i = 0; i < flags.size(); ++i) full += flags[i];
   const int64_t cap = N * N / 4;
@@ -1241,6 +1242,7 @@ int pph_assemble(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32
     h->A64.ensure((size_t)B * ldA * n * sizeof(cplx<double>));
     ap.D64 = h->A64.as<cplx<double>>();
     ap.ldD = ldA;
+    ap.nzflag = h->sp_flags.as<uint8_t>();
     if (what != 0) {
       h->dD.ensure((size_t)B * 3 * n * n * sizeof(cplx<double>));
       ap.dD64 = h->dD.as<cplx<double>>();
@@ -1290,6 +1292,7 @@ int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t
     h->A64.ensure((size_t)nb * h->ldA * n * sizeof(cplx<double>));
     ap.D64 = h->A64.as<cplx<double>>();
     ap.ldD = h->ldA;
+    ap.nzflag = h->sp_flags.as<uint8_t>();
     if (what > 0) {
       h->dD.ensure((size_t)nb * 3 * n * n * sizeof(cplx<double>));
       ap.dD64 = h->dD.as<cplx<double>>();

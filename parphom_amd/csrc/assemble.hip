@@ -134,6 +134,7 @@ struct AsmArgs {
   double cd_norm;           // |delta|
   int64_t N, nb, mode;
   int64_t ldD;              // leading dimension of D64 / D32 (>= n; dD uses n)
+  const uint8_t* nzflag;    // N x N: 0 where the block of Phi_sym is zero (may be null)
   cplx<double>* D64;
   cplx<float>* D32;
   cplx<double>* dD64;
@@ -160,10 +161,12 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) L[r][c] = A.lat[r + 3 * c];
   uint16_t m = 0;
+  bool nz = true;           // zero blocks give zero elements whatever the phase
   double ci0 = 0, ci1 = 0, ci2 = 0;
   const double cj0 = A.crys[j], cj1 = A.crys[j + N], cj2 = A.crys[j + 2 * N];
   if (i < N) {
     m = A.mask[i + j * N];
+    if (A.nzflag) nz = A.nzflag[i + j * N] != 0;
     smult[tid] = (double)__popc((unsigned)m);
     ssmm[tid] = sqrt(A.rmass[i] * A.rmass[j]);
     srmult[tid] = 1.0 / smult[tid];
@@ -184,7 +187,13 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
   const size_t mstride = (size_t)n * n;
   for (int64_t iq = 0; iq < A.nb; ++iq) {
     const double* q = A.q + 3 * iq;
-    if (i < N) {
+    if (i < N && !nz) {
+      sp0r[tid] = sp0i[tid] = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        sdr[a][tid] = sdi[a][tid] = 0.0;
+        ser[MODE == 1 ? a : 0][tid] = sei[MODE == 1 ? a : 0][tid] = 0.0;
+      }
+    } else if (i < N) {
       PhaseOut po;
       pair_phase(q, ci0, ci1, ci2, cj0, cj1, cj2, m, L, want_d && MODE == 0, po);
       sp0r[tid] = po.p0r;
@@ -395,6 +404,7 @@ void launch_assemble(const AsmParams& P, hipStream_t s) {
   A.q = P.q; A.cd_dc = P.cd_dc; A.cd_norm = P.cd_norm;
   A.N = P.N; A.nb = P.nb; A.mode = P.mode;
   A.ldD = P.ldD > 0 ? P.ldD : 3 * P.N;
+  A.nzflag = P.nzflag;
   A.D64 = P.D64; A.D32 = P.D32; A.dD64 = P.dD64; A.dD32 = P.dD32;
   dim3 grid((unsigned)ceil_div(P.N, 256), (unsigned)P.N);
   if (A.mode == 1) hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, s, A);

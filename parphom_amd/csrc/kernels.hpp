@@ -16,6 +16,7 @@ struct AsmParams {
   double cd_norm;
   int64_t N, nb, mode;      // mode 0 analytic, 1 central difference
   int64_t ldD;              // leading dimension of D64 / D32 (0 = n); dD64 / dD32 use n
+  const uint8_t* nzflag;    // N x N, 0 where the block of Phi_sym is zero (null: none known)
   cplx<double>* D64;
   cplx<float>* D32;
   cplx<double>* dD64;
