@@ -56,7 +56,8 @@ def test_shipped_fc_properties():
     fc = z["fc"]
     N = fc.shape[0]
     assert fc.shape == (28, 28, 3, 3) and fc.dtype == np.float64
-    # the survey's probe: ASR to <= 1e-6, Phi = Phi^T to <= 2.4e-7
+    # the shipped dataset satisfies the acoustic sum rule to <= 1e-6 and
+    # Phi = Phi^T to <= 2.4e-7 (checked with margin)
     assert np.abs(fc.sum(axis=1)).max() < 1e-5
     P = fc.transpose(0, 2, 1, 3).reshape(3 * N, 3 * N)
     assert np.abs(P - P.T).max() < 1e-6
