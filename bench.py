@@ -307,7 +307,7 @@ def large_bench(lib, torch, a, nvec=2000):
     eigenvalues-only on Gamma, K and M in both precisions.  Every run is
     pph_run with host output buffers, two q-points per call (one per lane, or
     one lane for the range-A FP32 run, whose all-vector refinement workspace
-    is ~100 GB per q-point); handle creation (force-constant upload and
+    is ~90 GB per q-point); handle creation (force-constant upload and
     symmetrisation) is not timed.  Reports q-pts/s per run, max |dw| of FP32
     against FP64, and the FP32 eigenvector residual |D z - lam z| / |D|_2 at K
     with D from the library's FP64 assembly."""
