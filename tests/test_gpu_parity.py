@@ -439,3 +439,32 @@ def test_block_sparse_central_difference_velocities(gpu_available):
     assert np.abs(r1["eval_cm"][0] - r0["eval_cm"][0]).max() <= 1e-9
     v1, v0 = r1["vel"][0], r0["vel"][0]
     assert np.abs(v1 - v0).max() <= 1e-6 * np.abs(v0).max()
+
+
+@pytest.mark.parametrize("prec", ["D", "S"])
+def test_block_sparse_range_i_with_velocities(gpu_available, prec):
+    """A range-I selection (60 lowest modes, eigenvectors and velocities) on
+    the block-sparse path, where the forms run over the selected modes only
+    and the FP32 path adds the complement correction with the block-sparse
+    D X: FP64 path against its dense products (frequencies 1e-9 cm^-1,
+    velocities 1e-8 of the largest); FP32 path against the FP64 path's
+    frequencies (1e-3 cm^-1)."""
+    lib = _lib()
+    from parphom_amd import synth
+    st = synth.tbg_structure(8, 9)
+    fc = synth.force_constants(st, 2008)
+    q = synth.q_grid_shifted(16, 2)[[9]]
+    res = {}
+    for p, modes in (("D", (1, 0)), (prec, (1,))):
+        for mode in modes:
+            with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, rng="I",
+                            il=1, iu=60, precision=p, batch=1, streams=1) as h:
+                assert (h.sparse(mode) > 0) == (mode == 1)
+                r = h.run(q)
+            assert r["info"][0] == 0 and len(r["eval_cm"][0]) == 60
+            res[(p, mode)] = r
+    d1, d0 = res[("D", 1)], res[("D", 0)]
+    assert np.abs(d1["eval_cm"][0] - d0["eval_cm"][0]).max() <= 1e-9
+    assert np.abs(d1["vel"][0] - d0["vel"][0]).max() <= 1e-8 * np.abs(d0["vel"][0]).max()
+    if prec == "S":
+        assert np.abs(res[("S", 1)]["eval_cm"][0] - d0["eval_cm"][0]).max() <= 1e-3
