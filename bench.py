@@ -466,17 +466,19 @@ def main():
     # launch, on the FP32 matrix cores.  Block-sparse forms (nblk lower 3x3
     # blocks, each a phase sum times a real block Phi~): per block and mode
     # c = z_a^H (Phi~ z_b), a real 3x3 block times a complex 3-vector (36
-    # flops) and the dot with the row's 3-vector (24), then per direction
-    # Re(p c) weighted and accumulated (4): 72 flops for the three directions,
-    # in FP64 on the vector units.
+    # flops) and the dot with the row's 3-vector (24), then per operator
+    # Re(p c) weighted and accumulated (4).  On the FP32 path one launch forms
+    # the Rayleigh numerators and the three velocity forms (76 flops per block
+    # and mode, in FP64 on the vector units); its phase also holds the
+    # transpose of the vectors to the mode-contiguous layout.
     vel_s = phases["velocity"] / 1e3
     if nblk > 0:
-        vel_flops = B * n * 72.0 * nblk
+        vel_flops = B * n * 76.0 * nblk
         vel_rl = {"bound": "fp64 vector", "achieved": vel_flops / vel_s / 1e12,
                   "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                   "frac": vel_flops / vel_s / FP64_PEAK, "traffic": None,
-                  "kernel": "k_sparse_dots<float, 3> (velocity forms on %d nonzero "
-                            "force-constant blocks)" % nblk}
+                  "kernel": "k_sparse_dots<float, 4> (Rayleigh numerators and velocity forms "
+                            "on %d nonzero force-constant blocks) + k_transpose" % nblk}
     else:
         vel_flops = B * 12.0 * n ** 3
         vel_rl = {"bound": "mfma", "achieved": vel_flops / vel_s / 1e12,

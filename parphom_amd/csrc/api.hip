@@ -720,22 +720,35 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     if (fp32) {
       // ---- FP64 refinement: lambda_k = Re(z_k^H D z_k) / |z_k|^2 with D in FP64
       L.Z64.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
-      L.tm.begin(PH_REFINE, s);
-      launch_convert<float, double>(L.Z.as<cplx<float>>(), L.Z64.as<cplx<double>>(), B * n * mcap, s);
       L.num.ensure((size_t)B * mcap * sizeof(double));
       L.den.ensure((size_t)B * mcap * sizeof(double));
       launch_fill_zero(L.num.p, B * mcap * sizeof(double), s);
-      if (h.use_sparse()) {
-        // from the nonzero blocks of D, with the FP32 vectors in a
-        // mode-contiguous copy (exact in FP64)
+      const bool forms = h.use_sparse();
+      if (forms) {
+        // Rayleigh numerators from the nonzero blocks of D, with the FP32
+        // vectors in a mode-contiguous copy (exact in FP64); with velocities
+        // requested, the three velocity forms come from the same pass over
+        // the vectors (one launch, timed as the velocity phase; the
+        // velocity step then only scales them)
+        const bool v = h.opts.vel != 0;
+        if (v) {
+          L.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
+          launch_fill_zero(L.vacc.p, B * mcap * 3 * sizeof(double), s);
+          L.tm.begin(PH_VEL, s);
+        }
         L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<float>));
         launch_transpose<float>(L.Z.as<cplx<float>>(), n, mmax, n, n * mcap, L.Zt.as<cplx<float>>(),
                                 mcap, n * mcap, B, s);
         launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                  h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, 1,
+                                  h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1,
                                   L.Zt.as<cplx<float>>(), mcap, n * mcap, h.N, mmax, B,
-                                  L.num.as<double>(), 0, mcap, s);
-      } else {
+                                  L.num.as<double>(), mcap, v ? L.vacc.as<double>() : nullptr, mcap,
+                                  3 * mcap, s);
+        if (v) L.tm.end(PH_VEL, s);
+      }
+      L.tm.begin(PH_REFINE, s);
+      launch_convert<float, double>(L.Z.as<cplx<float>>(), L.Z64.as<cplx<double>>(), B * n * mcap, s);
+      if (!forms) {
         GemmParams<double> g{};
         g.opA = 'N'; g.opB = 'N';
         g.m = n; g.n = mmax; g.k = n;
@@ -828,26 +841,21 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   launch_eval_to_cm(R.lam, L.om.as<double>(), B, mmax, mcap, s);
   out.om = L.om.as<double>();
 
-  if (vel && mmax > 0) {
+  // the FP32 path with block-sparse forms formed the velocity forms in the
+  // same launch as its Rayleigh numerators (timed there as the velocity phase)
+  const bool vel_done = sparse && fp32;
+  if (vel && mmax > 0 && !vel_done) {
     L.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
     launch_fill_zero(L.vacc.p, B * mcap * 3 * sizeof(double), s);
     L.tm.begin(PH_VEL, s);
     if (sparse) {
-      // the FP32 path transposed its vectors for the Rayleigh quotients
-      if (!fp32) {
-        L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
-        launch_transpose<double>(L.Z.as<cplx<double>>(), n, mmax, n, n * mcap,
-                                 L.Zt.as<cplx<double>>(), mcap, n * mcap, B, s);
-        launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                   h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 1, 3,
-                                   L.Zt.as<cplx<double>>(), mcap, n * mcap, N, mmax, B,
-                                   L.vacc.as<double>(), mcap, 3 * mcap, s);
-      } else {
-        launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                  h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 1, 3,
-                                  L.Zt.as<cplx<float>>(), mcap, n * mcap, N, mmax, B,
-                                  L.vacc.as<double>(), mcap, 3 * mcap, s);
-      }
+      L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
+      launch_transpose<double>(L.Z.as<cplx<double>>(), n, mmax, n, n * mcap,
+                               L.Zt.as<cplx<double>>(), mcap, n * mcap, B, s);
+      launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                                 h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 1, 3,
+                                 L.Zt.as<cplx<double>>(), mcap, n * mcap, N, mmax, B, nullptr, 0,
+                                 L.vacc.as<double>(), mcap, 3 * mcap, s);
     } else if (fp32) {
       GemmParams<float> g{};
       g.opA = 'N'; g.opB = 'N';
@@ -874,6 +882,8 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
       launch_gemm<double>(g, s);
     }
     L.tm.end(PH_VEL, s);
+  }
+  if (vel && mmax > 0) {
     launch_vel_scale(L.vacc.as<double>(), L.om.as<double>(), B, mmax, mcap, mcap, s);
     out.vel = L.vacc.as<double>() + B * 3 * mcap;
   }

@@ -41,15 +41,18 @@ void launch_sparse_blocks(const AsmParams& P, const int32_t* brow, const int32_t
                           int nops, cplx<double>* bph, hipStream_t s);
 
 // ---------------------------------------------------------------- block-sparse quadratic forms (sparse.hip)
-// out[b * out_b + o * out_op + i] += Re(z_i^H Op_{op0 + o} z_i), o < nops (1 or 3),
-// for modes i < M, from the blocks of rows a (rowptr[a] .. rowptr[a + 1]):
-// Op = phase bph[...op] times phit; z_i from the mode-contiguous Zt (row r of
-// matrix b at b * strideT + r * ldt).
+// Re(z_i^H Op z_i) for the operators op = op0 .. op0 + nops - 1 (nops 1, 3 or
+// 4; op 0 = D, 1..3 = dD/dq_{x,y,z}) and modes i < M, from the blocks of rows
+// a (rowptr[a] .. rowptr[a + 1]): Op = phase bph[...op] times phit; z_i from the
+// mode-contiguous Zt (row r of matrix b at b * strideT + r * ldt).  Added to
+// out0[b * out0_b + i] for op 0 and out1[b * out1_b + (op - 1) * out1_op + i]
+// for the others.
 template <typename Tz>
 void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk,
                         const double* phit, const cplx<double>* bph, int op0, int nops,
                         const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
-                        int64_t batch, double* out, int64_t out_op, int64_t out_b, hipStream_t s);
+                        int64_t batch, double* out0, int64_t out0_b, double* out1,
+                        int64_t out1_op, int64_t out1_b, hipStream_t s);
 // Y = D X for k columns (n x k per matrix, ld n) from the listed blocks: row
 // lists (rowptr, bcol) and their transposes (tptr, trow, tblk: for atom a the
 // blocks (c, a) with c > a), phases bph (op 0) and real blocks phit

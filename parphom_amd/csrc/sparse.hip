@@ -31,8 +31,9 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
                                                      const cplx<double>* __restrict__ bph, int op0,
                                                      const cplx<Tz>* __restrict__ Zt, int64_t ldt,
                                                      int64_t strideT, int64_t N, int64_t M,
-                                                     double* __restrict__ out, int64_t out_op,
-                                                     int64_t out_b) {
+                                                     double* __restrict__ out0, int64_t out0_b,
+                                                     double* __restrict__ out1, int64_t out1_op,
+                                                     int64_t out1_b) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.z;
   if (i >= M) return;
@@ -84,36 +85,43 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
     }
   }
 #pragma unroll
-  for (int o = 0; o < NOPS; ++o) atomicAdd(out + b * out_b + o * out_op + i, acc[o]);
+  for (int o = 0; o < NOPS; ++o) {
+    const int op = op0 + o;
+    double* dst = (op == 0) ? out0 + b * out0_b + i : out1 + b * out1_b + (op - 1) * out1_op + i;
+    atomicAdd(dst, acc[o]);
+  }
 }
 
 template <typename Tz>
 void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk,
                         const double* phit, const cplx<double>* bph, int op0, int nops,
                         const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
-                        int64_t batch, double* out, int64_t out_op, int64_t out_b, hipStream_t s) {
+                        int64_t batch, double* out0, int64_t out0_b, double* out1,
+                        int64_t out1_op, int64_t out1_b, hipStream_t s) {
   if (M <= 0 || batch <= 0 || N <= 0) return;
   if (op0 < 0 || op0 + nops > 4) throw std::runtime_error("launch_sparse_dots: operators out of range");
+  if ((op0 == 0 && !out0) || (op0 + nops > 1 && !out1))
+    throw std::runtime_error("launch_sparse_dots: missing output");
   dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, kDotAtoms), (unsigned)batch);
-  if (nops == 1)
-    hipLaunchKernelGGL((k_sparse_dots<Tz, 1>), grid, dim3(256), 0, s, rowptr, bcol, nblk, phit, bph,
-                       op0, Zt, ldt, strideT, N, M, out, out_op, out_b);
-  else if (nops == 3)
-    hipLaunchKernelGGL((k_sparse_dots<Tz, 3>), grid, dim3(256), 0, s, rowptr, bcol, nblk, phit, bph,
-                       op0, Zt, ldt, strideT, N, M, out, out_op, out_b);
-  else
-    throw std::runtime_error("launch_sparse_dots: nops must be 1 or 3");
+#define PPH_DOTS(K)                                                                          \
+  hipLaunchKernelGGL((k_sparse_dots<Tz, K>), grid, dim3(256), 0, s, rowptr, bcol, nblk, phit, \
+                     bph, op0, Zt, ldt, strideT, N, M, out0, out0_b, out1, out1_op, out1_b)
+  if (nops == 1) PPH_DOTS(1);
+  else if (nops == 3) PPH_DOTS(3);
+  else if (nops == 4) PPH_DOTS(4);
+  else throw std::runtime_error("launch_sparse_dots: nops must be 1, 3 or 4");
+#undef PPH_DOTS
   PPH_CHECK_LAUNCH();
 }
 
 template void launch_sparse_dots<float>(const int64_t*, const int32_t*, int64_t, const double*,
                                         const cplx<double>*, int, int, const cplx<float>*, int64_t,
                                         int64_t, int64_t, int64_t, int64_t, double*, int64_t,
-                                        int64_t, hipStream_t);
+                                        double*, int64_t, int64_t, hipStream_t);
 template void launch_sparse_dots<double>(const int64_t*, const int32_t*, int64_t, const double*,
                                          const cplx<double>*, int, int, const cplx<double>*,
                                          int64_t, int64_t, int64_t, int64_t, int64_t, double*,
-                                         int64_t, int64_t, hipStream_t);
+                                         int64_t, double*, int64_t, int64_t, hipStream_t);
 
 // Y = D X for k columns of X (n x k per matrix, column-major, ld n): thread =
 // (atom row a, column c).  Row a of D is the listed blocks (a, b), b <= a,
