@@ -659,6 +659,10 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   // the FP32 path's eigenvalues come from the FP64 refinement; bisection on
   // its FP32 tridiagonal only supplies the inverse-iteration shifts
   te.rtol = fp32 ? 1e-13 : 0.0;
+  // with shifts within 1e-13 |T| and pairs closer than 1e-9 |T| handled as
+  // clusters, two solves leave at most (1e-13 / 1e-9)^2 of another vector in
+  // an FP32-path vector, below what its FP32 back-transformation keeps
+  te.nsolve = fp32 ? 2 : 3;
   te.ilo = L.tilo.as<int64_t>(); te.ihi = L.tihi.as<int64_t>();
   te.tinfo = L.tinfo.as<double>();
   te.w = L.tw.as<double>(); te.ldw = mcap;

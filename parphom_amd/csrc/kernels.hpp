@@ -139,6 +139,7 @@ struct TriEigWork {
   int64_t strideT;
   char range; double vl, vu; int64_t il, iu; double abstol; double orfac;
   double rtol;                             // bisection stops at rtol * |T| too (0: abstol only)
+  int nsolve;                              // inverse-iteration solves per vector (2 or 3)
   int64_t* ilo; int64_t* ihi;              // per matrix index range found [ilo, ihi)
   double* tinfo;                           // per matrix: gl, gu, tnorm, pivmin
   double* w; int64_t ldw;                  // eigenvalues: w[b*ldw + k]

@@ -347,15 +347,16 @@ __global__ void __launch_bounds__(256) k_tri_inviter(TriEigWork w, int use_lds) 
   const double sigma = perturbed_shift(wv, k, tnorm);
   const double tol = fmax(ULP * tnorm, SAFMIN);
   const TriFac F = tri_fac(fac, n, ld);
-  // three inverse-iteration solves from a random start; each solve's scaling
-  // by 1 / max|y| is applied as the next forward pass reads y
+  // w.nsolve inverse-iteration solves from a random start (2 or 3); each
+  // solve's scaling by 1 / max|y| is applied as the next forward pass reads y
+  const int ns = w.nsolve > 1 ? w.nsolve : 3;
   const uint64_t seed = (uint64_t)(w.ilo[b] + k) * 0x100000001B3ull;
   tri_factor_start(d, e, n, sigma, tol, F, ld, k, seed, y);
   double ssq = 0.0, big = 0.0;
   double mx = tri_back(F, e, n, ld, k, y, false, ssq, big);
-  for (int it = 1; it < 3; ++it) {
+  for (int it = 1; it < ns; ++it) {
     tri_forward(F, n, ld, k, y, (mx > 0.0) ? 1.0 / mx : 1.0);
-    mx = tri_back(F, e, n, ld, k, y, it == 2, ssq, big);
+    mx = tri_back(F, e, n, ld, k, y, it == ns - 1, ssq, big);
   }
   // unit norm, largest entry positive
   double sc = 1.0 / sqrt(ssq);
