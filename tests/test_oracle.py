@@ -255,3 +255,43 @@ def test_golden_regression_tbg_1_2():
         r = S.solve(q, vel=True)
         assert np.allclose(r["eval_cm"], z["eval_cm"][iq], atol=1e-9)
         assert np.allclose(S.D(q), z["D"][iq], atol=1e-14)
+
+
+def test_block_factorisation_of_D_and_its_quadratic_forms():
+    """The block-sparse forms rest on two facts about the oracle's D(q) and
+    dD/dq (dynamical_matrix.F90:241-258): each 3x3 block is a phase sum times
+    the q-independent real block Phi~ = Phi_sym / |S| / sqrt(m m), and the
+    (j, i) block is the Hermitian transpose of the (i, j) block (mirrored tie
+    sets).  With both, the quadratic form z^H Op z over the nonzero lower
+    blocks (weight 2 off the diagonal, the diagonal block whole) equals the
+    dense form.  Checked on the (3,4) cell, whose spring model leaves 70% of
+    the blocks zero."""
+    st, _ = U.structure(3, 4, 1003)
+    S = U.oracle_system(3, 4, 1003)
+    N = st.natom
+    m = np.asarray(S.masses)[np.asarray(S.types) - 1]
+    blk = S.phi.reshape(N, 3, N, 3).transpose(0, 2, 1, 3)          # (i, j, alpha, beta)
+    nz = np.abs(blk).max(axis=(2, 3)) > 0
+    assert nz.mean() < 0.4
+    phit = blk / (S.mult * np.sqrt(m[:, None] * m[None, :]))[:, :, None, None]
+    rng = np.random.default_rng(7)
+    q = rng.uniform(-0.5, 0.5, 3) * np.array([1, 1, 0])
+    Z = rng.standard_normal((3 * N, 5)) + 1j * rng.standard_normal((3 * N, 5))
+    lo_a, lo_b = np.nonzero(np.tril(nz))
+    for deriv in (0, 1, 2, 3):
+        D = S.D(q, deriv)
+        p = O._pair_phases(q, S.crys, S.lat, S.mask, deriv)
+        # block = phase sum times Phi~; Hermitian partner
+        Db = D.reshape(N, 3, N, 3).transpose(0, 2, 1, 3)
+        scale = np.abs(D).max()
+        assert np.abs(Db - p[:, :, None, None] * phit).max() <= 1e-14 * scale
+        assert np.abs(p - p.T.conj())[nz].max() <= 1e-12 * max(1.0, np.abs(p).max())
+        assert np.abs(D - D.conj().T).max() <= 1e-14 * scale
+        # lower-block quadratic forms against the dense ones
+        zb = Z.reshape(N, 3, -1)
+        forms = np.zeros(Z.shape[1])
+        for a, b in zip(lo_a, lo_b):
+            c = np.einsum("xm,xy,ym->m", zb[a].conj(), phit[a, b], zb[b])
+            forms += (1.0 if a == b else 2.0) * np.real(p[a, b] * c)
+        dense = np.real(np.einsum("im,ij,jm->m", Z.conj(), D, Z))
+        assert np.abs(forms - dense).max() <= 1e-12 * np.abs(dense).max()
