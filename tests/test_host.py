@@ -235,3 +235,53 @@ def test_tuner_picks_fastest_candidate_and_records_rejections():
     assert len(rejected) == 8 and all(r["qps"] is None and "error" in r for r in rejected)
     # one warm-up and two timed calls per accepted candidate
     assert len(calls) == 3 * 8
+
+
+def test_complement_correction_formula():
+    """The refinement's second-order eigenvalue correction for a selection
+    that starts at the bottom of the spectrum: with x an approximate lowest
+    eigenvector (error spread over all modes, as rounding leaves it), theta its
+    Rayleigh quotient and r = A x - theta x, the sum over the computed modes
+    -sum_j |x_j^H r|^2 / (rho_j - theta) misses the modes above the
+    selection; their share is -r_perp^H (P (A - theta) P)^+ r_perp, r_perp
+    the residual outside the computed span.  With it the estimate is third
+    order in the error; without it, not.  Conjugate gradients on
+    P (A - theta) P (as the library runs them) give the same share."""
+    rng = np.random.default_rng(3)
+    n, M, eps = 120, 20, 1e-4
+    lam = np.sort(np.concatenate([[0.001], rng.uniform(0.5, 10.0, n - 1)]))
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = (Q * lam) @ Q.conj().T
+    A = 0.5 * (A + A.conj().T)
+    # computed modes: exact ones perturbed by eps in random directions
+    X = Q[:, :M] + eps * (rng.standard_normal((n, M)) + 1j * rng.standard_normal((n, M)))
+    X /= np.linalg.norm(X, axis=0)
+    rho = np.real(np.einsum("im,ij,jm->m", X.conj(), A, X))
+    x = X[:, 0]
+    th = rho[0]
+    r = A @ x - th * x
+    inspan = -sum(abs(np.vdot(X[:, j], r)) ** 2 / (rho[j] - th) for j in range(1, M))
+    P = np.eye(n) - X @ np.linalg.solve(X.conj().T @ X, X.conj().T)
+    rp = P @ r
+    Ap = P @ (A - th * np.eye(n)) @ P
+    comp = -np.real(np.vdot(rp, np.linalg.lstsq(Ap, rp, rcond=None)[0]))
+    # conjugate gradients on the projected operator, from zero
+    z = np.zeros(n, complex)
+    res = rp.copy()
+    pdir = rp.copy()
+    rr = np.real(np.vdot(res, res))
+    for _ in range(60):
+        qv = Ap @ pdir
+        al = rr / np.real(np.vdot(pdir, qv))
+        z += al * pdir
+        res -= al * qv
+        rn = np.real(np.vdot(res, res))
+        pdir = res + (rn / rr) * pdir
+        rr = rn
+    comp_cg = -np.real(np.vdot(rp, z))
+    assert abs(comp_cg - comp) <= 1e-6 * abs(comp)
+    err_th = abs(th - lam[0])
+    err_span = abs(th + inspan - lam[0])
+    err_full = abs(th + inspan + comp - lam[0])
+    assert err_full < 1e-3 * err_th
+    assert err_span > 100 * err_full
