@@ -195,6 +195,7 @@ class Handle:
         q = np.asfortranarray(np.atleast_2d(np.asarray(q, np.float64)))
         nq = q.shape[0]
         t = out_tensors
+        self._check_device_outputs(nq, t)
 
         def dp(x, ct=C.c_double):
             return C.cast(C.c_void_p(x.data_ptr()), C.POINTER(ct)) if x is not None else None
@@ -206,6 +207,47 @@ class Handle:
                      C.c_void_p(t["evec"].data_ptr()) if t.get("evec") is not None else None,
                      cols, dp(t.get("vel")), dp(t["info"], C.c_int32))
         _check(lib.pph_run_device(self._h, nq, _ptr(q), nq, C.byref(out)), self._h)
+
+    def _check_device_outputs(self, nq, t):
+        """The library writes every output at strides derived from its shape
+        (eval_cm rows of ld_eval, evec and vel blocks of evec_cols modes, m /
+        nz / info for all nq rows), so a tensor of the wrong dtype, layout,
+        device or size would be written out of bounds: reject it here."""
+        import torch
+        want = {"eval_cm": torch.float64, "m": torch.int64, "nz": torch.int64,
+                "info": torch.int32, "evec": torch.complex128, "vel": torch.float64}
+        dev = None
+        for key, dt in want.items():
+            x = t.get(key)
+            if x is None:
+                if key in ("eval_cm", "m", "info"):
+                    raise ValueError(f"run_device: output '{key}' is required")
+                continue
+            if not isinstance(x, torch.Tensor) or x.device.type != "cuda":
+                raise ValueError(f"run_device: '{key}' must be a CUDA tensor")
+            if x.dtype != dt:
+                raise ValueError(f"run_device: '{key}' must be {dt}, got {x.dtype}")
+            if not x.is_contiguous():
+                raise ValueError(f"run_device: '{key}' must be contiguous")
+            if x.device.index != self.opts.device:
+                raise ValueError(f"run_device: '{key}' is on cuda:{x.device.index}, the handle "
+                                 f"on device {self.opts.device}")
+            if x.shape[0] < nq:
+                raise ValueError(f"run_device: '{key}' has {x.shape[0]} rows for {nq} q-points")
+            dev = x.device
+        if t["eval_cm"].dim() != 2 or t["eval_cm"].shape[1] < self.mcap:
+            raise ValueError(f"run_device: eval_cm must be (nq, >= {self.mcap})")
+        ev, vel = t.get("evec"), t.get("vel")
+        if ev is not None and (ev.dim() != 3 or ev.shape[2] != self.n
+                               or ev.shape[1] < self.mcap):
+            raise ValueError(f"run_device: evec must be (nq, >= {self.mcap}, {self.n})")
+        if vel is not None:
+            if vel.dim() != 3 or vel.shape[2] != 3 or vel.shape[1] < self.mcap:
+                raise ValueError(f"run_device: vel must be (nq, >= {self.mcap}, 3)")
+            if ev is not None and vel.shape[1] != ev.shape[1]:
+                raise ValueError("run_device: vel and evec must have the same number of columns "
+                                 "(both are written evec_cols apart)")
+        return dev
 
     def blocking(self, nb_panel=32, nb_back=128):
         """Reflectors per reduction panel and per back-transformation block

@@ -53,12 +53,14 @@ def vel_cluster_sums(v, groups):
     return np.array([v[g].sum(axis=0) for g in groups])
 
 
-def vel_noise(dD_list, om_cm, lam, groups, kappa=4.0):
+def vel_noise(dD_list, om_cm, lam, groups, kappa=4.0, eps=None):
     """Conditioning of the analytic velocity of each mode in FP64.  A backward
     stable eigensolver returns e with errors ~ eps |D| / gap along the other
     eigenvectors (gap = distance to the nearest eigenvalue outside the mode's
     cluster), so <e|dD|e> is determined to ~ 2 eps |D| |dD| / gap; scaled by
-    the velocity factor / max(2|omega|, 1e-3)."""
+    the velocity factor / max(2|omega|, 1e-3).  eps defaults to the FP64 unit
+    roundoff; the FP32 path's vectors carry the FP32 one."""
+    eps = np.finfo(float).eps if eps is None else eps
     lam = np.asarray(lam)
     n = len(lam)
     dnorm = max(np.linalg.norm(d, 2) for d in dD_list)
@@ -75,7 +77,7 @@ def vel_noise(dD_list, om_cm, lam, groups, kappa=4.0):
     gap = np.maximum(gap, np.finfo(float).eps * Dnorm)
     den = np.maximum(2.0 * np.abs(om_cm), 1e-3)
     const = O.C2 * O.C1SQ32 * 100.0 * 2.0 * O.PI32
-    return kappa * 2.0 * n * np.finfo(float).eps * Dnorm * dnorm / gap * const / den
+    return kappa * 2.0 * n * eps * Dnorm * dnorm / gap * const / den
 
 
 def assert_vel_close(v, v_ref, groups, noise, rel=1e-6, floor=1e-9):

@@ -162,6 +162,22 @@ def test_run_tbg_3_4_path(gpu_available, prec, sparse):
             vp_ref = U.vel_projected(ref["evec"], dD, ref["eval_cm"], groups)
             tol = 1e-6 * np.abs(vp_ref) + 1e-9 + noise[:, None]
             assert np.all(np.abs(vp - vp_ref) <= tol)
+        else:
+            # FP32 path: the velocities are forms of the FP32 vectors (FP32
+            # matrix-core products here, FP64 block sums with the block lists),
+            # so the same gauge-invariant cluster sums are held to the noise
+            # of an FP32 eigenvector (eps32 in the conditioning model) and to
+            # 1e-5 of the largest velocity of the q-point
+            groups = U.clusters(ref["lam"], 1e-6)
+            dD = [S.D(q, a) for a in (1, 2, 3)]
+            noise = U.vel_noise(dD, ref["eval_cm"], ref["lam"], groups,
+                                eps=float(np.finfo(np.float32).eps))
+            vmax = np.abs(ref["vel"]).max()
+            for g in groups:
+                a = res["vel"][iq][g].sum(axis=0)
+                b = ref["vel"][g].sum(axis=0)
+                tol = 1e-5 * vmax * len(g) + noise[g].sum()
+                assert np.all(np.abs(a - b) <= tol), (g, a, b, tol)
 
 
 @pytest.mark.parametrize("prec", ["D", "S"])

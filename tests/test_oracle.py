@@ -199,6 +199,47 @@ def test_analytic_derivative_vs_finite_difference():
         assert np.abs(fd - an).max() < 1e-7 * np.abs(an).max()
 
 
+def _cd_error(S, q, a, cd):
+    """max |CD - analytic * pi32/pi| / max |analytic| for direction a."""
+    an = S.D(q, a) * (O.PI32 / np.pi)
+    return np.abs(cd - an).max() / np.abs(an).max()
+
+
+@pytest.mark.parametrize("a", [1, 2, 3])
+def test_central_difference_matches_analytic(a):
+    """The oracle's central difference (dynamical_matrix.F90:133-149) is the
+    derivative along Cartesian q_a: with the phase exp(-2 pi32 i q.Delta), the
+    exact derivative is the analytic dD/dq_a times pi32/pi, and the central
+    difference with step s = max(f32(1e-6) sqrt|b1x b2y - b1y b2x|, f32(1e-6))
+    is within O(s^2) truncation plus O(eps |D| / s) rounding of it (both far
+    below 1e-6 here).  The hexagonal cell makes inv(rec) non-symmetric, so a
+    step mapped with inv(rec) instead of inv(rec)^T moves q along the wrong
+    direction; a wrong normalisation (2|delta|) halves the result: the planted
+    variants below must fail the same check."""
+    _, S = _tblg21()
+    # a non-orthogonal reciprocal lattice is what separates inv(rec) from inv(rec)^T
+    assert abs(S.rec[0] @ S.rec[1]) > 0.1 * np.linalg.norm(S.rec[0]) ** 2
+    for q in (np.array([0.21, 0.33, 0.0]), np.array([-0.37, 0.08, 0.11])):
+        cd = O.assemble_cd(q, a, S.phi, S.crys, S.lat, S.rec, S.masses, S.types, S.mask, S.mult)
+        assert _cd_error(S, q, a, cd) < 1e-6
+        # the step itself: |delta| = s, its crystal components solve rec^T dc = delta
+        s = O.cd_delta(S.rec)
+        assert s == max(O.DEL_INC32 * np.sqrt(abs(S.rec[0, 0] * S.rec[1, 1]
+                                                   - S.rec[0, 1] * S.rec[1, 0])), O.DEL_INC32)
+        # planted mistakes (on the CPU, in a copy of the oracle's formula)
+        delta = np.zeros(3)
+        delta[a - 1] = s
+        dc_wrong = np.linalg.inv(S.rec) @ delta                  # transposed inv(rec)
+        Dp = S.D(q + dc_wrong / 2)
+        Dm = S.D(q - dc_wrong / 2)
+        wrong_dir = (Dp - Dm) / s
+        dc = np.linalg.inv(S.rec).T @ delta
+        wrong_norm = (S.D(q + dc / 2) - S.D(q - dc / 2)) / (2 * s)
+        if a != 3:          # rec has no xz / yz coupling: the z step is the same either way
+            assert _cd_error(S, q, a, wrong_dir) > 1e-2
+        assert _cd_error(S, q, a, wrong_norm) > 0.4
+
+
 # ------------------------------------------------------------ eigen
 def test_zheevx_against_eigh_and_ranges():
     S = U.oracle_system(3, 4, 1003)
