@@ -152,6 +152,24 @@ int pph_dos_triangulation(int64_t nk, int64_t nbands, const double* freq, const 
                           int64_t ntri, const int64_t* tri, int64_t nE, const double* E,
                           double* dos, double* nd, int32_t device, int32_t on_device);
 
+/* pph_dos_triangulation without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same sums as pph_dos_triangulation; every pointer is a device
+ * pointer on `device` (no host copies, no argument checks that would read the
+ * arrays: vertex indices must lie in [0, nk) and E must be ascending), func
+ * may be NULL (1 everywhere).  The kernels run asynchronously on `stream`
+ * (a hipStream_t; NULL = the null stream) and the call returns after
+ * enqueuing them.  The difference array of the triangulation is a static
+ * device array of 2^20 doubles per GPU (8 MiB, held until the process ends):
+ * nE above 2^20 is rejected with PPH_ERR_ARG.  Every call on one GPU shares
+ * that array (the call clears it first), so calls whose execution overlaps on
+ * one GPU (different streams, concurrent graph replays) race. */
+int pph_dos_triangulation_noalloc(int64_t nk, int64_t nbands, const double* freq,
+                                  const double* func, int64_t ntri, const int64_t* tri, int64_t nE,
+                                  const double* E, double* dos, double* nd, int32_t device,
+                                  void* stream);
+
 /* Density of states by smearing, bz_integration.f90:140-199
  * (standard_integration with gaussian / lorentzian) summed over q-points as
  * pyphutil.py:311-325 does: dos[i] = sum_q weight[q]/natom sum_b

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <exception>
 #include <thread>
@@ -1447,18 +1448,11 @@ int pph_dos_triangulation(int64_t nk, int64_t nbands, const double* freq, const 
   const hipStream_t s = st.s;
   const size_t fb = (size_t)nk * nbands * sizeof(double);
   DevIn df, dg, dE;
-  DevBuf ones, dtri, dout, dstep;
+  DevBuf dtri, dout, dstep;
   df.from(freq, fb, on_device, s);
-  if (func) {
-    dg.from(func, fb, on_device, s);
-  } else {
-    // the reference's default weight function is 1 everywhere
-    std::vector<double> h1((size_t)nk * nbands, 1.0);
-    ones.ensure(std::max<size_t>(fb, 8));
-    PPH_HIP(hipMemcpyAsync(ones.p, h1.data(), fb, hipMemcpyHostToDevice, s));
-    dg.p = ones.as<double>();
-    PPH_HIP(hipStreamSynchronize(s));
-  }
+  // no func: the reference's default weight function, 1 everywhere (k_lt
+  // reads a null weight array as ones)
+  if (func) dg.from(func, fb, on_device, s);
   dE.from(E, nE * sizeof(double), on_device, s);
   const int64_t* dt = tri;
   if (!on_device && ntri > 0) {
@@ -1480,6 +1474,70 @@ int pph_dos_triangulation(int64_t nk, int64_t nbands, const double* freq, const 
     PPH_HIP(hipMemcpyAsync(nd, on, nE * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   PPH_HIP(hipStreamSynchronize(s));
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
+// ---------------------------------------------------------------- no-allocation variant
+// The difference array of the triangulation (the energies' "above the
+// triangle" steps) lives in a file-scope device array instead of a buffer the
+// library allocates, so the entry below allocates nothing.  Every GPU holds
+// its own copy; host code uses the address hipGetSymbolAddress returns for the
+// current device, never the array's name.
+constexpr int64_t kDosScratchE = int64_t(1) << 20;       // energies (8 MiB of doubles)
+static __device__ __align__(256) double scratch[kDosScratchE];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (looked up once per device, with it current)
+double* dos_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int prev = 0;
+    PPH_HIP(hipGetDevice(&prev));
+    PPH_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      throw pph::hip_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
+int pph_dos_triangulation_noalloc(int64_t nk, int64_t nbands, const double* freq,
+                                  const double* func, int64_t ntri, const int64_t* tri, int64_t nE,
+                                  const double* E, double* dos, double* nd, int32_t device,
+                                  void* stream) {
+  if (nk < 0 || nbands < 0 || ntri < 0 || nE < 0)
+    return fail(nullptr, "pph_dos_triangulation_noalloc: negative size", PPH_ERR_ARG);
+  if (nE > kDosScratchE)
+    return fail(nullptr, "pph_dos_triangulation_noalloc: nE = " + std::to_string(nE) +
+                             " exceeds the scratch capacity of " + std::to_string(kDosScratchE) +
+                             " energies (use pph_dos_triangulation)", PPH_ERR_ARG);
+  if (nE > 0 && (!E || !dos || !nd))
+    return fail(nullptr, "pph_dos_triangulation_noalloc: E, dos and nd are required", PPH_ERR_ARG);
+  if (ntri > 0 && nbands > 0 && (!freq || !tri))
+    return fail(nullptr, "pph_dos_triangulation_noalloc: freq and tri are required", PPH_ERR_ARG);
+  PPH_API_TRY
+  double* step = dos_scratch(device);
+  int prev = 0;
+  PPH_HIP(hipGetDevice(&prev));
+  PPH_HIP(hipSetDevice(device));
+  try {
+    launch_dos_lt(freq, func, nbands, tri, ntri, E, nE, dos, nd, step,
+                  static_cast<hipStream_t>(stream));
+  } catch (...) {
+    (void)hipSetDevice(prev);
+    throw;
+  }
+  PPH_HIP(hipSetDevice(prev));
   return PPH_OK;
   PPH_API_CATCH(nullptr)
 }

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_lt(const double* freq, const double* fu
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t k = tri[3 * t + c];
-      v[c] = Vtx{freq[k * nb + b], func[k * nb + b]};
+      v[c] = Vtx{freq[k * nb + b], func ? func[k * nb + b] : 1.0};
       lo = (c == 0) ? v[c].e : fmin(lo, v[c].e);
       hi = (c == 0) ? v[c].e : fmax(hi, v[c].e);
     }

@@ -208,8 +208,9 @@ void launch_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count, hipStream_
 void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);
 
 // ---------------------------------------------------------------- density of states (dos.hip)
-// freq, func: nk x nb row-major (freq[k * nb + b]); tri: ntri x 3 row-major,
-// 0-based; E ascending; step: nE scratch.  All device pointers.
+// freq, func: nk x nb row-major (freq[k * nb + b]; func null: 1 everywhere, the
+// reference's default); tri: ntri x 3 row-major, 0-based; E ascending; step: nE
+// scratch.  All device pointers.
 void launch_dos_lt(const double* freq, const double* func, int64_t nb, const int64_t* tri,
                    int64_t ntri, const double* E, int64_t nE, double* dos, double* nd, double* step,
                    hipStream_t s);

@@ -39,7 +39,8 @@ class PphOut(C.Structure):
 
 
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
-            "pph_dos_triangulation", "pph_dos_smearing", "pph_fc_symmetrize", "pph_destroy",
+            "pph_dos_triangulation", "pph_dos_triangulation_noalloc", "pph_dos_smearing",
+            "pph_fc_symmetrize", "pph_destroy",
             "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
             "pph_debug_sparse")
@@ -84,11 +85,15 @@ def load(path: str = LIB_PATH):
                                     C.c_int64, C.c_int32, _dp]
     lib.pph_dos_triangulation.argtypes = [C.c_int64, C.c_int64, _dp, _dp, C.c_int64, _i64p,
                                           C.c_int64, _dp, _dp, _dp, C.c_int32, C.c_int32]
+    lib.pph_dos_triangulation_noalloc.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                                  C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.pph_dos_smearing.argtypes = [C.c_int32, C.c_int64, C.c_int64, _dp, _dp, C.c_double,
                                      C.c_int64, C.c_int64, _dp, _dp, C.c_int32, C.c_int32]
     lib.pph_fc_symmetrize.argtypes = [C.c_int64, _dp, C.c_int64, C.c_int32, C.c_int32]
     for name in ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
-                 "pph_destroy", "pph_debug_zgemm", "pph_dos_triangulation", "pph_dos_smearing",
+                 "pph_destroy", "pph_debug_zgemm", "pph_dos_triangulation",
+                 "pph_dos_triangulation_noalloc", "pph_dos_smearing",
                  "pph_fc_symmetrize"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
@@ -342,6 +347,34 @@ def dos_triangulation(freq, tri, E, func=None, device=0):
                                      len(tri), _ptr(tri, C.c_int64), len(E), _ptr(E), _ptr(dos),
                                      _ptr(nd), device, 0))
     return dos, nd
+
+
+def dos_triangulation_noalloc(freq, tri, E, dos, nd, func=None, stream=None):
+    """pph_dos_triangulation_noalloc on torch CUDA tensors (no allocation in
+    the library): freq, func (nk, nbands) float64, tri (ntri, 3) int64, E
+    (nE,) float64 ascending, dos and nd (nE,) float64 outputs, all contiguous
+    on one GPU.  Enqueued on `stream` (a torch.cuda.Stream; None = torch's
+    current stream); synchronise before reading dos / nd."""
+    import torch
+    for name, x, dt in (("freq", freq, torch.float64), ("tri", tri, torch.int64),
+                        ("E", E, torch.float64), ("dos", dos, torch.float64),
+                        ("nd", nd, torch.float64)) + ((("func", func, torch.float64),)
+                                                       if func is not None else ()):
+        if x.device.type != "cuda" or x.dtype != dt or not x.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous {dt} CUDA tensor")
+    nk, nb = freq.shape
+    if func is not None and func.shape != freq.shape:
+        raise ValueError("func must have the shape of freq")
+    if tri.dim() != 2 or tri.shape[1] != 3:
+        raise ValueError("tri must be (ntri, 3)")
+    nE = E.shape[0]
+    if dos.shape[0] < nE or nd.shape[0] < nE:
+        raise ValueError("dos and nd need nE entries")
+    dev = freq.device.index
+    st = torch.cuda.current_stream(freq.device) if stream is None else stream
+    _check(load().pph_dos_triangulation_noalloc(
+        nk, nb, freq.data_ptr(), None if func is None else func.data_ptr(), tri.shape[0],
+        tri.data_ptr(), nE, E.data_ptr(), dos.data_ptr(), nd.data_ptr(), dev, st.cuda_stream))
 
 
 def dos_smearing(method, freq, weight, width, E, natom, device=0):
