@@ -26,9 +26,12 @@ int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t
  * phase of pph_run / pph_run_device.  Slots: 0 assembly, 1 tridiagonal
  * reduction, 2 tridiagonal eigensolver, 3 back-transformation, 4 FP64
  * refinement, 5 velocity products, 6 whole batch, 7 the k_hemv_sym launches
- * of the reduction alone, sampled at columns j = 0, 8, 16, ... (an event
- * pair around each sampled launch; the count is the number of launches
- * timed).  ms/counts (8 entries each, may be NULL) receive
+ * of the one-stage reduction alone, sampled at columns j = 0, 8, 16, ... (an
+ * event pair around each sampled launch; the count is the number of launches
+ * timed), and for the two-stage reduction 8 dense -> band, 9 bulge chasing
+ * (both inside slot 1) and 10 the bulge-chasing reflectors' back-
+ * transformation (inside slot 3).  ms/counts (11 entries each, may be NULL)
+ * receive
  * the accumulated milliseconds and counts; reset != 0 clears them; enable
  * turns recording on or off. */
 int pph_profile(void* h, int32_t enable, int32_t reset, double* ms, int64_t* counts);
@@ -46,6 +49,14 @@ int pph_debug_blocking(void* h, int32_t nb_panel, int32_t nb_back);
  * of lower blocks the products now use, 0 for dense.  No counterpart in the
  * reference, whose velocity products are dense (write_output.F90:415-421). */
 int pph_debug_sparse(void* h, int32_t mode, int64_t* nblk);
+/* Reduction to the tridiagonal: mode 0 one-stage (blocked LATRD/HETRD), 1
+ * two-stage (dense -> band of half-bandwidth `band` on the matrix cores, then
+ * bulge chasing), -1 automatic; band 16 or 32 (0 keeps the current one).
+ * With h == NULL the setting becomes the default of every handle whose own
+ * mode is -1 (also the temporary ones pph_heevx makes).  Both reductions feed
+ * the same tridiagonal eigensolver and back-transformation; no counterpart
+ * in the reference (pzheevx reduces in one stage, diagonalize.F90:73-96). */
+int pph_debug_reduction(void* h, int32_t mode, int32_t band);
 
 #ifdef __cplusplus
 }

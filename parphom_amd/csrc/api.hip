@@ -22,6 +22,10 @@ namespace pph {
 
 static thread_local std::string g_last_error;
 
+// Automatic choice of the reduction: the two-stage path from this order on
+// (below it the one-stage reduction's per-column launches are cheap).
+constexpr int64_t kTwoStageMinN = 1 << 30;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -86,7 +90,10 @@ static void linsolve(const double* x, const double* A, double* u) {
 
 // phase timers recorded on the handle's stream (HIP events)
 // PH_HEMV: the k_hemv_sym launches alone (ms summed, count = launches)
-enum Phase { PH_ASM = 0, PH_TRID, PH_TRIEIG, PH_BACK, PH_REFINE, PH_VEL, PH_TOTAL, PH_HEMV, PH_N };
+// PH_HE2HB, PH_HB2ST, PH_Q2: the parts of the two-stage reduction and of its
+// back-transformation (inside PH_TRID and PH_BACK)
+enum Phase { PH_ASM = 0, PH_TRID, PH_TRIEIG, PH_BACK, PH_REFINE, PH_VEL, PH_TOTAL, PH_HEMV,
+             PH_HE2HB, PH_HB2ST, PH_Q2, PH_N };
 
 struct Timers {
   bool on = false;
@@ -158,6 +165,9 @@ struct Lane {
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
+  // two-stage reduction: band, bulge-chasing reflectors and their block
+  // factors, panel T / Mx, Gram partials, stall flags
+  DevBuf tsAB, tsV2, tsTau2, tsVTb, tsTp, tsG, tsInfo;
   // rocBLAS handle for the plain GEMMs of the back-transformation (made on
   // first use, bound to this lane's stream, atomics off so results repeat)
   rocblas_handle blas = nullptr;
@@ -195,6 +205,12 @@ struct Handle : Lane {
   // persistent device state
   DevBuf phi, crys, pos, lat, rmass, mask, cd_dc;
   double cd_norm = 0.0;
+  // reduction: 0 one-stage (LATRD/HETRD), 1 two-stage (dense -> band ->
+  // tridiagonal), -1 automatic; band: half-bandwidth of the two-stage band
+  int reduction = -1;
+  int band = -1;            // -1: the process default
+  bool two_stage() const;
+  int band_width() const;
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
   int nchunk = 64;
@@ -215,6 +231,20 @@ struct Handle : Lane {
   size_t lanes() const { return 1 + more.size(); }
 };
 
+
+// Reduction chosen for handles whose `reduction` is -1 (automatic); set by
+// pph_debug_reduction(NULL, ...), read by every handle (also the temporary
+// ones pph_heevx makes).
+static int g_reduction_default = -1;
+static int g_band_default = 32;
+
+int Handle::band_width() const { return band > 0 ? band : g_band_default; }
+
+bool Handle::two_stage() const {
+  const int r = reduction >= 0 ? reduction : g_reduction_default;
+  if (r >= 0) return r == 1;
+  return n >= kTwoStageMinN;
+}
 
 // ---------------------------------------------------------------- small dense FP64 (host)
 // Eigenvalues of the Hermitian pencil (H, G), G positive definite, k <= 64:
@@ -613,17 +643,16 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   const int64_t mcap = range_mcap(o, n);
   R.mcap = mcap;
 
-  // ---- tridiagonalisation
+  // ---- reduction to a real tridiagonal
+  const bool two = h.two_stage();
+  const int64_t bwd = h.band_width();
+  const int64_t kmax2 = (n > 1) ? hb2st_tasks(n, bwd, 0) : 1;
+  const int64_t G2 = (n > 1) ? ceil_div(n - 1, bwd) : 1;
   L.V.ensure((size_t)B * n * (n + kVpad) * sizeof(cplx<T>));
-  L.W.ensure((size_t)B * n * nb * sizeof(cplx<T>));
+  L.W.ensure((size_t)B * n * std::max<int64_t>(nb, 2 * bwd) * sizeof(cplx<T>));
   L.tau.ensure((size_t)B * n * sizeof(cplx<T>));
   L.dgl.ensure((size_t)B * n * sizeof(T));
   L.egl.ensure((size_t)B * n * sizeof(T));
-  L.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
-  L.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
-  L.scal.ensure((size_t)B * 256 * sizeof(double));
-  const int64_t nparts = n / 64 + 2;
-  L.part.ensure((size_t)B * 3 * nparts * sizeof(double));
   TridiagWork<T> tw{};
   tw.n = n; tw.lda = h.ldA; tw.batch = B; tw.nb = nb;
   tw.A = A; tw.strideA = h.ldA * n;
@@ -631,14 +660,49 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   tw.W = L.W.as<cplx<T>>(); tw.strideW = n * nb;
   tw.tau = L.tau.as<cplx<T>>(); tw.strideTau = n;
   tw.d = L.dgl.as<T>(); tw.e = L.egl.as<T>(); tw.strideDE = n;
-  tw.ypart = L.ypart.as<cplx<T>>(); tw.strideY = n;
-  tw.dpart = L.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
-  tw.scal = L.scal.as<double>(); tw.strideScal = 256;
-  tw.part = L.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
-  tw.nchunk = h.nchunk;
-  tw.hemv_ev = L.tm.kernel_events(ceil_div(n - 1, kHemvTimeStride));
+  TwoStageWork<T> ts{};
   L.tm.begin(PH_TRID, s);
-  tridiagonalise<T>(tw, s);
+  if (two) {
+    constexpr int64_t kChunks = 16;
+    L.tsAB.ensure((size_t)B * n * 2 * bwd * sizeof(cplx<T>));
+    L.tsV2.ensure((size_t)B * n * kmax2 * bwd * sizeof(cplx<T>));
+    L.tsTau2.ensure((size_t)B * n * kmax2 * sizeof(cplx<T>));
+    L.tsTp.ensure((size_t)B * 2 * bwd * bwd * sizeof(cplx<T>));
+    L.tsG.ensure((size_t)B * kChunks * bwd * bwd * sizeof(cplx<T>));
+    L.tsInfo.ensure((size_t)B * sizeof(int));
+    launch_fill_zero(L.tsInfo.p, B * sizeof(int), s);
+    ts.n = n; ts.lda = h.ldA; ts.batch = B; ts.bw = bwd;
+    ts.A = A; ts.strideA = h.ldA * n;
+    ts.V = tw.V; ts.strideV = tw.strideV;
+    ts.tau = tw.tau; ts.strideTau = n;
+    ts.X = L.W.as<cplx<T>>(); ts.VT = ts.X + n * bwd; ts.strideXV = 2 * n * bwd;
+    ts.Tp = L.tsTp.as<cplx<T>>(); ts.strideTp = 2 * bwd * bwd;
+    ts.gpart = L.tsG.as<cplx<T>>(); ts.nchunk = kChunks;
+    ts.AB = L.tsAB.as<cplx<T>>(); ts.strideAB = n * 2 * bwd;
+    ts.V2 = L.tsV2.as<cplx<T>>(); ts.tau2 = L.tsTau2.as<cplx<T>>(); ts.kmax = kmax2;
+    ts.strideV2 = n * kmax2 * bwd; ts.strideTau2 = n * kmax2;
+    ts.d = tw.d; ts.e = tw.e; ts.strideDE = n;
+    ts.info = L.tsInfo.as<int>();
+    L.tm.begin(PH_HE2HB, s);
+    he2hb<T>(ts, s);
+    L.tm.end(PH_HE2HB, s);
+    L.tm.begin(PH_HB2ST, s);
+    hb2st<T>(ts, s);
+    L.tm.end(PH_HB2ST, s);
+  } else {
+    L.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
+    L.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
+    L.scal.ensure((size_t)B * 256 * sizeof(double));
+    const int64_t nparts = n / 64 + 2;
+    L.part.ensure((size_t)B * 3 * nparts * sizeof(double));
+    tw.ypart = L.ypart.as<cplx<T>>(); tw.strideY = n;
+    tw.dpart = L.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
+    tw.scal = L.scal.as<double>(); tw.strideScal = 256;
+    tw.part = L.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
+    tw.nchunk = h.nchunk;
+    tw.hemv_ev = L.tm.kernel_events(ceil_div(n - 1, kHemvTimeStride));
+    tridiagonalise<T>(tw, s);
+  }
   L.tm.end(PH_TRID, s);
 
   // ---- tridiagonal eigenproblem in FP64
@@ -674,9 +738,13 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   std::vector<int64_t> ilo(B), ihi(B);
   PPH_HIP(hipMemcpyAsync(ilo.data(), te.ilo, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   PPH_HIP(hipMemcpyAsync(ihi.data(), te.ihi, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  std::vector<int> rinfo(B, 0);
+  if (two)
+    PPH_HIP(hipMemcpyAsync(rinfo.data(), ts.info, B * sizeof(int), hipMemcpyDeviceToHost, s));
   PPH_HIP(hipStreamSynchronize(s));
   R.M.assign(B, 0);
-  R.info.assign(B, 0);
+  // a stalled bulge-chasing pipeline reports as a convergence failure
+  R.info.assign(rinfo.begin(), rinfo.end());
   int64_t mmax = 0;
   for (int64_t b = 0; b < B; ++b) {
     R.M[b] = std::min(mcap, ihi[b] - ilo[b]);
@@ -718,6 +786,22 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     bw.Tm = L.Tm.as<cplx<T>>(); bw.VT = L.VT.as<cplx<T>>(); bw.Wt = L.Wt.as<cplx<T>>();
     bw.blas = L.blas_on(s);
     L.tm.begin(PH_BACK, s);
+    if (two) {
+      // Z = Q1 (Q2 Z_T): the bulge-chasing reflectors first, then the he2hb
+      // panels (their reflectors start bw rows below their column)
+      launch_zt_to_z<T>(te.zt, ldzt, te.strideZt, bw.Z, bw.ldz, bw.strideZ, n, mmax, B, s);
+      L.tsVTb.ensure((size_t)B * G2 * kmax2 * 2 * bwd * bwd * sizeof(cplx<T>));
+      Q2Work<T> q2{};
+      q2.n = n; q2.batch = B; q2.bw = bwd; q2.kmax = kmax2; q2.m = mmax;
+      q2.V2 = ts.V2; q2.tau2 = ts.tau2; q2.strideV2 = ts.strideV2; q2.strideTau2 = ts.strideTau2;
+      q2.VTb = L.tsVTb.as<cplx<T>>(); q2.strideVTb = G2 * kmax2 * 2 * bwd * bwd;
+      q2.Z = bw.Z; q2.ldz = bw.ldz; q2.strideZ = bw.strideZ;
+      L.tm.begin(PH_Q2, s);
+      apply_q2<T>(q2, s);
+      L.tm.end(PH_Q2, s);
+      bw.shift = bwd;
+      bw.z_ready = true;
+    }
     back_transform<T>(bw, s);
     L.tm.end(PH_BACK, s);
     R.Zt = L.Z.p;
@@ -1613,6 +1697,22 @@ int pph_destroy(pph_handle* hh) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- measurement hook
+extern "C" int pph_debug_reduction(pph_handle* hh, int32_t mode, int32_t band) {
+  if (mode < -1 || mode > 1)
+    return fail(nullptr, "pph_debug_reduction: mode must be -1, 0 or 1", PPH_ERR_ARG);
+  if (band != 0 && band != 16 && band != 32)
+    return fail(nullptr, "pph_debug_reduction: band must be 0 (keep), 16 or 32", PPH_ERR_ARG);
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) {
+    g_reduction_default = mode;
+    if (band) g_band_default = band;
+    return PPH_OK;
+  }
+  h->reduction = mode;
+  if (band) h->band = band;
+  return PPH_OK;
+}
+
 extern "C" int pph_debug_sparse(pph_handle* hh, int32_t mode, int64_t* nblk) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return fail(nullptr, "pph_debug_sparse: handle is NULL", PPH_ERR_ARG);

@@ -41,11 +41,11 @@ __global__ void __launch_bounds__(256) k_zt_to_z(const double* __restrict__ zt, 
 template <typename T>
 __global__ void __launch_bounds__(256) k_form_t(const cplx<T>* __restrict__ G, cplx<T>* __restrict__ Tm,
                                                const cplx<T>* __restrict__ tau, int64_t strideTau,
-                                               int64_t n, int64_t nb, int64_t npanel,
+                                               int64_t nrefl, int64_t nb, int64_t npanel,
                                                int64_t strideWs) {
   const int64_t p = blockIdx.x, b = blockIdx.y;
   const int64_t j0 = p * nb;
-  const int64_t pw = min(nb, n - 1 - j0);
+  const int64_t pw = min(nb, nrefl - j0);
   const cplx<T>* g = G + b * strideWs + p * nb * nb;
   cplx<T>* t = Tm + b * strideWs + p * nb * nb;
   const int i = threadIdx.x;
@@ -103,17 +103,28 @@ static void blas_gemm(rocblas_handle h, bool conjA, int64_t m, int64_t n, int64_
 }
 
 template <typename T>
+void launch_zt_to_z(const double* zt, int64_t ldzt, int64_t strideZt, cplx<T>* Z, int64_t ldz,
+                    int64_t strideZ, int64_t n, int64_t m, int64_t batch, hipStream_t s) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid((unsigned)ceil_div(m, 32), (unsigned)ceil_div(n, 32), (unsigned)batch);
+  hipLaunchKernelGGL(k_zt_to_z<T>, grid, dim3(256), 0, s, zt, ldzt, strideZt, Z, ldz, strideZ, n,
+                     m);
+  PPH_CHECK_LAUNCH();
+}
+template void launch_zt_to_z<float>(const double*, int64_t, int64_t, cplx<float>*, int64_t, int64_t,
+                                    int64_t, int64_t, int64_t, hipStream_t);
+template void launch_zt_to_z<double>(const double*, int64_t, int64_t, cplx<double>*, int64_t,
+                                     int64_t, int64_t, int64_t, int64_t, hipStream_t);
+
+template <typename T>
 void back_transform(const BackWork<T>& w, hipStream_t s) {
   const int64_t n = w.n, nb = w.nb, M = w.m, B = w.batch;
+  const int64_t sh = w.shift;
   if (M <= 0) return;
-  const int64_t npanel = (n > 1) ? ceil_div(n - 1, nb) : 0;
+  const int64_t nrefl = std::max<int64_t>(0, n - sh);
+  const int64_t npanel = ceil_div(nrefl, nb);
   // Z_T -> complex Z
-  {
-    dim3 grid((unsigned)ceil_div(M, 32), (unsigned)ceil_div(n, 32), (unsigned)B);
-    hipLaunchKernelGGL(k_zt_to_z<T>, grid, dim3(256), 0, s, w.zt, w.ldzt, w.strideZt, w.Z, w.ldz,
-                       w.strideZ, n, M);
-    PPH_CHECK_LAUNCH();
-  }
+  if (!w.z_ready) launch_zt_to_z<T>(w.zt, w.ldzt, w.strideZt, w.Z, w.ldz, w.strideZ, n, M, B, s);
   if (npanel == 0) return;
   // Gram matrices of all panels: G_p = V(:, p nb : p nb + nb)^H V(:, ...), all n rows
   cplx<T>* G = w.Wt;          // Wt workspace doubles as G storage before the panel loop
@@ -130,12 +141,12 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
   }
   if (nb > 256) throw std::runtime_error("back_transform: block width must be <= 256");
   hipLaunchKernelGGL(k_form_t<T>, dim3((unsigned)npanel, (unsigned)B), dim3((unsigned)nb), 0, s, G, w.Tm,
-                     w.tau, w.strideTau, n, nb, npanel, w.strideWs);
+                     w.tau, w.strideTau, nrefl, nb, npanel, w.strideWs);
   PPH_CHECK_LAUNCH();
   for (int64_t p = npanel - 1; p >= 0; --p) {
     const int64_t j0 = p * nb;
-    const int64_t mr = n - j0 - 1;                 // rows j0+1 .. n-1
-    const int64_t pw = std::min(nb, n - 1 - j0);
+    const int64_t mr = n - j0 - sh;                // rows j0 + sh .. n-1
+    const int64_t pw = std::min(nb, nrefl - j0);
     const int64_t kw = ceil_div(pw, 16) * 16;
     rocblas_handle blas = reinterpret_cast<rocblas_handle>(w.blas);
     // Wt = V_p^H Z(j0+1:, :)   (kw x M)
@@ -145,8 +156,8 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
       g.opA = 'C'; g.opB = 'N';
       g.m = kw; g.n = M; g.k = mr;
       g.alpha = cplx<T>{1, 0}; g.beta = cplx<T>{0, 0};
-      g.A = w.V + (j0 + 1) + j0 * n; g.lda = n; g.strideA = w.strideV;
-      g.B = w.Z + (j0 + 1); g.ldb = w.ldz; g.strideB = w.strideZ;
+      g.A = w.V + (j0 + sh) + j0 * n; g.lda = n; g.strideA = w.strideV;
+      g.B = w.Z + (j0 + sh); g.ldb = w.ldz; g.strideB = w.strideZ;
       g.C = w.Wt; g.ldc = nb; g.strideC = w.strideWs;
       g.batch = B;
       const int64_t tiles = ceil_div(M, 64) * B;
@@ -159,7 +170,7 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
       g.opA = 'N'; g.opB = 'N';
       g.m = mr; g.n = kw; g.k = kw;
       g.alpha = cplx<T>{1, 0}; g.beta = cplx<T>{0, 0};
-      g.A = w.V + (j0 + 1) + j0 * n; g.lda = n; g.strideA = w.strideV;
+      g.A = w.V + (j0 + sh) + j0 * n; g.lda = n; g.strideA = w.strideV;
       g.B = w.Tm + p * nb * nb; g.ldb = nb; g.strideB = w.strideWs;
       g.C = w.VT; g.ldc = n; g.strideC = w.strideWs;
       g.batch = B;
@@ -168,7 +179,7 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
     // Z(j0+1:, :) -= VT Wt
     if (blas) {
       blas_gemm(blas, false, mr, M, kw, T(-1), w.VT, n, w.strideWs, w.Wt, nb, w.strideWs, T(1),
-                w.Z + (j0 + 1), w.ldz, w.strideZ, B);
+                w.Z + (j0 + sh), w.ldz, w.strideZ, B);
     } else {
       GemmParams<T> g{};
       g.opA = 'N'; g.opB = 'N';
@@ -176,7 +187,7 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
       g.alpha = cplx<T>{-1, 0}; g.beta = cplx<T>{1, 0};
       g.A = w.VT; g.lda = n; g.strideA = w.strideWs;
       g.B = w.Wt; g.ldb = nb; g.strideB = w.strideWs;
-      g.C = w.Z + (j0 + 1); g.ldc = w.ldz; g.strideC = w.strideZ;
+      g.C = w.Z + (j0 + sh); g.ldc = w.ldz; g.strideC = w.strideZ;
       g.batch = B;
       launch_gemm<T>(g, s);
     }

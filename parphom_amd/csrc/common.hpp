@@ -97,6 +97,28 @@ __device__ __forceinline__ void atomic_store_rel(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---------------------------------------------------------------- Householder reflector
+// H = I - tau v v^H with v = (1, x / (alpha - beta)) and H^H (alpha, x) = (beta, 0):
+// beta = -sign(Re alpha) sqrt(|alpha|^2 + |x|^2), tau = (beta - alpha) / beta
+// (the ZLARFG formulas without the rescaling loop); x = 0 with a real alpha
+// gives tau = 0.  sr, si: scale = 1 / (alpha - beta).
+struct Reflector { double beta, tr, ti, sr, si; };
+
+__device__ __forceinline__ Reflector make_reflector(double xn2, double ar, double ai) {
+  Reflector r{ar, 0, 0, 0, 0};
+  if (!(xn2 == 0.0 && ai == 0.0)) {
+    const double nrm = sqrt(ar * ar + ai * ai + xn2);
+    r.beta = (ar >= 0.0) ? -nrm : nrm;
+    r.tr = (r.beta - ar) / r.beta;
+    r.ti = -ai / r.beta;
+    const double dr = ar - r.beta, di = ai;
+    const double den = dr * dr + di * di;
+    r.sr = dr / den;
+    r.si = -di / den;
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------- host errors
 struct hip_error : std::runtime_error {
   using std::runtime_error::runtime_error;

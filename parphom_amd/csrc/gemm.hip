@@ -283,9 +283,25 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
     throw std::runtime_error("launch_gemm: herm_dot needs the dot epilogue, opA = 'N', square A, one product");
   // a K step must not straddle the two products
   if (p.A2 != nullptr && p.k1 % GBK != 0) throw std::runtime_error("launch_gemm: k1 must be a multiple of 16");
+  const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);
+  // outputs at most 32 columns wide (the panel products of the band
+  // reduction) take 64 x 32 tiles, so no MFMA block is spent on empty columns
+  if (p.n <= 32 && !dot && !p.lower) {
+    dim3 g32((unsigned)ceil_div(p.m, 64), 1u, (unsigned)(p.batch * bin * ks));
+#define PPH_GEMM_CASE32(CA, CB)                                                    \
+    if (ca == CA && cb == CB) {                                                    \
+      hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, 2, 1>), g32, dim3(256), 0, s, p); \
+      PPH_CHECK_LAUNCH();                                                          \
+      return;                                                                      \
+    }
+    PPH_GEMM_CASE32(false, false)
+    PPH_GEMM_CASE32(true, false)
+    PPH_GEMM_CASE32(false, true)
+    PPH_GEMM_CASE32(true, true)
+#undef PPH_GEMM_CASE32
+  }
   dim3 grid((unsigned)ceil_div(p.m, 64), (unsigned)ceil_div(p.n, 64),
             (unsigned)(p.batch * bin * ks));
-  const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);
 #define PPH_GEMM_CASE(CA, CB, DT)                                                  \
   if (ca == CA && cb == CB && dot == DT) {                                         \
     hipLaunchKernelGGL((k_zgemm<T, CA, CB, DT>), grid, dim3(256), 0, s, p);        \

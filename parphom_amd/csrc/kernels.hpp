@@ -127,6 +127,54 @@ constexpr int64_t kHemvTimeStride = 8;
 
 template <typename T> void tridiagonalise(const TridiagWork<T>& w, hipStream_t s);
 
+// ---------------------------------------------------------------- two-stage reduction (twostage.hip)
+// Dense -> band of half-bandwidth bw (he2hb: panel QR of bw columns, then the
+// two-sided update of the trailing matrix on the matrix cores), band -> real
+// tridiagonal (hb2st: bulge chasing, one workgroup per matrix with sweeps
+// pipelined across its waves).  A must hold a Hermitian matrix in its lower
+// triangle; it is destroyed.
+template <typename T>
+struct TwoStageWork {
+  int64_t n, lda, batch, bw;          // bw: 16 or 32
+  cplx<T>* A; int64_t strideA;
+  // he2hb reflectors: V n x (n + pad) explicit (column c: unit at row c + bw,
+  // zeros above), tau n
+  cplx<T>* V; int64_t strideV;
+  cplx<T>* tau; int64_t strideTau;
+  // panel workspaces, leading dimension n: X (= A VT, then Y), VT = V T
+  cplx<T>* X; cplx<T>* VT; int64_t strideXV;      // n x bw each
+  cplx<T>* Tp; int64_t strideTp;                 // T (bw x bw) then Mx (bw x bw)
+  cplx<T>* gpart; int64_t nchunk;                // per-chunk Gram partials: batch x nchunk x bw^2
+  // band storage AB[c * 2bw + d] = A(c + d, c), d < 2bw (the bulges reach 2bw - 1)
+  cplx<T>* AB; int64_t strideAB;
+  // bulge-chasing reflectors: sweep j, step k at V2[(j kmax + k) bw + i], tau2[j kmax + k]
+  cplx<T>* V2; cplx<T>* tau2; int64_t kmax, strideV2, strideTau2;
+  T* d; T* e; int64_t strideDE;                  // the real tridiagonal
+  int* info;                                     // per matrix: bit PPH_INFO_NOCONV on a stalled pipeline
+};
+
+template <typename T> void he2hb(const TwoStageWork<T>& w, hipStream_t s);
+template <typename T> void hb2st(const TwoStageWork<T>& w, hipStream_t s);
+// bulge-chasing steps of sweep j (its reflectors k = 0 .. ntask - 1)
+__host__ __device__ inline int64_t hb2st_tasks(int64_t n, int64_t bw, int64_t j) {
+  return (n - 2 - j) / bw + 1;
+}
+
+// Q2 Z: the bulge-chasing reflectors applied to the columns of Z (n x m,
+// leading dimension ldz), in blocks of bw sweeps x one step (compact WY with
+// VT = V T precomputed per block into VTb, 2bw x bw each, g kmax + k).
+template <typename T>
+struct Q2Work {
+  int64_t n, batch, bw, kmax, m;
+  const cplx<T>* V2; const cplx<T>* tau2; int64_t strideV2, strideTau2;
+  cplx<T>* VTb; int64_t strideVTb;
+  cplx<T>* Z; int64_t ldz, strideZ;
+};
+template <typename T> void apply_q2(const Q2Work<T>& w, hipStream_t s);
+template <typename T>
+void launch_zt_to_z(const double* zt, int64_t ldzt, int64_t strideZt, cplx<T>* Z, int64_t ldz,
+                    int64_t strideZ, int64_t n, int64_t m, int64_t batch, hipStream_t s);
+
 // ---------------------------------------------------------------- tridiagonal eigensolver (FP64)
 // doubles per matrix for the inverse-iteration factors: u0, u1, l (n x ld
 // each) and a pivot byte per element
@@ -167,6 +215,11 @@ struct BackWork {
   // rocblas_handle bound to the stream: the two plain products of every
   // panel go to rocBLAS's GEMM (null: the library's own GEMM)
   void* blas;
+  // reflector j has its unit at row j + shift (1 after the one-stage
+  // reduction, bw after he2hb), so there are n - shift of them
+  int64_t shift = 1;
+  // Z already holds the vectors to transform (the Z_T conversion skipped)
+  bool z_ready = false;
 };
 
 template <typename T> void back_transform(const BackWork<T>& w, hipStream_t s);

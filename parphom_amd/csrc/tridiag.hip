@@ -37,27 +37,6 @@ namespace pph {
 // per-matrix scalar slots
 constexpr int S_TAU = 3, S_SCALE = 5, S_A2 = 8;
 
-// Reflector of column j: H = I - tau v v^H with v = (1, x / (alpha - beta)),
-// beta = -sign(Re alpha) sqrt(|alpha|^2 + |x|^2), tau = (beta - alpha) / beta
-// (the ZLARFG formulas without the rescaling loop).
-struct Reflector { double beta, tr, ti, sr, si; };
-
-__device__ __forceinline__ Reflector make_reflector(double xn2, double ar, double ai) {
-  Reflector r{ar, 0, 0, 0, 0};
-  if (!(xn2 == 0.0 && ai == 0.0)) {
-    const double nrm = sqrt(ar * ar + ai * ai + xn2);
-    r.beta = (ar >= 0.0) ? -nrm : nrm;
-    r.tr = (r.beta - ar) / r.beta;
-    r.ti = -ai / r.beta;
-    // scale = 1 / (alpha - beta)
-    const double dr = ar - r.beta, di = ai;
-    const double den = dr * dr + di * di;
-    r.sr = dr / den;
-    r.si = -di / den;
-  }
-  return r;
-}
-
 // Sum of np per-workgroup partials (stride `st` doubles apart), in a fixed order.
 __device__ __forceinline__ double sum_partials(const double* x, int64_t np, int st, double* sh) {
   double s = 0.0;

@@ -43,10 +43,10 @@ EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expe
             "pph_fc_symmetrize", "pph_destroy",
             "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
-            "pph_debug_sparse")
+            "pph_debug_sparse", "pph_debug_reduction")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
-          "hemv_launches")
+          "hemv_launches", "he2hb", "hb2st", "q2")
 
 _lib = None
 
@@ -80,6 +80,8 @@ def load(path: str = LIB_PATH):
     lib.pph_debug_blocking.restype = C.c_int
     lib.pph_debug_sparse.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
     lib.pph_debug_sparse.restype = C.c_int
+    lib.pph_debug_reduction.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.pph_debug_reduction.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -259,6 +261,12 @@ class Handle:
         (a test hook: the defaults are the measured best)."""
         _check(load().pph_debug_blocking(self._h, nb_panel, nb_back), self._h)
 
+    def reduction(self, mode=1, band=0):
+        """Reduction to the tridiagonal on this handle: 0 one-stage, 1
+        two-stage (dense -> band -> tridiagonal), -1 automatic; band 16 or 32
+        (0 keeps the current one)."""
+        _check(load().pph_debug_reduction(self._h, mode, band), self._h)
+
     def sparse(self, mode=1):
         """Products with D and dD in the Rayleigh quotients and velocities:
         0 dense, 1 automatic (the default), 2 block-sparse whenever block lists
@@ -283,6 +291,13 @@ class Handle:
         out = np.zeros((nq, self.n, self.n), np.complex128)   # each matrix column-major
         _check(lib.pph_assemble(self._h, nq, _ptr(q), nq, what, out.ctypes.data, 0), self._h)
         return out.transpose(0, 2, 1)
+
+
+def set_default_reduction(mode=-1, band=0):
+    """Process default of the reduction for handles left on automatic (also
+    the temporary handles of heevx): 0 one-stage, 1 two-stage, -1 the
+    library's choice."""
+    _check(load().pph_debug_reduction(None, mode, band))
 
 
 def heevx(A, jobz="V", rng="A", vl=0.0, vu=0.0, il=1, iu=1, precision="D",
