@@ -45,7 +45,7 @@ def device_outputs(torch, B, n):
     }
 
 
-def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams):
+def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1):
     """Each step is one pph_run_device call over streams x B q-points: the
     library keeps `streams` batches of B in flight (one stream and host thread
     each).  Returns the timed seconds, the phase profile of the timed steps
@@ -56,6 +56,7 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams):
     h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision=prec,
                    batch=B, streams=streams)
     nblk = h.sparse(1)               # blocks the quadratic forms use (0 = dense products)
+    h.reduction(reduction)           # -1: the library's choice
     per_step = B * streams
     out = device_outputs(torch, per_step, n)
     nq = len(qs)
@@ -389,6 +390,9 @@ def main():
     ap.add_argument("--streams", type=int, default=2,
                     help="batches in flight per pph_run call (one stream each)")
     ap.add_argument("--workload", default="tbg_11_12")
+    ap.add_argument("--reduction", type=int, default=-1, choices=(-1, 0, 1),
+                    help="reduction to the tridiagonal: 0 one-stage, 1 two-stage, "
+                         "-1 the library's choice")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fcsym", action="store_true",
@@ -434,7 +438,7 @@ def main():
 
     B, S = a.batch, a.streams
     dt32, prof32, iso32, ev32, z32, nblk = time_path(lib, torch, st, fc, qs, "S", B, a.steps,
-                                                     a.warmup, S)
+                                                     a.warmup, S, a.reduction)
     qps32 = a.steps * B * S / dt32
     res = {"metric": "q-pts/s at 3N=4764 eigpairs+v_g: FP32+refine vs FP64, max|dw|<=1e-3 cm^-1",
            "value": qps32, "unit": "q-pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -522,7 +526,8 @@ def main():
         # FP64 workspaces are larger: half the batch per stream keeps both in HBM
         B64 = max(1, B // 2) if S > 1 else B
         dt64, prof64, _, ev64, _, _ = time_path(lib, torch, st, fc, qs, "D", B64,
-                                             max(1, a.steps - 1), 0 if a.warmup == 0 else 1, S)
+                                             max(1, a.steps - 1), 0 if a.warmup == 0 else 1, S,
+                                             a.reduction)
         qps64 = max(1, a.steps - 1) * B64 * S / dt64
         ev64 = ev64[:min(B, B64)]
         ev32 = ev32[:min(B, B64)]

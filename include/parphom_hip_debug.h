@@ -11,6 +11,9 @@
  *   mode 4: as mode 1 for Hermitian A (m = k, opA = 'N'), reading only the
  *           lower triangle of A (the half-work path of the velocity and
  *           Rayleigh-quotient products)
+ *   mode 5: C = A B for Hermitian A (m = k, opA = 'N') from the lower
+ *           triangle of A alone (the two triangular products of the band
+ *           reduction's panel update)
  * prec 'D' complex128, 'S' complex64; column-major; returns 0 or < 0.
  */
 #ifndef PARPHOM_HIP_DEBUG_H

@@ -790,11 +790,12 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       // Z = Q1 (Q2 Z_T): the bulge-chasing reflectors first, then the he2hb
       // panels (their reflectors start bw rows below their column)
       launch_zt_to_z<T>(te.zt, ldzt, te.strideZt, bw.Z, bw.ldz, bw.strideZ, n, mmax, B, s);
-      L.tsVTb.ensure((size_t)B * G2 * kmax2 * 2 * bwd * bwd * sizeof(cplx<T>));
+      L.tsVTb.ensure((size_t)B * G2 * kmax2 * 2 * bwd * bwd * sizeof(cplx<T>) * 2);
       Q2Work<T> q2{};
       q2.n = n; q2.batch = B; q2.bw = bwd; q2.kmax = kmax2; q2.m = mmax;
       q2.V2 = ts.V2; q2.tau2 = ts.tau2; q2.strideV2 = ts.strideV2; q2.strideTau2 = ts.strideTau2;
       q2.VTb = L.tsVTb.as<cplx<T>>(); q2.strideVTb = G2 * kmax2 * 2 * bwd * bwd;
+      q2.Vb = q2.VTb + B * q2.strideVTb;
       q2.Z = bw.Z; q2.ldz = bw.ldz; q2.strideZ = bw.strideZ;
       L.tm.begin(PH_Q2, s);
       apply_q2<T>(q2, s);
@@ -1797,6 +1798,12 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
       // C = A(:, 0:k/2) B(0:k/2, :) + A(:, k/2:) B(k/2:, :) through the two-product path (opA = opB = 'N')
       g.k1 = k / 2; g.A2 = dA.as<cplx<T>>() + (k / 2) * lda; g.B2 = dB.as<cplx<T>>() + k / 2;
       g.strideA2 = 0; g.strideB2 = 0;
+    } else if (mode == 5) {
+      // C = A B for Hermitian A (m = k, opA = 'N') from its lower triangle:
+      // the lower triangle with the diagonal, then its strict part transposed
+      g.tri = 1;
+      launch_gemm<T>(g, s);
+      g.opA = 'C'; g.tri = 2; g.beta = cplx<T>{1, 0};
     }
     launch_gemm<T>(g, s);
   };

@@ -97,6 +97,29 @@ __device__ __forceinline__ void atomic_store_rel(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---------------------------------------------------------------- matrix cores
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct mfma;
+// Operand layout of both 16x16x4 instructions: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16].  Result: lane l holds column l % 16; accumulator
+// register r holds row 4 * (l / 16) + r for f32 but row 4 * r + l / 16 for f64.
+template <> struct mfma<double> {
+  typedef d4 acc;
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + lane / 16; }
+};
+template <> struct mfma<float> {
+  typedef f4 acc;
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
+};
+
 // ---------------------------------------------------------------- Householder reflector
 // H = I - tau v v^H with v = (1, x / (alpha - beta)) and H^H (alpha, x) = (beta, 0):
 // beta = -sign(Re alpha) sqrt(|alpha|^2 + |x|^2), tau = (beta - alpha) / beta

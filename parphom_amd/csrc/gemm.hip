@@ -14,28 +14,6 @@
 
 namespace pph {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct mfma;
-// Operand layout of both 16x16x4 instructions: lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16].  Result: lane l holds column l % 16; accumulator
-// register r holds row 4 * (l / 16) + r for f32 but row 4 * r + l / 16 for f64.
-template <> struct mfma<double> {
-  typedef d4 acc;
-  static __device__ __forceinline__ acc op(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + lane / 16; }
-};
-template <> struct mfma<float> {
-  typedef f4 acc;
-  static __device__ __forceinline__ acc op(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane / 16) + r; }
-};
-
 constexpr int GBK = 16;
 
 // WM x WN MFMA blocks of 16 x 16 per wave: the workgroup tile is
@@ -77,10 +55,13 @@ k_zgemm(GemmParams<T> p) {
     kbeg = ks_idx * kc;
     K = min(p.k, kbeg + kc);
   }
-  // Hermitian dot: A'(r, k) = 0 for k > r, so the row tile needs k < row0 + GBM
+  // Hermitian dot: A'(r, k) = 0 for k > r, so the row tile needs k < row0 + GBM;
+  // triangular op(A) likewise limits the K range of the row tile
   const bool herm = p.herm_dot != 0;
-  if (herm) K = min(K, row0 + GBM);
+  if (herm || p.tri == 1) K = min(K, row0 + GBM);
+  if (p.tri == 2) kbeg = max(kbeg, (row0 / GBK) * GBK);
   const int64_t K1 = two ? p.k1 : K;
+  const bool tri1 = p.tri == 1, tri2 = p.tri == 2;
 
   constexpr bool k3M = true;
   auto product = [](T p1, T p2, T p3) {
@@ -152,13 +133,15 @@ k_zgemm(GemmParams<T> p) {
       if (!CONJA) {
         const int r = tid % GBM, kk = tid / GBM + (256 / GBM) * i;
         const int64_t gr = row0 + r, gk = g.k0 + kk;
-        cplx<T> v = mask(ra[i], gr < M && gk < g.Kseg, false);
+        cplx<T> v = mask(ra[i], gr < M && gk < g.Kseg && !(tri1 && gk > gr) &&
+                                    !(tri2 && gk <= gr), false);
         if (herm && gk >= gr) v = (gk == gr) ? cplx<T>{T(0.5) * v.x, T(0)} : cplx<T>{0, 0};
         As[st][kk][r] = v;
       } else {
         const int kk = tid % 16, r = tid / 16 + 16 * i;
         const int64_t gr = row0 + r, gk = g.k0 + kk;
-        As[st][kk][r] = mask(ra[i], gr < M && gk < g.Kseg, true);
+        As[st][kk][r] = mask(ra[i], gr < M && gk < g.Kseg && !(tri1 && gk > gr) &&
+                                        !(tri2 && gk <= gr), true);
       }
     }
 #pragma unroll
@@ -281,6 +264,8 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
   if (ks > 1 && p.A2 != nullptr) throw std::runtime_error("launch_gemm: split-K with two products");
   if (p.herm_dot && (p.dot_out == nullptr || p.opA != 'N' || p.A2 != nullptr || p.m != p.k))
     throw std::runtime_error("launch_gemm: herm_dot needs the dot epilogue, opA = 'N', square A, one product");
+  if (p.tri && (p.dot_out != nullptr || p.A2 != nullptr || p.ksplit > 1))
+    throw std::runtime_error("launch_gemm: a triangular A takes one product, no dot epilogue, no split-K");
   // a K step must not straddle the two products
   if (p.A2 != nullptr && p.k1 % GBK != 0) throw std::runtime_error("launch_gemm: k1 must be a multiple of 16");
   const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);

@@ -101,6 +101,12 @@ struct GemmParams {
   // x^H A x = 2 Re(x^H A' x); each row tile stops at its own diagonal, which
   // halves the work.
   int herm_dot;
+  // tri != 0 (no dot epilogue, one product): op(A) is triangular and only its
+  // nonzero part is read: 1 lower with the diagonal (op(A)(r, k) = 0 for
+  // k > r), 2 strictly upper (op(A)(r, k) = 0 for k <= r).  With A Hermitian
+  // and stored in its lower triangle, tri 1 with opA 'N' plus tri 2 with opA
+  // 'C' (beta 1) give A B from the lower triangle alone.
+  int tri;
 };
 
 template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);
@@ -162,12 +168,13 @@ __host__ __device__ inline int64_t hb2st_tasks(int64_t n, int64_t bw, int64_t j)
 
 // Q2 Z: the bulge-chasing reflectors applied to the columns of Z (n x m,
 // leading dimension ldz), in blocks of bw sweeps x one step (compact WY with
-// VT = V T precomputed per block into VTb, 2bw x bw each, g kmax + k).
+// V and VT = V T precomputed per block into Vb and VTb, 2bw x bw each, block
+// g kmax + k).
 template <typename T>
 struct Q2Work {
   int64_t n, batch, bw, kmax, m;
   const cplx<T>* V2; const cplx<T>* tau2; int64_t strideV2, strideTau2;
-  cplx<T>* VTb; int64_t strideVTb;
+  cplx<T>* VTb; cplx<T>* Vb; int64_t strideVTb;    // VT and V of every block, same layout
   cplx<T>* Z; int64_t ldz, strideZ;
 };
 template <typename T> void apply_q2(const Q2Work<T>& w, hipStream_t s);

@@ -8,7 +8,8 @@
 //          update of the trailing matrix
 //              VT = V T,  X = A VT,  Mx = VT^H X,  Y = X - V Mx / 2,
 //              A <- A - Y V^H - V Y^H
-//          as GEMMs on the matrix cores.  The trailing matrix is read three
+//          as GEMMs on the matrix cores, on the lower triangle only (X from
+//          two triangular products).  The trailing matrix is read about three
 //          times per panel of bw columns instead of once per column.
 //   hb2st  band -> real tridiagonal by bulge chasing (k_hb2st): sweep j
 //          annihilates column j below the subdiagonal with a reflector of
@@ -27,32 +28,6 @@
 #include "../../include/parphom_hip.h"
 
 namespace pph {
-
-// ---------------------------------------------------------------- Hermitian fill
-// A(r, c) for r < c := conj(A(c, r)) and Im A(c, c) := 0, from the lower
-// triangle (the triangle the library's reductions treat as authoritative).
-template <typename T>
-__global__ void __launch_bounds__(256) k_herm_from_lower(cplx<T>* A, int64_t lda, int64_t strideA,
-                                                         int64_t n) {
-  __shared__ cplx<T> t[32][33];
-  const int64_t ti = blockIdx.y, tj = blockIdx.x;      // source tile: rows ti, columns tj
-  if (tj > ti) return;
-  cplx<T>* M = A + blockIdx.z * strideA;
-  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
-  for (int r = ty; r < 32; r += 8) {
-    const int64_t row = ti * 32 + tx, col = tj * 32 + r;
-    t[tx][r] = (row < n && col < n) ? M[row + col * lda] : cplx<T>{0, 0};
-  }
-  __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    // destination (tj * 32 + tx, ti * 32 + r) = conj(source (ti * 32 + r, tj * 32 + tx))
-    const int64_t drow = tj * 32 + tx, dcol = ti * 32 + r;
-    if (drow >= n || dcol >= n) continue;
-    const cplx<T> v = t[r][tx];
-    if (dcol > drow) M[drow + dcol * lda] = cplx<T>{v.x, -v.y};
-    else if (dcol == drow) M[drow + dcol * lda] = cplx<T>{v.x, T(0)};
-  }
-}
 
 // ---------------------------------------------------------------- panel QR
 // Householder QR of the panel P = A(r0:n, p0:p0+bw), r0 = p0 + bw, in place:
@@ -108,21 +83,24 @@ __global__ void __launch_bounds__(kQrThreads) k_panel_qr(TwoStageWork<T> w, int6
     for (int c = wave; c < bw; c += NWV) {
       if (c <= i) continue;
       cplx<T>* Pc = P + (int64_t)c * lda;
+      // QU rows per lane in flight per iteration (from clamped addresses, so
+      // the loads are unconditional and issued together)
+      constexpr int QU = 8;
       double zr = 0.0, zi = 0.0;
-      for (int64_t r0_ = i; r0_ < m; r0_ += 4 * WAVE) {
-        cplx<T> vv[4], xx[4];
+      for (int64_t r0_ = i; r0_ < m; r0_ += QU * WAVE) {
+        cplx<T> vv[QU], xx[QU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int64_t r = r0_ + lane + u * WAVE;
-          const bool ok = r < m;
-          vv[u] = ok ? Vi[r] : cplx<T>{0, 0};
-          xx[u] = ok ? Pc[r] : cplx<T>{0, 0};
+        for (int u = 0; u < QU; ++u) {
+          const int64_t r = min(r0_ + lane + u * WAVE, m - 1);
+          vv[u] = Vi[r];
+          xx[u] = Pc[r];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < QU; ++u) {
           // conj(v) x
-          zr += (double)vv[u].x * xx[u].x + (double)vv[u].y * xx[u].y;
-          zi += (double)vv[u].x * xx[u].y - (double)vv[u].y * xx[u].x;
+          const double ok = (r0_ + lane + u * WAVE < m) ? 1.0 : 0.0;
+          zr += ok * ((double)vv[u].x * xx[u].x + (double)vv[u].y * xx[u].y);
+          zi += ok * ((double)vv[u].x * xx[u].y - (double)vv[u].y * xx[u].x);
         }
       }
       zr = wave_sum(zr);
@@ -131,12 +109,19 @@ __global__ void __launch_bounds__(kQrThreads) k_panel_qr(TwoStageWork<T> w, int6
       const cplx<T> f{(T)(ctr * zr - cti * zi), (T)(ctr * zi + cti * zr)};
       const bool next = (c == i + 1);
       double s = 0.0;
-      for (int64_t r0_ = i; r0_ < m; r0_ += 4 * WAVE) {
+      for (int64_t r0_ = i; r0_ < m; r0_ += QU * WAVE) {
+        cplx<T> vv[QU], xx[QU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < QU; ++u) {
+          const int64_t r = min(r0_ + lane + u * WAVE, m - 1);
+          vv[u] = Vi[r];
+          xx[u] = Pc[r];
+        }
+#pragma unroll
+        for (int u = 0; u < QU; ++u) {
           const int64_t r = r0_ + lane + u * WAVE;
           if (r >= m) continue;
-          const cplx<T> x = csub(Pc[r], cmul(f, Vi[r]));
+          const cplx<T> x = csub(xx[u], cmul(f, vv[u]));
           Pc[r] = x;
           if (next) {
             if (r > i + 1) s += (double)cabs2(x);
@@ -286,12 +271,6 @@ template <typename T>
 void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
   const int64_t n = w.n, bw = w.bw, B = w.batch;
   if (bw != 16 && bw != 32) throw std::runtime_error("he2hb: band width must be 16 or 32");
-  {
-    const int64_t nt = ceil_div(n, 32);
-    hipLaunchKernelGGL(k_herm_from_lower<T>, dim3((unsigned)nt, (unsigned)nt, (unsigned)B),
-                       dim3(256), 0, s, w.A, w.lda, w.strideA, n);
-    PPH_CHECK_LAUNCH();
-  }
   launch_fill_zero(w.V, B * w.strideV * (int64_t)sizeof(cplx<T>), s);
   launch_fill_zero(w.tau, B * w.strideTau * (int64_t)sizeof(cplx<T>), s);
   for (int64_t p0 = 0; p0 + bw < n; p0 += bw) {
@@ -317,7 +296,8 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.batch = B;
       launch_gemm<T>(g, s);
     }
-    // X = A22 VT   (m x bw)
+    // X = A22 VT   (m x bw), from the lower triangle of A22: its lower part
+    // with the diagonal, then the strict lower part conjugate-transposed
     {
       GemmParams<T> g{};
       g.opA = 'N'; g.opB = 'N';
@@ -327,6 +307,9 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = w.VT + r0; g.ldb = n; g.strideB = w.strideXV;
       g.C = w.X + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
+      g.tri = 1;
+      launch_gemm<T>(g, s);
+      g.opA = 'C'; g.tri = 2; g.beta = cplx<T>{1, 0};
       launch_gemm<T>(g, s);
     }
     // Mx = VT^H X = T^H V^H A22 V T   (bw x bw, into Tp + bw^2)
@@ -347,8 +330,8 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.batch = B;
       launch_gemm<T>(g, s);
     }
-    // A22 -= Y V^H + V Y^H   (both triangles, so the next panel's products
-    // can read A22 whole)
+    // A22 -= Y V^H + V Y^H   (the lower triangle: tiles wholly above the
+    // diagonal are skipped, and nothing above the diagonal is read later)
     {
       GemmParams<T> g{};
       g.opA = 'N'; g.opB = 'C';
@@ -360,6 +343,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B2 = w.X + r0; g.strideB2 = w.strideXV;
       g.C = w.A + r0 + r0 * w.lda; g.ldc = w.lda; g.strideC = w.strideA;
       g.batch = B;
+      g.lower = 1;
       launch_gemm<T>(g, s);
     }
   }
@@ -377,24 +361,17 @@ template void he2hb<double>(const TwoStageWork<double>&, hipStream_t);
 //   the reflector H from the first column of X = A(R_k, R_{k-1}) (k = 0: from
 //   column j), X <- H^H X (its first column becomes (beta, 0, ...)),
 //   D = A(R_k, R_k) <- H^H D H, and the block below, Y = A(R_{k+1}, R_k) <- Y H,
-//   which is the next step's X (kept in registers).
-// A wave holds a bw x bw block as lane = row r (lane % bw) and column group
-// lane / bw of bw^2 / 64 consecutive columns.  Band element (r, c), r >= c,
-// sits at AB[c * 2bw + r - c].
+//   which is the next step's X.
+// Band element (r, c), r >= c, sits at AB[c * 2bw + r - c]: band column c is
+// 2bw contiguous entries.  A step loads its band columns R_k whole (one
+// coalesced load per column, lane = entry) into registers, finishes the
+// previous step's columns (its X = the previous Y lives there) in the wave's
+// LDS buffer and stores them whole, then moves the new columns into the
+// buffer and updates D and Y there.  So every band column is loaded and
+// stored once per sweep, with whole-line accesses; the published progress
+// counts the steps whose columns are stored.
 template <typename T, int BW>
-struct HbLane {
-  static constexpr int NG = WAVE / BW;     // column groups
-  static constexpr int CPL = BW / NG;      // columns per lane
-};
-
-template <typename T, int BW>
-__device__ __forceinline__ T sum_rows(T v) {        // over the bw lanes of a column group
-#pragma unroll
-  for (int o = 1; o < BW; o <<= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
-}
-template <typename T, int BW>
-__device__ __forceinline__ T sum_groups(T v) {      // over the column groups of a row
+__device__ __forceinline__ T sum_halves(T v) {      // over the lanes of one column / row
 #pragma unroll
   for (int o = BW; o < WAVE; o <<= 1) v += __shfl_xor(v, o, WAVE);
   return v;
@@ -402,9 +379,12 @@ __device__ __forceinline__ T sum_groups(T v) {      // over the column groups of
 
 template <typename T, int BW, int NW>
 __global__ void __launch_bounds__(WAVE * NW) k_hb2st(TwoStageWork<T> w) {
-  constexpr int CPL = HbLane<T, BW>::CPL;
+  constexpr int NG = WAVE / BW;
+  constexpr int RPL = BW / NG;             // rows (column layout) or columns (row layout) per lane
   constexpr int LDB = 2 * BW;
-  constexpr int SH = 4096;                 // progress = sweep * SH + steps done
+  constexpr int CPI = WAVE / LDB;          // band columns per load instruction (1 or 2)
+  constexpr int NLD = BW / CPI;            // load instructions per step
+  constexpr int SH = 4096;                 // progress = sweep * SH + steps stored
   const int64_t b = blockIdx.x;
   const int64_t n = w.n;
   cplx<T>* AB = w.AB + b * w.strideAB;
@@ -412,160 +392,216 @@ __global__ void __launch_bounds__(WAVE * NW) k_hb2st(TwoStageWork<T> w) {
   cplx<T>* tau2 = w.tau2 + b * w.strideTau2;
   const int64_t kmax = w.kmax;
   const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-  const int r = lane % BW, c0 = (lane / BW) * CPL;
+  const int cl = lane % BW, hl = lane / BW, q0 = hl * RPL;
+  const int dl = lane % LDB, il = lane / LDB;          // band entry, column offset of a load
   __shared__ int prog[NW];
+  // band columns of the step, [column][entry]; a pitch of 2bw + 2 makes both the
+  // walk along a column (lane = entry) and the walk along a diagonal (lane =
+  // column, entry - column fixed) free of LDS bank conflicts
+  __shared__ cplx<T> buf[NW][BW][LDB + 2];
   __shared__ cplx<T> sv[NW][BW], sw[NW][BW];
   __shared__ int stalled;
   if (threadIdx.x < NW) prog[threadIdx.x] = -1;
   if (threadIdx.x == 0) stalled = 0;
   __syncthreads();
+  cplx<T>(*B)[LDB + 2] = buf[wave];
+  auto wave_sync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  auto publish = [&](int64_t j, int64_t done) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0)
+      __hip_atomic_store(&prog[wave], (int)(j * SH + done), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  // store the buffer's first ncol columns as band columns c0.. (whole columns)
+  auto store_cols = [&](int c0, int ncol) {
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+      const int i = q * CPI + il;
+      if (i < ncol) AB[(c0 + i) * LDB + dl] = B[i][dl];
+    }
+  };
   const int pw = (wave + NW - 1) % NW;
   for (int64_t j = wave; j < n - 1; j += NW) {
     const int64_t K = hb2st_tasks(n, BW, j);
     const int64_t Kp = (j > 0) ? hb2st_tasks(n, BW, j - 1) : 0;
-    cplx<T> X[CPL];
-#pragma unroll
-    for (int i = 0; i < CPL; ++i) X[i] = cplx<T>{0, 0};
+    const int ji = (int)j;
     for (int64_t k = 0; k < K; ++k) {
       if (j > 0) {
-        // sweep j - 1 must have finished its steps 0 .. k + 2.  The wait is
-        // bounded (about 0.2 s, far beyond any step): a stall is flagged in
-        // info and every later wait of the workgroup is skipped, so the
-        // kernel always ends
+        // sweep j - 1 must have stored the columns of its steps 0 .. k + 2.
+        // The wait is bounded (about 0.4 s, far beyond any step): a stall is
+        // flagged in info and every later wait of the workgroup is skipped,
+        // so the kernel always ends
         const int need = (int)((j - 1) * SH + min<int64_t>(k + 3, Kp));
         int spins = 0;
         while (__hip_atomic_load(&prog[pw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need &&
                __hip_atomic_load(&stalled, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins > (1 << 22)) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > (1 << 23)) {
             if (lane == 0)
               __hip_atomic_store(&stalled, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
           }
         }
       }
-      const int64_t s = j + 1 + k * BW;
-      const int L = (int)min<int64_t>(BW, n - s);
-      const int64_t sp = (k == 0) ? j : s - BW;      // first column of X
-      if (k == 0) {
-        if (c0 == 0 && r < L) X[0] = AB[j * LDB + (s + r - j)];
+      const int si = (int)(j + 1 + k * BW);             // 32-bit offsets into the band
+      const int L = (int)min<int64_t>(BW, n - si);
+      const bool below = k + 1 < K;
+      const int L2 = below ? (int)min<int64_t>(BW, n - si - BW) : 0;
+      // this step's band columns si .. si + L - 1, whole
+      // FP32: loaded into registers now, so the loads overlap the left update;
+      // FP64 (twice the registers): loaded after the previous columns are stored
+      constexpr bool early = sizeof(T) == 4;
+      cplx<T> R[NLD];
+      if (early) {
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+          const int i = q * CPI + il;
+          R[q] = AB[(si + min(i, L - 1)) * LDB + dl];  // columns past the matrix: never used
+        }
       }
-      // reflector from the first column of X (lanes of column group 0, slot 0)
-      double xr = 0.0, xi = 0.0, nn = 0.0;
-      if (c0 == 0 && r < L) {
-        xr = (double)X[0].x;
-        xi = (double)X[0].y;
-        if (r >= 1) nn = xr * xr + xi * xi;
+      if (k == 0) {
+        // X = column j: band entries 1 .. L of column j, into buffer column 0
+        if (lane < LDB) B[0][lane] = AB[ji * LDB + lane];
+        wave_sync();
+      }
+      // X(r, i) = B[i][xo + r - i]: the previous step's Y (xo = bw), or column j (xo = 1)
+      const int xo = (k == 0) ? 1 : BW;
+      const int xcols = (k == 0) ? 1 : BW;
+      // reflector from column 0 of X
+      double nn = 0.0;
+      {
+        const int r = lane;
+        if (r >= 1 && r < L) {
+          const cplx<T> x = B[0][xo + r];
+          nn = (double)x.x * x.x + (double)x.y * x.y;
+        }
       }
       nn = wave_sum(nn);
-      const double ar = __shfl(xr, 0, WAVE), ai = __shfl(xi, 0, WAVE);
-      const Reflector rf = make_reflector(nn, ar, ai);
-      const double xrr = __shfl(xr, r, WAVE), xir = __shfl(xi, r, WAVE);
-      cplx<T> v{0, 0};
-      if (r == 0) v = cplx<T>{1, 0};
-      else if (r < L) v = cplx<T>{(T)(xrr * rf.sr - xir * rf.si), (T)(xrr * rf.si + xir * rf.sr)};
+      const cplx<T> x0 = B[0][xo];
+      const Reflector rf = make_reflector(nn, (double)x0.x, (double)x0.y);
       const cplx<T> tau{(T)rf.tr, (T)rf.ti};
-      if (c0 == 0) sv[wave][r] = v;
+      if (lane < BW) {
+        const int r = lane;
+        cplx<T> v{0, 0};
+        if (r == 0) v = cplx<T>{1, 0};
+        else if (r < L) {
+          const cplx<T> x = B[0][xo + r];
+          v = cplx<T>{(T)(x.x * rf.sr - x.y * rf.si), (T)(x.x * rf.si + x.y * rf.sr)};
+        }
+        sv[wave][r] = v;
+      }
+      wave_sync();
+      cplx<T> vr[RPL];                                   // v at the lane's rows
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) vr[i] = sv[wave][q0 + i];
+      const cplx<T> vcl = sv[wave][cl];                   // v at the lane's column
+      // X <- H^H X on columns 1 ..: z_c = sum_r conj(v_r) X(r, c),
+      // X(r, c) -= conj(tau) v_r z_c (column layout, in the buffer)
       if (k >= 1) {
-        // X <- H^H X on columns 1 ..: z_c = sum_r conj(v_r) X(r, c)
-        const cplx<T> ctau{tau.x, -tau.y};
+        // v_r = 0 for r >= L, so rows past the matrix neither contribute nor change;
+        // column 0 is overwritten below
+        cplx<T> xs[RPL];
 #pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-          T zr = v.x * X[i].x + v.y * X[i].y;
-          T zi = v.x * X[i].y - v.y * X[i].x;
-          zr = sum_rows<T, BW>(zr);
-          zi = sum_rows<T, BW>(zi);
-          if (c0 + i >= 1) X[i] = csub(X[i], cmul(cmul(ctau, v), cplx<T>{zr, zi}));
-        }
+        for (int i = 0; i < RPL; ++i) xs[i] = B[cl][BW + q0 + i - cl];
+        cplx<T> z{0, 0};
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) z = cadd(z, cmulc(vr[i], xs[i]));
+        z.x = sum_halves<T, BW>(z.x);
+        z.y = sum_halves<T, BW>(z.y);
+        const cplx<T> f = cmul(cplx<T>{tau.x, -tau.y}, z);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) B[cl][BW + q0 + i - cl] = csub(xs[i], cmul(f, vr[i]));
       }
-      if (c0 == 0) X[0] = (r == 0) ? cplx<T>{(T)rf.beta, 0} : cplx<T>{0, 0};
+      // column 0 of X becomes (beta, 0, ...)
+      if (lane < BW && lane < L) B[0][xo + lane] = (lane == 0) ? cplx<T>{(T)rf.beta, 0} : cplx<T>{0, 0};
+      wave_sync();
+      // the previous step's columns (or column j) are final: store them whole
       if (k == 0) {
-        if (c0 == 0 && r < L) AB[j * LDB + (s + r - j)] = X[0];
-      } else if (r < L) {
+        if (lane < LDB) AB[ji * LDB + lane] = B[0][lane];
+      } else {
+        store_cols(si - BW, xcols);
+      }
+      wave_sync();
+      // this step's columns into the buffer
+      if (!early) {
 #pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-          const int64_t col = sp + c0 + i;
-          AB[col * LDB + (s + r - col)] = X[i];
+        for (int q = 0; q < NLD; ++q) {
+          const int i = q * CPI + il;
+          R[q] = AB[(si + min(i, L - 1)) * LDB + dl];
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      cplx<T> vc[CPL];
 #pragma unroll
-      for (int i = 0; i < CPL; ++i) vc[i] = (c0 + i < L) ? sv[wave][c0 + i] : cplx<T>{0, 0};
-      // D <- H^H D H on the diagonal block (Hermitian; lower triangle stored)
-      cplx<T> D[CPL];
+      for (int q = 0; q < NLD; ++q) B[q * CPI + il][dl] = R[q];
+      wave_sync();
+      // D <- H^H D H (column layout): y_c = sum_r conj(D(r, c)) v_r, w = tau y -
+      // |tau|^2 (v^H y) v / 2, D -= w v^H + v w^H (lower triangle kept)
+      cplx<T> D[RPL];
 #pragma unroll
-      for (int i = 0; i < CPL; ++i) {
-        const int c = c0 + i;
-        cplx<T> d{0, 0};
-        if (r < L && c < L) {
-          if (r >= c) d = AB[(s + c) * LDB + (r - c)];
-          else {
-            const cplx<T> u = AB[(s + r) * LDB + (c - r)];
-            d = cplx<T>{u.x, -u.y};
-          }
-        }
-        D[i] = d;
+      for (int i = 0; i < RPL; ++i) {
+        const int r = q0 + i, c = cl;
+        const bool lo = r >= c;
+        const cplx<T> u = lo ? B[c][r - c] : B[r][c - r];
+        const bool in = r < L && c < L;
+        D[i] = cplx<T>{in ? u.x : T(0), in ? (lo ? u.y : -u.y) : T(0)};
       }
       cplx<T> y{0, 0};
 #pragma unroll
-      for (int i = 0; i < CPL; ++i) y = cadd(y, cmul(D[i], vc[i]));
-      y.x = sum_groups<T, BW>(y.x);
-      y.y = sum_groups<T, BW>(y.y);
-      double h = (c0 == 0 && r < L) ? (double)v.x * y.x + (double)v.y * y.y : 0.0;
-      h = wave_sum(h);                                // v^H D v (real)
+      for (int i = 0; i < RPL; ++i) y = cadd(y, cmulc(D[i], vr[i]));
+      y.x = sum_halves<T, BW>(y.x);
+      y.y = sum_halves<T, BW>(y.y);
+      double hh = (hl == 0 && cl < L) ? (double)vcl.x * y.x + (double)vcl.y * y.y : 0.0;
+      hh = wave_sum(hh);                                  // v^H D v (real)
       const double t2 = rf.tr * rf.tr + rf.ti * rf.ti;
-      const cplx<T> ty = cmul(tau, y);
-      const T hh = (T)(0.5 * t2 * h);
-      const cplx<T> wr{ty.x - hh * v.x, ty.y - hh * v.y};
-      if (c0 == 0) sw[wave][r] = wr;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const cplx<T> tyv = cmul(tau, y);
+      const T hs = (T)(0.5 * t2 * hh);
+      const cplx<T> wc{tyv.x - hs * vcl.x, tyv.y - hs * vcl.y};
+      if (hl == 0) sw[wave][cl] = wc;
+      // Y <- Y H (row layout: t_r = sum_c Y(r, c) v_c), Y(r, c) = B[c][bw + r - c]
+      // (v_c = 0 for c >= L; rows yr >= L2 are past the matrix and never stored)
+      cplx<T> t{0, 0};
+      cplx<T> ys[RPL];
 #pragma unroll
-      for (int i = 0; i < CPL; ++i) {
-        const int c = c0 + i;
-        if (r < L && c < L && r >= c) {
-          const cplx<T> wc = sw[wave][c];
-          // D -= w v^H + v w^H
-          cplx<T> d = csub(D[i], cmulc(vc[i], wr));            // w_r conj(v_c)
-          d = csub(d, cmulc(wc, v));                           // v_r conj(w_c)
-          if (r == c) d.y = T(0);
-          AB[(s + c) * LDB + (r - c)] = d;
-        }
+      for (int i = 0; i < RPL; ++i) ys[i] = B[q0 + i][BW + cl - q0 - i];
+      if (below) {
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) t = cadd(t, cmul(ys[i], vr[i]));
+        t.x = sum_halves<T, BW>(t.x);
+        t.y = sum_halves<T, BW>(t.y);
       }
-      // Y <- Y H on the block below, carried as the next step's X
-      if (k + 1 < K) {
-        const int64_t s2 = s + BW;
-        const int L2 = (int)min<int64_t>(BW, n - s2);
-        cplx<T> t{0, 0};
+      wave_sync();
+      cplx<T> wrs[RPL];
 #pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-          const int c = c0 + i;
-          cplx<T> yv{0, 0};
-          if (r < L2 && c < L) yv = AB[(s + c) * LDB + (s2 + r - s - c)];
-          X[i] = yv;
-          t = cadd(t, cmul(yv, vc[i]));
-        }
-        t.x = sum_groups<T, BW>(t.x);
-        t.y = sum_groups<T, BW>(t.y);
+      for (int i = 0; i < RPL; ++i) wrs[i] = sw[wave][q0 + i];
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int r = q0 + i, c = cl;
+        cplx<T> d = csub(D[i], cmulc(vcl, wrs[i]));        // w_r conj(v_c)
+        d = csub(d, cmulc(wc, vr[i]));                     // v_r conj(w_c)
+        if (r == c) d.y = T(0);
+        if (r < L && c < L && r >= c) B[c][r - c] = d;
+      }
+      if (below) {
         const cplx<T> tt = cmul(tau, t);
 #pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-          const int c = c0 + i;
-          X[i] = csub(X[i], cmulc(vc[i], tt));                   // tau t_r conj(v_c)
-          if (r < L2 && c < L) AB[(s + c) * LDB + (s2 + r - s - c)] = X[i];
-        }
+        for (int i = 0; i < RPL; ++i)
+          B[q0 + i][BW + cl - q0 - i] = csub(ys[i], cmulc(vr[i], tt));   // tau t_r conj(v_c)
       }
       // the reflector for the back-transformation
-      if (c0 == 0) V2[(j * kmax + k) * BW + r] = v;
+      if (hl == 0) V2[(j * kmax + k) * BW + cl] = sv[wave][cl];
       if (lane == 0) tau2[j * kmax + k] = tau;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      if (lane == 0)
-        __hip_atomic_store(&prog[wave], (int)(j * SH + k + 1), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      wave_sync();
+      if (!below) {
+        // the sweep's last columns are final too
+        store_cols(si, L);
+        publish(j, K);
+      } else {
+        publish(j, k);                                     // steps 0 .. k - 1 stored
+      }
+      wave_sync();
     }
   }
   __syncthreads();
@@ -584,6 +620,7 @@ void hb2st(const TwoStageWork<T>& w, hipStream_t s) {
   const int64_t n = w.n, B = w.batch;
   if (n < 2) return;
   if (hb2st_tasks(n, w.bw, 0) >= 4096) throw std::runtime_error("hb2st: n too large for the band width");
+  if (n * 2 * w.bw >= (int64_t(1) << 31)) throw std::runtime_error("hb2st: band too large for 32-bit offsets");
   constexpr int NW = sizeof(T) == 4 ? 8 : 4;      // the register budget of a wave: 256 or 512 VGPRs
   if (w.bw == 32)
     hipLaunchKernelGGL((k_hb2st<T, 32, NW>), dim3((unsigned)B), dim3(WAVE * NW), 0, s, w);
@@ -626,7 +663,8 @@ __device__ __forceinline__ void gather_v(const cplx<T>* V2, const cplx<T>* tau2,
   }
 }
 
-// VT = V T per block (column-major 2bw x bw at VTb[(g kmax + k) 2bw^2]).
+// Per block: V row-major (Vb[rr * bw + i]) and VT = V T column-major
+// (VTb[rr + i * 2bw]), at block index (g kmax + k) * 2bw^2 of each.
 // One workgroup per (k, g, matrix).
 template <typename T, int BW>
 __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
@@ -639,6 +677,9 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
   const int tid = threadIdx.x;
   gather_v<T, BW>(w.V2 + b * w.strideV2, w.tau2 + b * w.strideTau2, w.kmax, n, j0, k, Vs, ts);
   __syncthreads();
+  const int64_t blk = (g * w.kmax + k) * (int64_t)(2 * BW * BW);
+  cplx<T>* vout = w.Vb + b * w.strideVTb + blk;
+  for (int e = tid; e < 2 * BW * BW; e += 256) vout[e] = Vs[e / BW][e % BW];
   // G = V^H V (column i nonzero on rows i .. i + bw - 1)
   for (int e = tid; e < BW * BW; e += 256) {
     const int a = e % BW, c = e / BW;
@@ -666,7 +707,7 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
     __syncthreads();
   }
   // VT(rr, c) = sum_{l <= c} V(rr, l) T(l, c)
-  cplx<T>* out = w.VTb + b * w.strideVTb + (g * w.kmax + k) * (int64_t)(2 * BW * BW);
+  cplx<T>* out = w.VTb + b * w.strideVTb + blk;
   for (int e = tid; e < 2 * BW * BW; e += 256) {
     const int rr = e % (2 * BW), c = e / (2 * BW);
     double sr = 0.0, si = 0.0;
@@ -680,117 +721,158 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
   }
 }
 
-// Z <- Q2 Z for a strip of SW columns of one matrix per workgroup; the window
-// of 2bw rows the current block acts on lives in LDS as two halves of bw rows
-// (physical half (k + h) & 1 holds logical half h), so consecutive blocks of
-// a group share one half and each row of the strip is loaded and stored once
-// per group.
-template <typename T, int BW, int SW>
-__global__ void __launch_bounds__(256) k_q2_apply(Q2Work<T> w) {
-  constexpr int NT = 256;
-  constexpr int IW = BW * SW / NT;          // reflectors per thread in W = V^H Zw
-  constexpr int RW = 2 * BW * SW / NT;      // rows per thread in Zw -= VT W
-  static_assert(NT % SW == 0 && IW >= 1 && RW >= 1, "strip shape");
-  __shared__ cplx<T> Zw[2][BW][SW + 1];
-  __shared__ cplx<T> Vs[2 * BW][BW + 1];
-  __shared__ cplx<T> VTs[BW][2 * BW + 1];   // VTs[i][rr] = VT(rr, i)
-  __shared__ cplx<T> Ws[BW][SW + 1];
-  __shared__ cplx<T> ts[BW];
+// One 16 x 16 complex tile of C = A B on the matrix cores (3M: three real
+// MFMAs per K step of 4), A(m, k) and B(k, n) read from LDS through the
+// accessors; K steps [k0, k1).  Lane l holds column l % 16 of the tile, rows
+// mfma<T>::row(l, r), r < 4.
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void tile3m(FA&& A, FB&& B, int k0, int k1, cplx<T> out[4]) {
+  using acc_t = typename mfma<T>::acc;
+  const int lane = threadIdx.x % WAVE;
+  acc_t p1{0, 0, 0, 0}, p2{0, 0, 0, 0}, p3{0, 0, 0, 0};
+  for (int k = k0; k < k1; k += 4) {
+    const cplx<T> a = A(lane % 16, k + lane / 16);
+    const cplx<T> b = B(k + lane / 16, lane % 16);
+    p1 = mfma<T>::op(a.x, b.x, p1);
+    p2 = mfma<T>::op(a.y, b.y, p2);
+    p3 = mfma<T>::op(a.x + a.y, b.x + b.y, p3);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = cplx<T>{p1[r] - p2[r], p3[r] - p1[r] - p2[r]};
+}
+
+// Z <- Q2 Z for a strip of 16 NWV columns of one matrix per workgroup; wave w
+// owns columns 16 w .. 16 w + 15 of the strip.  The window of 2bw rows the
+// current block acts on lives in LDS as two halves of bw rows (slot (k + h) & 1
+// holds logical half h), so consecutive blocks of a group share a half and
+// each row of the strip is loaded and stored once per group.  Per block, on
+// the matrix cores and for the wave's own columns only (no cross-wave
+// dependency):
+//   W = V^H Zw   (bw x 16, K = 2bw; reflectors i0.. meet rows i0 .. i0 + bw + 15)
+//   Zw -= VT W   (2bw x 16, K = bw; VT(rr, i) = 0 for i <= rr - bw)
+// V and VT of a block are shared by the waves: two barriers per block.  The
+// next block's V and VT and the wave's next half of Z are loaded into
+// registers while the current block computes.
+template <typename T, int BW, int NWV>
+__global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
+  constexpr int NT = WAVE * NWV;
+  constexpr int SW = 16 * NWV;
+  constexpr int VE = (2 * BW * BW + NT - 1) / NT;   // V (and VT) elements per thread
+  constexpr int ZE = BW * 16 / WAVE;                // elements of the wave's Z half per lane
+  // rows padded by 128 bytes: the four rows one MFMA operand load spans then
+  // fall in alternate halves of the LDS banks
+  constexpr int PC = 128 / (int)sizeof(cplx<T>);
+  __shared__ cplx<T> Zw[2][BW][SW + PC];
+  __shared__ cplx<T> Vs[2 * BW][BW + PC];           // V(rr, i)
+  __shared__ cplx<T> VTs[BW][2 * BW + PC];          // VT(rr, i) at VTs[i][rr]
+  __shared__ cplx<T> Ws[BW][SW + PC];
   const int64_t n = w.n, b = blockIdx.y;
-  const int64_t col0 = (int64_t)blockIdx.x * SW;
-  const int tid = threadIdx.x;
-  const int c = tid % SW;
-  const int grp = tid / SW;
+  const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+  const int cw = 16 * wave;                         // the wave's first column in the strip
+  const int64_t col0 = (int64_t)blockIdx.x * SW + cw;
   cplx<T>* Z = w.Z + b * w.strideZ;
-  const cplx<T>* V2 = w.V2 + b * w.strideV2;
-  const cplx<T>* tau2 = w.tau2 + b * w.strideTau2;
+  const cplx<T>* Vb = w.Vb + b * w.strideVTb;
   const cplx<T>* VTb = w.VTb + b * w.strideVTb;
   const int64_t G = ceil_div_dev(n - 1, BW);
-  // half h of the window at block k: rows j0 + 1 + (k + h) bw .. , slot (k + h) & 1
-  auto load_half = [&](int64_t row0, int slot) {
-    for (int e = tid; e < BW * SW; e += NT) {
-      const int rr = e % BW, cc = e / BW;
-      const int64_t row = row0 + rr, col = col0 + cc;
-      Zw[slot][rr][cc] = (row < n && col < w.m) ? Z[row + col * w.ldz] : cplx<T>{0, 0};
+  cplx<T> pv[VE], pvt[VE], pz[ZE];
+  auto fetch_v = [&](int64_t g, int64_t k) {
+    const int64_t blk = (g * w.kmax + k) * (int64_t)(2 * BW * BW);
+#pragma unroll
+    for (int q = 0; q < VE; ++q) {
+      const int e = tid + NT * q;
+      const bool ok = e < 2 * BW * BW;
+      pv[q] = ok ? Vb[blk + e] : cplx<T>{0, 0};
+      pvt[q] = ok ? VTb[blk + e] : cplx<T>{0, 0};
     }
   };
-  auto store_half = [&](int64_t row0, int slot) {
-    for (int e = tid; e < BW * SW; e += NT) {
-      const int rr = e % BW, cc = e / BW;
-      const int64_t row = row0 + rr, col = col0 + cc;
-      if (row < n && col < w.m) Z[row + col * w.ldz] = Zw[slot][rr][cc];
+  auto put_v = [&]() {
+#pragma unroll
+    for (int q = 0; q < VE; ++q) {
+      const int e = tid + NT * q;
+      if (e < 2 * BW * BW) {
+        Vs[e / BW][e % BW] = pv[q];
+        VTs[e / (2 * BW)][e % (2 * BW)] = pvt[q];
+      }
     }
+  };
+  // the wave's 16 columns of a half: lane -> row lane % bw, columns lane / bw + (64 / bw) q
+  auto fetch_z = [&](int64_t row0) {
+#pragma unroll
+    for (int q = 0; q < ZE; ++q) {
+      const int rr = lane % BW, cc = lane / BW + (WAVE / BW) * q;
+      const int64_t row = row0 + rr, col = col0 + cc;
+      pz[q] = (row < n && col < w.m) ? Z[row + col * w.ldz] : cplx<T>{0, 0};
+    }
+  };
+  auto put_z = [&](int slot) {
+#pragma unroll
+    for (int q = 0; q < ZE; ++q) Zw[slot][lane % BW][cw + lane / BW + (WAVE / BW) * q] = pz[q];
+  };
+  auto store_z = [&](int64_t row0, int slot) {
+#pragma unroll
+    for (int q = 0; q < ZE; ++q) {
+      const int rr = lane % BW, cc = lane / BW + (WAVE / BW) * q;
+      const int64_t row = row0 + rr, col = col0 + cc;
+      if (row < n && col < w.m) Z[row + col * w.ldz] = Zw[slot][rr][cw + cc];
+    }
+  };
+  auto wave_sync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
   for (int64_t g = G - 1; g >= 0; --g) {
     const int64_t j0 = g * BW;
     const int64_t Kg = hb2st_tasks(n, BW, j0);
-    __syncthreads();
-    load_half(j0 + 1, 0);
-    load_half(j0 + 1 + BW, 1);
+    fetch_z(j0 + 1);
+    put_z(0);
+    fetch_z(j0 + 1 + BW);
+    put_z(1);
+    fetch_v(g, 0);
     for (int64_t k = 0; k < Kg; ++k) {
       const int64_t row0 = j0 + 1 + k * BW;
-      gather_v<T, BW>(V2, tau2, w.kmax, n, j0, k, Vs, ts);
-      const cplx<T>* vt = VTb + (g * w.kmax + k) * (int64_t)(2 * BW * BW);
-      for (int e = tid; e < 2 * BW * BW; e += NT) {
-        const int rr = e % (2 * BW), i = e / (2 * BW);
-        VTs[i][rr] = vt[rr + i * (2 * BW)];
-      }
+      const int s0 = (int)(k & 1);                   // slot of logical half 0
+      __syncthreads();                               // every wave is done with the previous V, VT
+      put_v();
       __syncthreads();
-      const int s0 = (int)(k & 1);         // slot of logical half 0
-      // W(i, c) = sum_rr conj(V(rr, i)) Zw(rr, c), i = grp IW .. + IW
-      {
-        const int i0 = grp * IW;
-        cplx<T> acc[IW];
-#pragma unroll
-        for (int q = 0; q < IW; ++q) acc[q] = cplx<T>{0, 0};
-        const int rlo = i0, rhi = min(2 * BW, i0 + IW - 1 + BW);
-        for (int rr = rlo; rr < rhi; ++rr) {
-          const cplx<T> z = Zw[(s0 + rr / BW) & 1][rr % BW][c];
-#pragma unroll
-          for (int q = 0; q < IW; ++q) {
-            const cplx<T> v = Vs[rr][i0 + q];
-            acc[q].x += v.x * z.x + v.y * z.y;
-            acc[q].y += v.x * z.y - v.y * z.x;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < IW; ++q) Ws[i0 + q][c] = acc[q];
-      }
-      __syncthreads();
-      // Zw(rr, c) -= sum_i VT(rr, i) W(i, c), rows grp RW .. + RW; VT(rr, i) = 0
-      // for rr >= i + bw
-      {
-        const int rr0 = grp * RW;
-        cplx<T> acc[RW];
-#pragma unroll
-        for (int q = 0; q < RW; ++q) acc[q] = cplx<T>{0, 0};
-        for (int i = max(0, rr0 - BW + 1); i < BW; ++i) {
-          const cplx<T> wv = Ws[i][c];
-#pragma unroll
-          for (int q = 0; q < RW; ++q) {
-            const cplx<T> a = VTs[i][rr0 + q];
-            acc[q].x += a.x * wv.x - a.y * wv.y;
-            acc[q].y += a.x * wv.y + a.y * wv.x;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < RW; ++q) {
-          const int rr = rr0 + q;
-          cplx<T>& zz = Zw[(s0 + rr / BW) & 1][rr % BW][c];
-          zz = csub(zz, acc[q]);
-        }
-      }
-      __syncthreads();
-      // logical half 0 is finished for this group: store it, and bring in the
-      // half after the window into its slot
-      store_half(row0, s0);
       if (k + 1 < Kg) {
-        __syncthreads();
-        load_half(row0 + 2 * BW, s0);
-      } else {
-        store_half(row0 + BW, s0 ^ 1);
+        fetch_v(g, k + 1);
+        fetch_z(row0 + 2 * BW);
       }
-      __syncthreads();
+      // W = V^H Zw on the wave's columns
+#pragma unroll
+      for (int rt = 0; rt < BW / 16; ++rt) {
+        const int i0 = 16 * rt;
+        cplx<T> o[4];
+        tile3m<T>([&](int m, int kk) { const cplx<T> v = Vs[kk][i0 + m]; return cplx<T>{v.x, -v.y}; },
+                  [&](int kk, int c) { return Zw[(s0 + kk / BW) & 1][kk % BW][cw + c]; },
+                  i0, min(2 * BW, i0 + 16 + BW), o);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ws[i0 + mfma<T>::row(lane, r)][cw + lane % 16] = o[r];
+      }
+      wave_sync();
+      // Zw -= VT W on the wave's columns
+#pragma unroll
+      for (int rt = 0; rt < 2 * BW / 16; ++rt) {
+        const int r0 = 16 * rt;
+        const int kb = max(0, r0 - BW + 1) & ~3;
+        cplx<T> o[4];
+        tile3m<T>([&](int m, int kk) { return VTs[kk][r0 + m]; },
+                  [&](int kk, int c) { return Ws[kk][cw + c]; }, kb, BW, o);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = r0 + mfma<T>::row(lane, r);
+          cplx<T>& zz = Zw[(s0 + rr / BW) & 1][rr % BW][cw + lane % 16];
+          zz = csub(zz, o[r]);
+        }
+      }
+      wave_sync();
+      // logical half 0 is finished for this group: store it; the half after
+      // the window takes its slot
+      store_z(row0, s0);
+      if (k + 1 < Kg) put_z(s0);
+      else store_z(row0 + BW, s0 ^ 1);
+      wave_sync();
     }
   }
 }
@@ -800,20 +882,22 @@ void apply_q2(const Q2Work<T>& w, hipStream_t s) {
   const int64_t n = w.n, B = w.batch;
   if (n < 2 || w.m <= 0) return;
   const int64_t G = ceil_div(n - 1, w.bw);
+  // columns per workgroup: 16 per wave, 8 waves for FP32 and 2 for FP64 (LDS)
+  constexpr int NWV = sizeof(T) == 4 ? 8 : 2;
+  const unsigned strips = (unsigned)ceil_div(w.m, 16 * NWV);
   if (w.bw == 32) {
     hipLaunchKernelGGL((k_q2_prep<T, 32>), dim3((unsigned)w.kmax, (unsigned)G, (unsigned)B),
                        dim3(256), 0, s, w);
     PPH_CHECK_LAUNCH();
-    constexpr int SW = sizeof(T) == 4 ? 64 : 32;
-    hipLaunchKernelGGL((k_q2_apply<T, 32, SW>), dim3((unsigned)ceil_div(w.m, SW), (unsigned)B),
-                       dim3(256), 0, s, w);
+    hipLaunchKernelGGL((k_q2_apply<T, 32, NWV>), dim3(strips, (unsigned)B), dim3(WAVE * NWV), 0, s,
+                       w);
     PPH_CHECK_LAUNCH();
   } else if (w.bw == 16) {
     hipLaunchKernelGGL((k_q2_prep<T, 16>), dim3((unsigned)w.kmax, (unsigned)G, (unsigned)B),
                        dim3(256), 0, s, w);
     PPH_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_q2_apply<T, 16, 64>), dim3((unsigned)ceil_div(w.m, 64), (unsigned)B),
-                       dim3(256), 0, s, w);
+    hipLaunchKernelGGL((k_q2_apply<T, 16, NWV>), dim3(strips, (unsigned)B), dim3(WAVE * NWV), 0, s,
+                       w);
     PPH_CHECK_LAUNCH();
   } else {
     throw std::runtime_error("apply_q2: band width must be 16 or 32");

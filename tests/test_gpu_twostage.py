@@ -132,3 +132,20 @@ def test_run_tbg_3_4_two_stage(two_stage, prec):
             for g in groups:
                 a, b = res["vel"][iq][g].sum(axis=0), ref["vel"][g].sum(axis=0)
                 assert np.all(np.abs(a - b) <= 1e-5 * vmax * len(g) + noise[g].sum())
+
+
+@pytest.mark.parametrize("prec,tol", [("D", 1e-12), ("S", 2e-5)])
+@pytest.mark.parametrize("n,nx", [(200, 32), (333, 17), (64, 5), (130, 96)])
+def test_gemm_hermitian_from_lower(gpu_available, prec, tol, n, nx):
+    """C = A X for Hermitian A from its lower triangle alone (the lower
+    product with the diagonal plus the transposed strict lower product); the
+    upper triangle holds garbage that must not be read."""
+    lib = _lib()
+    rng = np.random.default_rng(11)
+    H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = H + H.conj().T
+    X = rng.standard_normal((n, nx)) + 1j * rng.standard_normal((n, nx))
+    ref = H @ X
+    A = np.tril(H) + np.triu(np.full((n, n), 3e3 - 5e2j), 1)
+    C, _ = lib.debug_zgemm(A, X, prec=prec, mode=5)
+    assert np.abs(C - ref).max() <= tol * np.abs(ref).max() * 10
