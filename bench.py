@@ -48,10 +48,11 @@ def device_outputs(torch, B, n):
 def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1):
     """Each step is one pph_run_device call over streams x B q-points: the
     library keeps `streams` batches of B in flight (one stream and host thread
-    each).  Returns the timed seconds, the phase profile of the timed steps
-    (concurrent batches), the profile of one batch run alone, and the
-    frequencies of the first B q-points, and the number of force-constant
-    blocks the Rayleigh quotients and velocities use (0: dense products)."""
+    each).  Returns a dict: the timed seconds, the phase profile of the timed
+    steps (concurrent batches) and of one batch run alone, the frequencies and
+    velocities of the first B q-points, the eigenvectors of the first q, the
+    number of force-constant blocks the Rayleigh quotients and velocities use
+    (0: dense products), and whether the two-stage reduction ran."""
     n = 3 * st.natom
     h = lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision=prec,
                    batch=B, streams=streams)
@@ -80,12 +81,34 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
     # one batch alone (untimed; also the accuracy check): the first B q-points
     h.run_device(qs[:B], out)
     prof_iso = h.profile(enable=False)
-    ev = out["eval_cm"][:B].cpu().numpy().copy()
-    z0 = out["evec"][0].cpu().numpy().T.copy()        # eigenvectors of the first q, columns
+    res = {"dt": dt, "prof": prof, "iso": prof_iso, "nblk": nblk,
+           "ev": out["eval_cm"][:B].cpu().numpy().copy(),
+           "vel": out["vel"][:B].cpu().numpy().copy(),
+           "z0": out["evec"][0].cpu().numpy().T.copy(),   # eigenvectors of the first q, columns
+           "two_stage": prof_iso["q2"][1] > 0}
     h.close()
     del out
     torch.cuda.empty_cache()
-    return dt, prof, prof_iso, ev, z0, nblk
+    return res
+
+
+def velocity_difference(v, v_ref, om_ref, rel=1e-6):
+    """Largest difference of the velocities of two runs on the same q, through
+    sums over clusters of eigenvalues closer than rel * max|omega^2| (inside a
+    degenerate cluster the individual velocities depend on the basis; their
+    sum does not).  Returns (max abs difference m/s, same / max |v_ref|)."""
+    lam = np.sign(om_ref) * om_ref ** 2
+    tol = rel * np.abs(lam).max()
+    groups, cur = [], [0]
+    for k in range(1, len(lam)):
+        if lam[k] - lam[k - 1] <= tol:
+            cur.append(k)
+        else:
+            groups.append(cur)
+            cur = [k]
+    groups.append(cur)
+    d = max(float(np.abs(v[g].sum(axis=0) - v_ref[g].sum(axis=0)).max()) for g in groups)
+    return d, d / float(np.abs(v_ref).max())
 
 
 def split_profile(prof, n):
@@ -99,11 +122,11 @@ def split_profile(prof, n):
     return phases, launch_ms
 
 
-def hemv_traffic(workload, n, B):
-    """HBM bytes per k_hemv_sym launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of the same workload and batch (counters scaled by a
-    1 GiB copy), or None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", f"k_hemv_sym_{workload}_b{B}_traffic.md")
+def kernel_traffic(kernel, workload, n, B):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the same workload and batch (counters
+    scaled by a device copy of known size), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_b{B}_traffic.md")
     if not os.path.exists(path):
         return None
     kv = {}
@@ -437,8 +460,10 @@ def main():
         f"(inputs built in {time.perf_counter() - t0:.1f}s)")
 
     B, S = a.batch, a.streams
-    dt32, prof32, iso32, ev32, z32, nblk = time_path(lib, torch, st, fc, qs, "S", B, a.steps,
-                                                     a.warmup, S, a.reduction)
+    r32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S, a.reduction)
+    dt32, prof32, iso32, ev32, z32, nblk = (r32["dt"], r32["prof"], r32["iso"], r32["ev"],
+                                            r32["z0"], r32["nblk"])
+    two = r32["two_stage"]
     qps32 = a.steps * B * S / dt32
     res = {"metric": "q-pts/s at 3N=4764 eigpairs+v_g: FP32+refine vs FP64, max|dw|<=1e-3 cm^-1",
            "value": qps32, "unit": "q-pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -448,7 +473,8 @@ def main():
            "config": {"workload": a.workload, "natom": st.natom, "n": n,
                       "q_per_step": B * S, "batch": B, "streams": S,
                       "range": "A", "eigenvectors": True, "group_velocity": True,
-                      "precision": "S"}}
+                      "precision": "S",
+                      "reduction": "two-stage (band 32)" if two else "one-stage"}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
     iso, iso_launch_ms = split_profile(iso32, n)
@@ -489,54 +515,94 @@ def main():
                   "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
                   "frac": vel_flops / vel_s / FP32_PEAK, "traffic": None,
                   "kernel": "velocity products (k_zgemm dot epilogue, FP32)"}
-    rl = {"assemble": {"bound": "hbm", "achieved": asm_bytes / asm_s / 1e9, "peak": HBM_PEAK / 1e9,
-                       "unit": "GB/s", "frac": asm_bytes / asm_s / HBM_PEAK, "traffic": None},
-          "velocity": vel_rl}
-    # roofline of the dominant kernel, the hemv of the reduction (k_hemv_sym,
-    # one launch per column j, HBM-bound): algorithmic bytes per launch = the
-    # lower triangle of the trailing matrix, m (m + 1) / 2 complex64 elements
-    # with m = n - 1 - j, plus the vector in and the product out (2 m
-    # elements), for each of the B matrices.  The library times the launches
-    # of columns j = 0, 8, 16, ... with HIP-event pairs on its stream during
-    # the timed steps; bytes are averaged over the same columns.
-    m = (n - 1 - np.arange(0, n - 1, HEMV_TIME_STRIDE)).astype(np.float64)
-    hemv_bytes = B * 8.0 * float(np.mean(m * (m + 1) / 2 + 2 * m))
-    hemv_s = hemv_launch_ms / 1e3
-    rl["hemv"] = {"bound": "hbm", "achieved": hemv_bytes / hemv_s / 1e9, "peak": HBM_PEAK / 1e9,
-                  "unit": "GB/s", "frac": hemv_bytes / hemv_s / HBM_PEAK,
-                  "traffic": hemv_traffic(a.workload, n, B)}
+    res["roofline_velocity"] = vel_rl
+    res["config"]["fc_blocks_used"] = nblk
+    res["roofline_assembly"] = {"bound": "hbm", "achieved": asm_bytes / asm_s / 1e9,
+                                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                "frac": asm_bytes / asm_s / HBM_PEAK, "traffic": None,
+                                "kernel": "k_assemble" + (" + k_sparse_blocks" if nblk > 0 else "")}
     dominant = max(("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity"),
                    key=lambda k: phases[k])
     res["dominant_phase"] = dominant
-    res["roofline"] = dict(rl["hemv"], kernel="k_hemv_sym<float>",
-                           bytes_per_launch=hemv_bytes, avg_launch_us=hemv_launch_ms * 1e3,
-                           ms_per_batch=round(phases["hemv"], 3),
-                           note=f"timed steps, {S} batches in flight sharing the GPU")
-    iso_s = iso_launch_ms / 1e3
-    res["roofline_one_batch_alone"] = {
-        "bound": "hbm", "achieved": hemv_bytes / iso_s / 1e9, "peak": HBM_PEAK / 1e9,
-        "unit": "GB/s", "frac": hemv_bytes / iso_s / HBM_PEAK, "kernel": "k_hemv_sym<float>",
-        "avg_launch_us": iso_launch_ms * 1e3}
-    res["roofline_velocity"] = rl["velocity"]
-    res["config"]["fc_blocks_used"] = nblk
-    res["roofline_assembly"] = dict(rl["assemble"], kernel="k_assemble" + (
-        " + k_sparse_blocks" if nblk > 0 else ""))
+    if two:
+        # dominant kernel of the two-stage path: k_q2_apply, the bulge-chasing
+        # reflectors applied to all M = n eigenvectors (MFMA-bound).  Its
+        # algorithmic work: every reflector of length l touches l x M entries
+        # twice (a dot and an update); the n^2 / 2 reflector entries make
+        # n^2 M complex multiply-adds = 8 n^2 M flops per matrix, B matrices
+        # per launch (the blocked form's products on the zero part of each
+        # block are not counted).  One launch per batch, timed with a HIP-event
+        # pair by the library (slot "q2").
+        q2_flops = B * 8.0 * float(n) ** 3
+        q2_ms, q2_cnt = prof32["q2"]
+        q2_s = q2_ms / max(1, q2_cnt) / 1e3
+        res["roofline"] = {"bound": "mfma", "achieved": q2_flops / q2_s / 1e12,
+                           "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                           "frac": q2_flops / q2_s / FP32_PEAK,
+                           "traffic": kernel_traffic("k_q2_apply", a.workload, n, B),
+                           "kernel": "k_q2_apply<float, 32, 8>", "flops_per_launch": q2_flops,
+                           "avg_launch_us": q2_s * 1e6,
+                           "note": f"timed steps, {S} batches in flight sharing the GPU"}
+        iso_q2 = iso32["q2"][0] / max(1, iso32["q2"][1]) / 1e3
+        res["roofline_one_batch_alone"] = {
+            "bound": "mfma", "achieved": q2_flops / iso_q2 / 1e12, "peak": FP32_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": q2_flops / iso_q2 / FP32_PEAK,
+            "kernel": "k_q2_apply<float, 32, 8>", "avg_launch_us": iso_q2 * 1e6}
+        # the band reduction's matrix-core work: the trailing updates of the
+        # lower triangle, about 2 m^2 bw complex multiply-adds per panel (X = A
+        # VT and the rank-2 bw update), i.e. 16/3 n^3 flops per matrix in total
+        # (the textbook count of a Hermitian reduction); rated on one batch alone
+        he_flops = B * 16.0 / 3.0 * float(n) ** 3
+        he_s = iso["he2hb"] / 1e3
+        res["roofline_band_reduction"] = {
+            "bound": "mfma", "achieved": he_flops / he_s / 1e12, "peak": FP32_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": he_flops / he_s / FP32_PEAK,
+            "kernel": "he2hb (k_panel_qr + k_zgemm panel products), one batch alone"}
+    else:
+        # roofline of the dominant kernel, the hemv of the one-stage reduction
+        # (k_hemv_sym, one launch per column j, HBM-bound): algorithmic bytes
+        # per launch = the lower triangle of the trailing matrix, m (m + 1) / 2
+        # complex64 elements with m = n - 1 - j, plus the vector in and the
+        # product out (2 m elements), for each of the B matrices.  The library
+        # times the launches of columns j = 0, 8, 16, ... with HIP-event pairs
+        # on its stream during the timed steps; bytes are averaged over the
+        # same columns.
+        m = (n - 1 - np.arange(0, n - 1, HEMV_TIME_STRIDE)).astype(np.float64)
+        hemv_bytes = B * 8.0 * float(np.mean(m * (m + 1) / 2 + 2 * m))
+        hemv_s = hemv_launch_ms / 1e3
+        res["roofline"] = {"bound": "hbm", "achieved": hemv_bytes / hemv_s / 1e9,
+                           "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": hemv_bytes / hemv_s / HBM_PEAK,
+                           "traffic": kernel_traffic("k_hemv_sym", a.workload, n, B),
+                           "kernel": "k_hemv_sym<float>", "bytes_per_launch": hemv_bytes,
+                           "avg_launch_us": hemv_launch_ms * 1e3,
+                           "ms_per_batch": round(phases["hemv"], 3),
+                           "note": f"timed steps, {S} batches in flight sharing the GPU"}
+        iso_s = iso_launch_ms / 1e3
+        res["roofline_one_batch_alone"] = {
+            "bound": "hbm", "achieved": hemv_bytes / iso_s / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": hemv_bytes / iso_s / HBM_PEAK, "kernel": "k_hemv_sym<float>",
+            "avg_launch_us": iso_launch_ms * 1e3}
 
     if not a.no_fp64:
         # FP64 workspaces are larger: half the batch per stream keeps both in HBM
         B64 = max(1, B // 2) if S > 1 else B
-        dt64, prof64, _, ev64, _, _ = time_path(lib, torch, st, fc, qs, "D", B64,
-                                             max(1, a.steps - 1), 0 if a.warmup == 0 else 1, S,
-                                             a.reduction)
-        qps64 = max(1, a.steps - 1) * B64 * S / dt64
-        ev64 = ev64[:min(B, B64)]
-        ev32 = ev32[:min(B, B64)]
-        dw = float(np.abs(ev32 - ev64).max())
-        ph64, _ = split_profile(prof64, n)
+        r64 = time_path(lib, torch, st, fc, qs, "D", B64, max(1, a.steps - 1),
+                        0 if a.warmup == 0 else 1, S, a.reduction)
+        qps64 = max(1, a.steps - 1) * B64 * S / r64["dt"]
+        k = min(B, B64)
+        dw = float(np.abs(ev32[:k] - r64["ev"][:k]).max())
+        ph64, _ = split_profile(r64["prof"], n)
         res["fp64_path"] = {"value": qps64, "unit": "q-pts/s", "batch": B64, "streams": S,
-                            "phase_ms_per_batch": {k: round(v, 3) for k, v in ph64.items()}}
+                            "reduction": "two-stage (band 32)" if r64["two_stage"] else "one-stage",
+                            "phase_ms_per_batch": {kk: round(v, 3) for kk, v in ph64.items()}}
         res["speedup_vs_fp64"] = qps32 / qps64
         res["max_abs_domega_vs_fp64_cm"] = dw
+        # the FP32 path's velocities against the FP64 path's on the same q,
+        # through sums over degenerate clusters (gauge-invariant), all modes
+        dv = [velocity_difference(r32["vel"][i], r64["vel"][i], r64["ev"][i]) for i in range(k)]
+        res["max_abs_dvel_vs_fp64_m_per_s"] = max(x[0] for x in dv)
+        res["max_rel_dvel_vs_fp64"] = max(x[1] for x in dv)
         # whole-path roofline of both paths, against the spec-sheet matrix peaks
         # and against what the library GEMM reaches on this device
         p32, p64 = gemm_peak(torch, torch.float32), gemm_peak(torch, torch.float64)

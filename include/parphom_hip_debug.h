@@ -13,7 +13,7 @@
  *           Rayleigh-quotient products)
  *   mode 5: C = A B for Hermitian A (m = k, opA = 'N') from the lower
  *           triangle of A alone (the two triangular products of the band
- *           reduction's panel update)
+ *           reduction's panel update, with four real products per complex one)
  * prec 'D' complex128, 'S' complex64; column-major; returns 0 or < 0.
  */
 #ifndef PARPHOM_HIP_DEBUG_H
@@ -32,11 +32,10 @@ int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t n, int64_t
  * of the one-stage reduction alone, sampled at columns j = 0, 8, 16, ... (an
  * event pair around each sampled launch; the count is the number of launches
  * timed), and for the two-stage reduction 8 dense -> band, 9 bulge chasing
- * (both inside slot 1) and 10 the bulge-chasing reflectors' back-
- * transformation (inside slot 3).  ms/counts (11 entries each, may be NULL)
- * receive
- * the accumulated milliseconds and counts; reset != 0 clears them; enable
- * turns recording on or off. */
+ * (both inside slot 1) and 10 the k_q2_apply launch that applies the
+ * bulge-chasing reflectors (inside slot 3).  ms/counts (11 entries each, may
+ * be NULL) receive the accumulated milliseconds and counts; reset != 0 clears
+ * them; enable turns recording on or off. */
 int pph_profile(void* h, int32_t enable, int32_t reset, double* ms, int64_t* counts);
 /* Blocking of the reduction and the back-transformation on this handle:
  * nb_panel reflectors per reduction panel (16, 32, 48 or 64) and nb_back

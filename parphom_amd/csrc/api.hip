@@ -22,9 +22,16 @@ namespace pph {
 
 static thread_local std::string g_last_error;
 
-// Automatic choice of the reduction: the two-stage path from this order on
-// (below it the one-stage reduction's per-column launches are cheap).
-constexpr int64_t kTwoStageMinN = 1 << 30;
+// Automatic choice of the reduction (measured on MI355X, all eigenpairs):
+// the two-stage path on the FP32 path for orders kTwoStageMinN ..
+// kTwoStageMaxN, where its matrix-core band reduction outweighs the second
+// back-transformation (n = 1092 x 256 q: 980 against 838 q-points/s; n = 2604:
+// even; n = 4764, two batches of 32 in flight: 23.7 against 22.2).  Its panel
+// factorisation and bulge chasing give each matrix one workgroup, so a single
+// very large matrix stays on the one-stage path, and so does FP64 (its bulge
+// chasing is register-bound: n = 4764, 16 q: 1.7 s against 1.1 s for the whole
+// one-stage reduction).
+constexpr int64_t kTwoStageMinN = 512, kTwoStageMaxN = 16384;
 
 struct DevBuf {
   void* p = nullptr;
@@ -243,7 +250,7 @@ int Handle::band_width() const { return band > 0 ? band : g_band_default; }
 bool Handle::two_stage() const {
   const int r = reduction >= 0 ? reduction : g_reduction_default;
   if (r >= 0) return r == 1;
-  return n >= kTwoStageMinN;
+  return opts.precision == 'S' && n >= kTwoStageMinN && n <= kTwoStageMaxN;
 }
 
 // ---------------------------------------------------------------- small dense FP64 (host)
@@ -797,6 +804,7 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       q2.VTb = L.tsVTb.as<cplx<T>>(); q2.strideVTb = G2 * kmax2 * 2 * bwd * bwd;
       q2.Vb = q2.VTb + B * q2.strideVTb;
       q2.Z = bw.Z; q2.ldz = bw.ldz; q2.strideZ = bw.strideZ;
+      prepare_q2<T>(q2, s);
       L.tm.begin(PH_Q2, s);
       apply_q2<T>(q2, s);
       L.tm.end(PH_Q2, s);
@@ -1802,6 +1810,7 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
       // C = A B for Hermitian A (m = k, opA = 'N') from its lower triangle:
       // the lower triangle with the diagonal, then its strict part transposed
       g.tri = 1;
+      g.four_m = 1;
       launch_gemm<T>(g, s);
       g.opA = 'C'; g.tri = 2; g.beta = cplx<T>{1, 0};
     }

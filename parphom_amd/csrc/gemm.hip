@@ -18,7 +18,7 @@ constexpr int GBK = 16;
 
 // WM x WN MFMA blocks of 16 x 16 per wave: the workgroup tile is
 // (32 WM) x (32 WN); the library dispatches 2 x 2 (64 x 64)
-template <typename T, bool CONJA, bool CONJB, bool DOT, int WM = 2, int WN = 2>
+template <typename T, bool CONJA, bool CONJB, bool DOT, int WM = 2, int WN = 2, bool M3 = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WM * WN > 4 ? 1 : 2)))
 k_zgemm(GemmParams<T> p) {
   using acc_t = typename mfma<T>::acc;
@@ -63,7 +63,7 @@ k_zgemm(GemmParams<T> p) {
   const int64_t K1 = two ? p.k1 : K;
   const bool tri1 = p.tri == 1, tri2 = p.tri == 2;
 
-  constexpr bool k3M = true;
+  constexpr bool k3M = M3;
   auto product = [](T p1, T p2, T p3) {
     return k3M ? cplx<T>{p1 - p2, p3 - p1 - p2} : cplx<T>{p1, p2};
   };
@@ -275,7 +275,10 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
     dim3 g32((unsigned)ceil_div(p.m, 64), 1u, (unsigned)(p.batch * bin * ks));
 #define PPH_GEMM_CASE32(CA, CB)                                                    \
     if (ca == CA && cb == CB) {                                                    \
-      hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, 2, 1>), g32, dim3(256), 0, s, p); \
+      if (p.four_m)                                                                \
+        hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, 2, 1, false>), g32, dim3(256), 0, s, p); \
+      else                                                                         \
+        hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, 2, 1>), g32, dim3(256), 0, s, p); \
       PPH_CHECK_LAUNCH();                                                          \
       return;                                                                      \
     }
@@ -292,6 +295,19 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
     hipLaunchKernelGGL((k_zgemm<T, CA, CB, DT>), grid, dim3(256), 0, s, p);        \
     PPH_CHECK_LAUNCH();                                                            \
     return;                                                                        \
+  }
+  if (p.four_m && !dot) {
+#define PPH_GEMM_CASE4(CA, CB)                                                     \
+    if (ca == CA && cb == CB) {                                                    \
+      hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, 2, 2, false>), grid, dim3(256), 0, s, p); \
+      PPH_CHECK_LAUNCH();                                                          \
+      return;                                                                      \
+    }
+    PPH_GEMM_CASE4(false, false)
+    PPH_GEMM_CASE4(true, false)
+    PPH_GEMM_CASE4(false, true)
+    PPH_GEMM_CASE4(true, true)
+#undef PPH_GEMM_CASE4
   }
   PPH_GEMM_CASE(false, false, false)
   PPH_GEMM_CASE(true, false, false)

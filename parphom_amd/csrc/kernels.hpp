@@ -107,6 +107,9 @@ struct GemmParams {
   // and stored in its lower triangle, tri 1 with opA 'N' plus tri 2 with opA
   // 'C' (beta 1) give A B from the lower triangle alone.
   int tri;
+  // four_m != 0: four real products per complex step instead of three (no
+  // cancellation in the imaginary part; a third more matrix-core work)
+  int four_m;
 };
 
 template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);
@@ -177,6 +180,8 @@ struct Q2Work {
   cplx<T>* VTb; cplx<T>* Vb; int64_t strideVTb;    // VT and V of every block, same layout
   cplx<T>* Z; int64_t ldz, strideZ;
 };
+// V and VT of every block (k_q2_prep), then Z <- Q2 Z (k_q2_apply)
+template <typename T> void prepare_q2(const Q2Work<T>& w, hipStream_t s);
 template <typename T> void apply_q2(const Q2Work<T>& w, hipStream_t s);
 template <typename T>
 void launch_zt_to_z(const double* zt, int64_t ldzt, int64_t strideZt, cplx<T>* Z, int64_t ldz,

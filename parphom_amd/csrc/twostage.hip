@@ -8,7 +8,8 @@
 //          update of the trailing matrix
 //              VT = V T,  X = A VT,  Mx = VT^H X,  Y = X - V Mx / 2,
 //              A <- A - Y V^H - V Y^H
-//          as GEMMs on the matrix cores, on the lower triangle only (X from
+//          as GEMMs on the matrix cores (four real products per complex one,
+//          for the FP32 path's accuracy), on the lower triangle only (X from
 //          two triangular products).  The trailing matrix is read about three
 //          times per panel of bw columns instead of once per column.
 //   hb2st  band -> real tridiagonal by bulge chasing (k_hb2st): sweep j
@@ -294,6 +295,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = w.Tp; g.ldb = bw; g.strideB = w.strideTp;
       g.C = w.VT + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
+      g.four_m = 1;
       launch_gemm<T>(g, s);
     }
     // X = A22 VT   (m x bw), from the lower triangle of A22: its lower part
@@ -307,6 +309,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = w.VT + r0; g.ldb = n; g.strideB = w.strideXV;
       g.C = w.X + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
+      g.four_m = 1;
       g.tri = 1;
       launch_gemm<T>(g, s);
       g.opA = 'C'; g.tri = 2; g.beta = cplx<T>{1, 0};
@@ -328,6 +331,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = Mx; g.ldb = bw; g.strideB = w.strideTp;
       g.C = w.X + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
+      g.four_m = 1;
       launch_gemm<T>(g, s);
     }
     // A22 -= Y V^H + V Y^H   (the lower triangle: tiles wholly above the
@@ -343,6 +347,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B2 = w.X + r0; g.strideB2 = w.strideXV;
       g.C = w.A + r0 + r0 * w.lda; g.ldc = w.lda; g.strideC = w.strideA;
       g.batch = B;
+      g.four_m = 1;
       g.lower = 1;
       launch_gemm<T>(g, s);
     }
@@ -750,6 +755,8 @@ __device__ __forceinline__ void tile3m(FA&& A, FB&& B, int k0, int k1, cplx<T> o
 // dependency):
 //   W = V^H Zw   (bw x 16, K = 2bw; reflectors i0.. meet rows i0 .. i0 + bw + 15)
 //   Zw -= VT W   (2bw x 16, K = bw; VT(rr, i) = 0 for i <= rr - bw)
+// with four real MFMAs per complex step (the three-product form's cancellation
+// in the imaginary part costs the FP32 eigenvectors a factor of accuracy)
 // V and VT of a block are shared by the waves: two barriers per block.  The
 // next block's V and VT and the wave's next half of Z are loaded into
 // registers while the current block computes.
@@ -839,32 +846,69 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
         fetch_v(g, k + 1);
         fetch_z(row0 + 2 * BW);
       }
-      // W = V^H Zw on the wave's columns
+      // W = V^H Zw on the wave's columns: all row tiles of W in one K loop
+      // (their K ranges overlap and share the Zw operand), fully unrolled, so
+      // the LDS loads of later steps are issued ahead of the MFMAs
+      {
+        using acc_t = typename mfma<T>::acc;
+        constexpr int NRT = BW / 16;
+        acc_t p1[NRT], p2[NRT], p3[NRT];
 #pragma unroll
-      for (int rt = 0; rt < BW / 16; ++rt) {
-        const int i0 = 16 * rt;
-        cplx<T> o[4];
-        tile3m<T>([&](int m, int kk) { const cplx<T> v = Vs[kk][i0 + m]; return cplx<T>{v.x, -v.y}; },
-                  [&](int kk, int c) { return Zw[(s0 + kk / BW) & 1][kk % BW][cw + c]; },
-                  i0, min(2 * BW, i0 + 16 + BW), o);
+        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; p3[t] = p1[t]; }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Ws[i0 + mfma<T>::row(lane, r)][cw + lane % 16] = o[r];
+        for (int kk = 0; kk < 2 * BW; kk += 4) {
+          const int kr = kk + lane / 16;
+          const cplx<T> bz = Zw[(s0 + kr / BW) & 1][kr % BW][cw + lane % 16];
+#pragma unroll
+          for (int t = 0; t < NRT; ++t) {
+            // reflectors 16 t .. 16 t + 15 meet rows 16 t .. 16 t + bw + 15;
+            // conj(v) z with four real products (p1 = Re, p2 = Im): the
+            // three-product form loses digits of Im to cancellation
+            if (kk < 16 * t || kk >= 16 * t + 16 + BW) continue;
+            const cplx<T> v = Vs[kr][16 * t + lane % 16];
+            p1[t] = mfma<T>::op(v.x, bz.x, p1[t]);
+            p1[t] = mfma<T>::op(v.y, bz.y, p1[t]);
+            p2[t] = mfma<T>::op(v.x, bz.y, p2[t]);
+            p2[t] = mfma<T>::op(-v.y, bz.x, p2[t]);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < NRT; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            Ws[16 * t + mfma<T>::row(lane, r)][cw + lane % 16] = cplx<T>{p1[t][r], p2[t][r]};
       }
       wave_sync();
-      // Zw -= VT W on the wave's columns
+      // Zw -= VT W on the wave's columns: the row tiles share the W operand
+      {
+        using acc_t = typename mfma<T>::acc;
+        constexpr int NRT = 2 * BW / 16;
+        acc_t p1[NRT], p2[NRT], p3[NRT];
 #pragma unroll
-      for (int rt = 0; rt < 2 * BW / 16; ++rt) {
-        const int r0 = 16 * rt;
-        const int kb = max(0, r0 - BW + 1) & ~3;
-        cplx<T> o[4];
-        tile3m<T>([&](int m, int kk) { return VTs[kk][r0 + m]; },
-                  [&](int kk, int c) { return Ws[kk][cw + c]; }, kb, BW, o);
+        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; p3[t] = p1[t]; }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int rr = r0 + mfma<T>::row(lane, r);
-          cplx<T>& zz = Zw[(s0 + rr / BW) & 1][rr % BW][cw + lane % 16];
-          zz = csub(zz, o[r]);
+        for (int kk = 0; kk < BW; kk += 4) {
+          const int kr = kk + lane / 16;
+          const cplx<T> wv = Ws[kr][cw + lane % 16];
+#pragma unroll
+          for (int t = 0; t < NRT; ++t) {
+            // VT(rr, i) = 0 for i <= rr - bw
+            if (kk + 3 < 16 * t - BW + 1) continue;
+            const cplx<T> a = VTs[kr][16 * t + lane % 16];
+            p1[t] = mfma<T>::op(a.x, wv.x, p1[t]);
+            p1[t] = mfma<T>::op(-a.y, wv.y, p1[t]);
+            p2[t] = mfma<T>::op(a.x, wv.y, p2[t]);
+            p2[t] = mfma<T>::op(a.y, wv.x, p2[t]);
+          }
         }
+#pragma unroll
+        for (int t = 0; t < NRT; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int rr = 16 * t + mfma<T>::row(lane, r);
+            cplx<T>& zz = Zw[(s0 + rr / BW) & 1][rr % BW][cw + lane % 16];
+            zz = csub(zz, cplx<T>{p1[t][r], p2[t][r]});
+          }
       }
       wave_sync();
       // logical half 0 is finished for this group: store it; the half after
@@ -878,32 +922,32 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
 }
 
 template <typename T>
-void apply_q2(const Q2Work<T>& w, hipStream_t s) {
+void prepare_q2(const Q2Work<T>& w, hipStream_t s) {
   const int64_t n = w.n, B = w.batch;
   if (n < 2 || w.m <= 0) return;
   const int64_t G = ceil_div(n - 1, w.bw);
-  // columns per workgroup: 16 per wave, 8 waves for FP32 and 2 for FP64 (LDS)
-  constexpr int NWV = sizeof(T) == 4 ? 8 : 2;
-  const unsigned strips = (unsigned)ceil_div(w.m, 16 * NWV);
-  if (w.bw == 32) {
-    hipLaunchKernelGGL((k_q2_prep<T, 32>), dim3((unsigned)w.kmax, (unsigned)G, (unsigned)B),
-                       dim3(256), 0, s, w);
-    PPH_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_q2_apply<T, 32, NWV>), dim3(strips, (unsigned)B), dim3(WAVE * NWV), 0, s,
-                       w);
-    PPH_CHECK_LAUNCH();
-  } else if (w.bw == 16) {
-    hipLaunchKernelGGL((k_q2_prep<T, 16>), dim3((unsigned)w.kmax, (unsigned)G, (unsigned)B),
-                       dim3(256), 0, s, w);
-    PPH_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_q2_apply<T, 16, NWV>), dim3(strips, (unsigned)B), dim3(WAVE * NWV), 0, s,
-                       w);
-    PPH_CHECK_LAUNCH();
-  } else {
-    throw std::runtime_error("apply_q2: band width must be 16 or 32");
-  }
+  const dim3 grid((unsigned)w.kmax, (unsigned)G, (unsigned)B);
+  if (w.bw == 32) hipLaunchKernelGGL((k_q2_prep<T, 32>), grid, dim3(256), 0, s, w);
+  else if (w.bw == 16) hipLaunchKernelGGL((k_q2_prep<T, 16>), grid, dim3(256), 0, s, w);
+  else throw std::runtime_error("prepare_q2: band width must be 16 or 32");
+  PPH_CHECK_LAUNCH();
 }
 
+template <typename T>
+void apply_q2(const Q2Work<T>& w, hipStream_t s) {
+  const int64_t n = w.n, B = w.batch;
+  if (n < 2 || w.m <= 0) return;
+  // columns per workgroup: 16 per wave, 8 waves for FP32 and 2 for FP64 (LDS)
+  constexpr int NWV = sizeof(T) == 4 ? 8 : 2;
+  const dim3 grid((unsigned)ceil_div(w.m, 16 * NWV), (unsigned)B);
+  if (w.bw == 32) hipLaunchKernelGGL((k_q2_apply<T, 32, NWV>), grid, dim3(WAVE * NWV), 0, s, w);
+  else if (w.bw == 16) hipLaunchKernelGGL((k_q2_apply<T, 16, NWV>), grid, dim3(WAVE * NWV), 0, s, w);
+  else throw std::runtime_error("apply_q2: band width must be 16 or 32");
+  PPH_CHECK_LAUNCH();
+}
+
+template void prepare_q2<float>(const Q2Work<float>&, hipStream_t);
+template void prepare_q2<double>(const Q2Work<double>&, hipStream_t);
 template void apply_q2<float>(const Q2Work<float>&, hipStream_t);
 template void apply_q2<double>(const Q2Work<double>&, hipStream_t);
 
