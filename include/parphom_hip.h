@@ -57,7 +57,12 @@ typedef struct pph_opts {
   char jobz;              /* 'N' eigenvalues only, 'V' with eigenvectors (forced by vel) */
   char range;             /* 'A' all, 'V' interval, 'I' index range */
   char vel_method;        /* 'A' analytic dD/dq, 'C' central difference */
-  char precision;         /* 'D' FP64 throughout, 'S' FP32 matrix cores + FP64 refinement */
+  char precision;         /* 'D' FP64 throughout, 'S' FP32 matrix cores + FP64 refinement.
+                             On 'S' the eigenvalues are FP64 Rayleigh quotients of the
+                             FP32 vectors (refined); bisection of the FP32 tridiagonal,
+                             which only supplies the inverse-iteration shifts, stops at
+                             max(abstol, 1e-13 |T|); with range 'V' a mode whose refined
+                             eigenvalue leaves (vl, vu] is dropped */
   int32_t vel;            /* 1: group velocities */
   int32_t device;         /* HIP device ordinal */
   double vl, vu;          /* range 'V' bounds in the input file's units (cm^-1); like

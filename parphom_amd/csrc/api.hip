@@ -5,7 +5,6 @@
 #include "../../include/parphom_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <complex>
@@ -175,25 +174,10 @@ struct Lane {
   // two-stage reduction: band, bulge-chasing reflectors and their block
   // factors, panel T / Mx, Gram partials, stall flags
   DevBuf tsAB, tsV2, tsTau2, tsVTb, tsTp, tsG, tsInfo;
-  // rocBLAS handle for the plain GEMMs of the back-transformation (made on
-  // first use, bound to this lane's stream, atomics off so results repeat)
-  rocblas_handle blas = nullptr;
-  rocblas_handle blas_on(hipStream_t s) {
-    if (!blas) {
-      if (rocblas_create_handle(&blas) != rocblas_status_success)
-        throw std::runtime_error("rocblas_create_handle failed");
-      if (rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
-        throw std::runtime_error("rocblas_set_atomics_mode failed");
-    }
-    if (rocblas_set_stream(blas, s) != rocblas_status_success)
-      throw std::runtime_error("rocblas_set_stream failed");
-    return blas;
-  }
   Lane() = default;
   Lane(const Lane&) = delete;
   Lane& operator=(const Lane&) = delete;
   ~Lane() {
-    if (blas) (void)rocblas_destroy_handle(blas);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -360,6 +344,7 @@ constexpr int64_t kVpad = 128;
 
 struct EigResult {
   std::vector<int64_t> M;     // per matrix
+  std::vector<int64_t> first; // per matrix: index of the first returned mode
   std::vector<int32_t> info;
   int64_t mcap = 0;           // leading dimension of w / columns of Z
   double* lam = nullptr;      // device, B x mcap (lambda)
@@ -750,6 +735,7 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     PPH_HIP(hipMemcpyAsync(rinfo.data(), ts.info, B * sizeof(int), hipMemcpyDeviceToHost, s));
   PPH_HIP(hipStreamSynchronize(s));
   R.M.assign(B, 0);
+  R.first.assign(B, 0);
   // a stalled bulge-chasing pipeline reports as a convergence failure
   R.info.assign(rinfo.begin(), rinfo.end());
   int64_t mmax = 0;
@@ -791,7 +777,6 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     L.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     L.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     bw.Tm = L.Tm.as<cplx<T>>(); bw.VT = L.VT.as<cplx<T>>(); bw.Wt = L.Wt.as<cplx<T>>();
-    bw.blas = L.blas_on(s);
     L.tm.begin(PH_BACK, s);
     if (two) {
       // Z = Q1 (Q2 Z_T): the bulge-chasing reflectors first, then the he2hb
@@ -866,6 +851,23 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
                        mcap, s);
       L.tm.end(PH_REFINE, s);
       R.Z64 = L.Z64.as<cplx<double>>();
+      if (o.range == 'V') {
+        // RANGE = 'V' selects lambda in (vl, vu] by Sturm counts of the FP32
+        // tridiagonal; a refined eigenvalue that left the interval (one within
+        // the FP32 error of a bound) is dropped.  (A mode that the FP32 counts
+        // put just outside is not computed, so it cannot be added.)
+        std::vector<double> wl((size_t)B * mcap);
+        PPH_HIP(hipMemcpyAsync(wl.data(), te.w, wl.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        PPH_HIP(hipStreamSynchronize(s));
+        for (int64_t b = 0; b < B; ++b) {
+          const double* wb = wl.data() + b * mcap;
+          int64_t f = 0, l = R.M[b];
+          while (f < l && !(wb[f] > vl_l)) ++f;
+          while (l > f && wb[l - 1] > vu_l) --l;
+          R.first[b] = f;
+          R.M[b] = l - f;
+        }
+      }
     } else {
       R.Z64 = reinterpret_cast<cplx<double>*>(L.Z.p);
     }
@@ -933,8 +935,8 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   else eig_batch<double>(h, L, B, L.A64.as<cplx<double>>(), nullptr, want_vec, vl_l, vu_l, R);
 
   const int64_t mcap = R.mcap;
-  int64_t mmax = 0;
-  for (auto m : R.M) mmax = std::max(mmax, m);
+  int64_t mmax = 0;    // modes 0 .. first + M - 1 of each matrix
+  for (int64_t b = 0; b < B; ++b) mmax = std::max(mmax, R.first[b] + R.M[b]);
   L.om.ensure((size_t)B * mcap * sizeof(double));
   launch_eval_to_cm(R.lam, L.om.as<double>(), B, mmax, mcap, s);
   out.om = L.om.as<double>();
@@ -1272,14 +1274,15 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
         }
         infos[b] = info;
         nzs[b] = nz;
+        const int64_t f0 = b * R.mcap + R.first[b];      // first returned mode
         if (m > 0)
-          PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + b * R.mcap,
-                                 m * sizeof(double), kind, s));
+          PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + f0, m * sizeof(double),
+                                 kind, s));
         if (out->evec && nz > 0)
           PPH_HIP(hipMemcpyAsync((char*)out->evec + (size_t)iq * n * out->evec_cols * 16,
-                                 R.Z64 + (size_t)b * n * R.mcap, (size_t)nz * n * 16, kind, s));
+                                 R.Z64 + (size_t)f0 * n, (size_t)nz * n * 16, kind, s));
         if (out->vel && nz > 0)
-          PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + b * R.mcap * 3,
+          PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + f0 * 3,
                                  (size_t)nz * 3 * sizeof(double), kind, s));
       }
       if (device_out) {
@@ -1476,9 +1479,10 @@ int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w,
     for (int64_t b = 0; b < B; ++b) {
       const int64_t mm = R.M[b];
       if (mm > 0) {
-        PPH_HIP(hipMemcpyAsync(w + (m0 + b) * n, R.lam + b * R.mcap, mm * sizeof(double), out_kind, s));
+        const int64_t f0 = b * R.mcap + R.first[b];
+        PPH_HIP(hipMemcpyAsync(w + (m0 + b) * n, R.lam + f0, mm * sizeof(double), out_kind, s));
         if (Z && R.Z64)
-          PPH_HIP(hipMemcpyAsync((char*)Z + (size_t)(m0 + b) * n * n * 16, R.Z64 + (size_t)b * n * R.mcap,
+          PPH_HIP(hipMemcpyAsync((char*)Z + (size_t)(m0 + b) * n * n * 16, R.Z64 + (size_t)f0 * n,
                                  (size_t)mm * n * 16, out_kind, s));
       }
       int32_t inf = R.info[b];

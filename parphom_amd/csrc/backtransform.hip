@@ -1,17 +1,13 @@
 // Back-transformation of tridiagonal eigenvectors, Z = Q Z_T with
 // Q = H_0 H_1 ... H_{n-2}, applied one panel of reflectors at a time in
 // compact-WY form Q_p = I - V_p T_p V_p^H (the PZUNMTR stage of pzheevx):
-//   Wt = V_p^H Z          (split-K on the library's GEMM)
+//   Wt = V_p^H Z          (split-K)
 //   VT = V_p T_p
-//   Z  = Z - VT Wt        (rocBLAS GEMM; or the library's own)
-// T_p for every panel is formed up front from the Gram matrices V_p^H V_p.
-// The update of Z is a plain GEMM with K = 128 and no fused epilogue, the
-// one shape here where rocBLAS's tuned kernels beat the library's 64 x 64
-// tile (measured at n = 4764, batch 32: 114 vs 102 TFLOP/s; for Wt the
-// library's kernel is the faster, 140 vs 127).
+//   Z  = Z - VT Wt
+// all on the library's matrix-core GEMM.  T_p for every panel is formed up
+// front from the Gram matrices V_p^H V_p.
 #include "common.hpp"
 #include "kernels.hpp"
-#include <rocblas/rocblas.h>
 
 namespace pph {
 
@@ -67,41 +63,6 @@ __global__ void __launch_bounds__(256) k_form_t(const cplx<T>* __restrict__ G, c
   }
 }
 
-#define PPH_BLAS(call)                                                                  \
-  do {                                                                                  \
-    const rocblas_status st_ = (call);                                                  \
-    if (st_ != rocblas_status_success)                                                  \
-      throw std::runtime_error(std::string("rocBLAS: ") + rocblas_status_to_string(st_) + \
-                               " (" #call ")");                                         \
-  } while (0)
-
-// C = alpha op(A) B + beta C, batched with strides (op = conjugate transpose
-// when conjA), column-major
-static void blas_gemm(rocblas_handle h, bool conjA, int64_t m, int64_t n, int64_t k, float alpha,
-                      const cplx<float>* A, int64_t lda, int64_t sA, const cplx<float>* B,
-                      int64_t ldb, int64_t sB, float beta, cplx<float>* C, int64_t ldc, int64_t sC,
-                      int64_t batch) {
-  const rocblas_float_complex a{alpha, 0.f}, bt{beta, 0.f};
-  PPH_BLAS(rocblas_cgemm_strided_batched(
-      h, conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
-      rocblas_operation_none, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &a,
-      reinterpret_cast<const rocblas_float_complex*>(A), (rocblas_int)lda, sA,
-      reinterpret_cast<const rocblas_float_complex*>(B), (rocblas_int)ldb, sB, &bt,
-      reinterpret_cast<rocblas_float_complex*>(C), (rocblas_int)ldc, sC, (rocblas_int)batch));
-}
-static void blas_gemm(rocblas_handle h, bool conjA, int64_t m, int64_t n, int64_t k, double alpha,
-                      const cplx<double>* A, int64_t lda, int64_t sA, const cplx<double>* B,
-                      int64_t ldb, int64_t sB, double beta, cplx<double>* C, int64_t ldc,
-                      int64_t sC, int64_t batch) {
-  const rocblas_double_complex a{alpha, 0.0}, bt{beta, 0.0};
-  PPH_BLAS(rocblas_zgemm_strided_batched(
-      h, conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
-      rocblas_operation_none, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &a,
-      reinterpret_cast<const rocblas_double_complex*>(A), (rocblas_int)lda, sA,
-      reinterpret_cast<const rocblas_double_complex*>(B), (rocblas_int)ldb, sB, &bt,
-      reinterpret_cast<rocblas_double_complex*>(C), (rocblas_int)ldc, sC, (rocblas_int)batch));
-}
-
 template <typename T>
 void launch_zt_to_z(const double* zt, int64_t ldzt, int64_t strideZt, cplx<T>* Z, int64_t ldz,
                     int64_t strideZ, int64_t n, int64_t m, int64_t batch, hipStream_t s) {
@@ -148,8 +109,7 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
     const int64_t mr = n - j0 - sh;                // rows j0 + sh .. n-1
     const int64_t pw = std::min(nb, nrefl - j0);
     const int64_t kw = ceil_div(pw, 16) * 16;
-    rocblas_handle blas = reinterpret_cast<rocblas_handle>(w.blas);
-    // Wt = V_p^H Z(j0+1:, :)   (kw x M)
+    // Wt = V_p^H Z(j0 + sh:, :)   (kw x M)
     {
       launch_fill_zero(w.Wt, (int64_t)((B - 1) * w.strideWs + nb * M) * (int64_t)sizeof(cplx<T>), s);
       GemmParams<T> g{};
@@ -176,11 +136,8 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
       g.batch = B;
       launch_gemm<T>(g, s);
     }
-    // Z(j0+1:, :) -= VT Wt
-    if (blas) {
-      blas_gemm(blas, false, mr, M, kw, T(-1), w.VT, n, w.strideWs, w.Wt, nb, w.strideWs, T(1),
-                w.Z + (j0 + sh), w.ldz, w.strideZ, B);
-    } else {
+    // Z(j0 + sh:, :) -= VT Wt
+    {
       GemmParams<T> g{};
       g.opA = 'N'; g.opB = 'N';
       g.m = mr; g.n = M; g.k = kw;

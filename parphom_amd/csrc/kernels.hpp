@@ -224,9 +224,6 @@ struct BackWork {
   const double* zt; int64_t ldzt; int64_t strideZt;   // row-major T eigenvectors
   cplx<T>* Z; int64_t ldz; int64_t strideZ;            // output n x m column-major
   cplx<T>* Tm; cplx<T>* VT; cplx<T>* Wt; int64_t strideWs;   // workspaces
-  // rocblas_handle bound to the stream: the two plain products of every
-  // panel go to rocBLAS's GEMM (null: the library's own GEMM)
-  void* blas;
   // reflector j has its unit at row j + shift (1 after the one-stage
   // reduction, bw after he2hb), so there are n - shift of them
   int64_t shift = 1;

@@ -133,6 +133,38 @@ def test_heevx_ranges(gpu_available):
     assert len(w) == len(sel) and np.abs(w - sel).max() < 1e-11
 
 
+@pytest.mark.parametrize("n", [150, 600])
+def test_heevx_range_v_fp32_bounds(gpu_available, n):
+    """RANGE='V' on the FP32 path with bounds placed within the FP32 error of
+    an eigenvalue (1e-7 |A| outside the window): the FP32 Sturm counts may
+    select such a mode, and its FP64-refined eigenvalue then lies outside
+    (vl, vu], so it must be dropped.  Every returned eigenvalue lies in the
+    window, matches numpy's to 1e-7 |A|, and only modes within 1e-5 |A| of a
+    bound may be missing.  n = 600 runs the two-stage reduction."""
+    lib = _lib()
+    rng = np.random.default_rng(16 + n)
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = X + X.conj().T
+    wref = np.linalg.eigvalsh(A)
+    nrm = np.abs(wref).max()
+    eps = 1e-7 * nrm
+    lo, hi = n // 4, (3 * n) // 4
+    vl, vu = wref[lo] + eps, wref[hi] - eps          # wref[lo], wref[hi] just outside
+    w, Z, info = lib.heevx(A.copy(), jobz="V", rng="V", vl=vl, vu=vu, precision="S")
+    assert info == 0
+    assert len(w) > 0 and (w > vl).all() and (w <= vu).all()
+    sel = wref[(wref > vl) & (wref <= vu)]
+    near = np.minimum(np.abs(sel - vl), np.abs(sel - vu)) <= 1e-5 * nrm
+    idx = np.searchsorted(sel, w)
+    idx = np.clip(idx, 0, len(sel) - 1)
+    idx = np.where(np.abs(sel[np.clip(idx - 1, 0, None)] - w) < np.abs(sel[idx] - w),
+                   np.clip(idx - 1, 0, None), idx)
+    assert np.abs(sel[idx] - w).max() <= 1e-7 * nrm
+    missing = np.setdiff1d(np.arange(len(sel)), idx)
+    assert near[missing].all() and len(w) >= len(sel) - near.sum()
+    assert np.linalg.norm(A @ Z - Z * w, axis=0).max() <= 1e-5 * nrm
+
+
 @pytest.mark.parametrize("prec", ["D", "S"])
 @pytest.mark.parametrize("sparse", [1, 2])
 def test_run_tbg_3_4_path(gpu_available, prec, sparse):
