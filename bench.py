@@ -418,6 +418,8 @@ def main():
                          "-1 the library's choice")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip timing the FP32 path with the other reduction")
     ap.add_argument("--fcsym", action="store_true",
                     help="measure the force-constant symmetrisation instead (see fcsym_bench)")
     ap.add_argument("--dos", action="store_true",
@@ -611,6 +613,18 @@ def main():
             "D": path_roofline(st.natom, n, B64, qps64, FP64_PEAK, p64),
             "gemm_peak_measured_tflops": {"fp32": p32 / 1e12, "fp64": p64 / 1e12},
             "gemm_peak_spec_tflops": {"fp32": FP32_PEAK / 1e12, "fp64": FP64_PEAK / 1e12}}
+        if not a.no_compare:
+            # the FP32 path with the other reduction, same batches and lanes
+            other = 0 if two else 1
+            ro = time_path(lib, torch, st, fc, qs, "S", B, a.steps, min(a.warmup, 1), S, other)
+            pho, _ = split_profile(ro["iso"], n)
+            dvo = [velocity_difference(ro["vel"][i], r64["vel"][i], r64["ev"][i]) for i in range(k)]
+            res["fp32_other_reduction"] = {
+                "reduction": "two-stage (band 32)" if ro["two_stage"] else "one-stage",
+                "value": a.steps * B * S / ro["dt"], "unit": "q-pts/s",
+                "max_abs_domega_vs_fp64_cm": float(np.abs(ro["ev"][:k] - r64["ev"][:k]).max()),
+                "max_rel_dvel_vs_fp64": max(x[1] for x in dvo),
+                "phase_ms_one_batch_alone": {kk: round(v, 3) for kk, v in pho.items()}}
 
     if not a.no_cpu:
         cdt, r = cpu_baseline(st, fc, qs[0])

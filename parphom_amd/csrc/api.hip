@@ -171,6 +171,7 @@ struct Lane {
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
+  DevBuf gws, sws;         // partials of the GEMM dot epilogue / split-K and of the sparse forms
   // two-stage reduction: band, bulge-chasing reflectors and their block
   // factors, panel T / Mx, Gram partials, stall flags
   DevBuf tsAB, tsV2, tsTau2, tsVTb, tsTp, tsG, tsInfo;
@@ -682,13 +683,15 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     hb2st<T>(ts, s);
     L.tm.end(PH_HB2ST, s);
   } else {
-    L.ypart.ensure((size_t)B * n * sizeof(cplx<T>));
-    L.dpart.ensure((size_t)B * 2 * nb * sizeof(cplx<T>));
+    // the hemv's per-workgroup partials: 2 ntmax slots of ntmax * 64 rows
+    const int64_t ntmax = hemv_tiles(n), ldY = ntmax * 64;
+    L.ypart.ensure((size_t)B * 2 * ntmax * ldY * sizeof(cplx<T>));
+    L.dpart.ensure((size_t)B * ntmax * 2 * nb * sizeof(cplx<T>));
     L.scal.ensure((size_t)B * 256 * sizeof(double));
     const int64_t nparts = n / 64 + 2;
     L.part.ensure((size_t)B * 3 * nparts * sizeof(double));
-    tw.ypart = L.ypart.as<cplx<T>>(); tw.strideY = n;
-    tw.dpart = L.dpart.as<cplx<T>>(); tw.strideDp = 2 * nb;
+    tw.ypart = L.ypart.as<cplx<T>>(); tw.strideY = 2 * ntmax * ldY; tw.ldY = ldY; tw.ntmax = ntmax;
+    tw.dpart = L.dpart.as<cplx<T>>(); tw.strideDp = ntmax * 2 * nb;
     tw.scal = L.scal.as<double>(); tw.strideScal = 256;
     tw.part = L.part.as<double>(); tw.strideP = 3 * nparts; tw.yvOff = nparts;
     tw.nchunk = h.nchunk;
@@ -777,6 +780,8 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     L.VT.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     L.Wt.ensure((size_t)B * bw.strideWs * sizeof(cplx<T>));
     bw.Tm = L.Tm.as<cplx<T>>(); bw.VT = L.VT.as<cplx<T>>(); bw.Wt = L.Wt.as<cplx<T>>();
+    L.gws.ensure((size_t)back_transform_ws_bytes<T>(n, B, nbt, mmax, two ? bwd : 1));
+    bw.ws = L.gws.p;
     L.tm.begin(PH_BACK, s);
     if (two) {
       // Z = Q1 (Q2 Z_T): the bulge-chasing reflectors first, then the he2hb
@@ -822,11 +827,12 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
         L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<float>));
         launch_transpose<float>(L.Z.as<cplx<float>>(), n, mmax, n, n * mcap, L.Zt.as<cplx<float>>(),
                                 mcap, n * mcap, B, s);
+        L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, mmax, B, v ? 4 : 1));
         launch_sparse_dots<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
                                   h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1,
                                   L.Zt.as<cplx<float>>(), mcap, n * mcap, h.N, mmax, B,
                                   L.num.as<double>(), mcap, v ? L.vacc.as<double>() : nullptr, mcap,
-                                  3 * mcap, s);
+                                  3 * mcap, L.sws.as<double>(), s);
         if (v) L.tm.end(PH_VEL, s);
       }
       L.tm.begin(PH_REFINE, s);
@@ -842,6 +848,8 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
         g.dot_out = L.num.as<double>(); g.dot_stride = mcap;
         g.herm_dot = 1;            // D is Hermitian: lower triangle, half the work
         g.dotX = L.Z64.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap;
+        L.gws.ensure((size_t)gemm_ws_bytes(g));
+        g.ws = L.gws.p;
         launch_gemm<double>(g, s);
       }
       launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
@@ -952,10 +960,11 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
       L.Zt.ensure((size_t)B * n * mcap * sizeof(cplx<double>));
       launch_transpose<double>(L.Z.as<cplx<double>>(), n, mmax, n, n * mcap,
                                L.Zt.as<cplx<double>>(), mcap, n * mcap, B, s);
+      L.sws.ensure((size_t)sparse_dots_ws_bytes(N, mmax, B, 3));
       launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
                                  h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 1, 3,
                                  L.Zt.as<cplx<double>>(), mcap, n * mcap, N, mmax, B, nullptr, 0,
-                                 L.vacc.as<double>(), mcap, 3 * mcap, s);
+                                 L.vacc.as<double>(), mcap, 3 * mcap, L.sws.as<double>(), s);
     } else if (fp32) {
       GemmParams<float> g{};
       g.opA = 'N'; g.opB = 'N';
@@ -967,6 +976,8 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
       g.dot_out = L.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
       g.herm_dot = 1;            // dD/dq is Hermitian: lower triangle, half the work
       g.dotX = L.Z.as<cplx<float>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
+      L.gws.ensure((size_t)gemm_ws_bytes(g));
+      g.ws = L.gws.p;
       launch_gemm<float>(g, s);
     } else {
       GemmParams<double> g{};
@@ -979,6 +990,8 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
       g.dot_out = L.vacc.as<double>(); g.dot_stride = 3 * mcap; g.sDotin = mcap;
       g.herm_dot = 1;            // dD/dq is Hermitian: lower triangle, half the work
       g.dotX = L.Z.as<cplx<double>>(); g.lddx = n; g.strideX = n * mcap; g.sXin = 0;
+      L.gws.ensure((size_t)gemm_ws_bytes(g));
+      g.ws = L.gws.p;
       launch_gemm<double>(g, s);
     }
     L.tm.end(PH_VEL, s);
@@ -1386,7 +1399,7 @@ int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t
   hipStream_t s = h->stream;
   const int64_t n = h->n;
   const int64_t B = std::max<int64_t>(1, std::min<int64_t>(nq, 4));
-  DevBuf zb, ob;
+  DevBuf zb, ob, wsb;
   std::vector<double> qrows;
   for (int64_t q0 = 0; q0 < nq; q0 += B) {
     const int64_t nb = std::min(B, nq - q0);
@@ -1432,6 +1445,8 @@ int pph_expect(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, int32_t
     g.dot_out = od; g.dot_stride = nz;
     g.herm_dot = 1;
     g.dotX = Zq; g.lddx = n; g.strideX = n * zcols;
+    wsb.ensure((size_t)gemm_ws_bytes(g));
+    g.ws = wsb.p;
     launch_gemm<double>(g, s);
     if (!on_device)
       PPH_HIP(hipMemcpyAsync(out + q0 * nz, od, nb * nz * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1786,7 +1801,7 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
   (void)ar; (void)br;
   hipStream_t s;
   PPH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  DevBuf dA, dB, dC, dX;
+  DevBuf dA, dB, dC, dX, dW;
   dA.ensure(es * lda * ac); dB.ensure(es * ldb * bc); dC.ensure(es * ldc * n); dX.ensure(8 * n + 8);
   PPH_HIP(hipMemcpyAsync(dA.p, A, es * lda * ac, hipMemcpyHostToDevice, s));
   PPH_HIP(hipMemcpyAsync(dB.p, B, es * ldb * bc, hipMemcpyHostToDevice, s));
@@ -1804,7 +1819,6 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
       g.dot_out = dX.as<double>(); g.dotX = dC.as<cplx<T>>(); g.lddx = ldc;
       g.herm_dot = (mode == 4) ? 1 : 0;
     } else if (mode == 2) {
-      launch_fill_zero(dC.p, es * ldc * n, s);
       g.ksplit = 4;
     } else if (mode == 3) {
       // C = A(:, 0:k/2) B(0:k/2, :) + A(:, k/2:) B(k/2:, :) through the two-product path (opA = opB = 'N')
@@ -1818,6 +1832,8 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
       launch_gemm<T>(g, s);
       g.opA = 'C'; g.tri = 2; g.beta = cplx<T>{1, 0};
     }
+    dW.ensure((size_t)gemm_ws_bytes(g));
+    g.ws = dW.p;
     launch_gemm<T>(g, s);
   };
   if (prec == 'S') run(float{}); else run(double{});

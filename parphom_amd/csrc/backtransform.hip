@@ -111,7 +111,6 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
     const int64_t kw = ceil_div(pw, 16) * 16;
     // Wt = V_p^H Z(j0 + sh:, :)   (kw x M)
     {
-      launch_fill_zero(w.Wt, (int64_t)((B - 1) * w.strideWs + nb * M) * (int64_t)sizeof(cplx<T>), s);
       GemmParams<T> g{};
       g.opA = 'C'; g.opB = 'N';
       g.m = kw; g.n = M; g.k = mr;
@@ -120,8 +119,9 @@ void back_transform(const BackWork<T>& w, hipStream_t s) {
       g.B = w.Z + (j0 + sh); g.ldb = w.ldz; g.strideB = w.strideZ;
       g.C = w.Wt; g.ldc = nb; g.strideC = w.strideWs;
       g.batch = B;
-      const int64_t tiles = ceil_div(M, 64) * B;
-      g.ksplit = std::max<int64_t>(1, std::min<int64_t>(ceil_div(mr, 256), ceil_div(1024, tiles)));
+      g.ksplit = back_ksplit(mr, M, B);
+      g.ws = w.ws;
+      if (g.ksplit > 1 && w.ws == nullptr) throw std::runtime_error("back_transform: split-K needs a workspace");
       launch_gemm<T>(g, s);
     }
     // VT = V_p T_p   (mr x kw)

@@ -80,23 +80,6 @@ __device__ __forceinline__ T block_max(T v, T* sh) {
   return r;
 }
 
-// Device-scope atomics used for the "last block finishes" pattern.
-__device__ __forceinline__ unsigned atomic_inc_acqrel(unsigned* p) {
-  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void atomic_add_relaxed(T* p, T v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T atomic_load_acq(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void atomic_store_rel(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // ---------------------------------------------------------------- matrix cores
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));

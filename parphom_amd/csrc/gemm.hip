@@ -205,7 +205,8 @@ k_zgemm(GemmParams<T> p) {
   }
 
   if (!DOT) {
-    cplx<T>* C = p.C + b * p.strideC + bi * p.sCin;
+    // a split-K piece writes its own slice (launch_gemm points C at the workspace)
+    cplx<T>* C = p.C + b * p.strideC + bi * p.sCin + ks_idx * p.sCsplit;
     const bool beta0 = (p.beta.x == T(0) && p.beta.y == T(0));
 #pragma unroll
     for (int mi = 0; mi < WM; ++mi)
@@ -220,18 +221,17 @@ k_zgemm(GemmParams<T> p) {
           cplx<T> v = product(acc1[mi][ni][i], acc2[mi][ni][i], acc3[mi][ni][i]);
           v = cmul(p.alpha, v);
           cplx<T>* c = C + row + col * p.ldc;
-          if (ksplit > 1) {
-            atomicAdd(&c->x, v.x);
-            atomicAdd(&c->y, v.y);
-          } else {
-            if (!beta0) v = cadd(v, cmul(p.beta, *c));
-            *c = v;
-          }
+          if (!beta0) v = cadd(v, cmul(p.beta, *c));
+          *c = v;
         }
       }
   } else {
+    // per column: the two waves of a column half add their rows in LDS (upper
+    // half first), and the row tile's sum goes to its partial slot
+    __shared__ double dlow[GBN];
     const cplx<T>* X = p.dotX + b * p.strideX + bi * p.sXin;
-    double* dout = p.dot_out + b * p.dot_stride + bi * p.sDotin;
+    double* dpart = static_cast<double*>(p.ws) + (z * (int64_t)gridDim.x + blockIdx.x) * N;
+    double dsum[WN];
 #pragma unroll
     for (int ni = 0; ni < WN; ++ni) {
       const int64_t col = col0 + wc0 + ni * 16 + lane % 16;
@@ -252,10 +252,61 @@ k_zgemm(GemmParams<T> p) {
       if (herm) s *= 2.0;
       s += __shfl_xor(s, 16, WAVE);
       s += __shfl_xor(s, 32, WAVE);
-      if (lane < 16 && col < N) atomicAdd(dout + col, s);
+      dsum[ni] = s;
+      if (wave >= 2 && lane < 16) dlow[wc0 + ni * 16 + lane] = s;
+    }
+    __syncthreads();
+    if (wave < 2 && lane < 16) {
+#pragma unroll
+      for (int ni = 0; ni < WN; ++ni) {
+        const int c = wc0 + ni * 16 + lane;
+        if (col0 + c < N) dpart[col0 + c] = dsum[ni] + dlow[c];
+      }
     }
   }
 }
+
+// out[(b binner + bi) ...] += sum over the row tiles of the dot partials, in order
+__global__ void __launch_bounds__(256) k_dot_reduce(const double* __restrict__ part, int64_t rt,
+                                                    int64_t N, int64_t binner, double* dout,
+                                                    int64_t dot_stride, int64_t sDotin) {
+  const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t z = blockIdx.y;
+  if (col >= N) return;
+  const double* P = part + z * rt * N + col;
+  double s = 0.0;
+  for (int64_t t = 0; t < rt; ++t) s += P[t * N];
+  dout[(z / binner) * dot_stride + (z % binner) * sDotin + col] += s;
+}
+
+// C = sum over the split-K pieces (in order) + beta C
+template <typename T>
+__global__ void __launch_bounds__(256) k_split_reduce(const cplx<T>* __restrict__ W, int64_t ks,
+                                                      int64_t M, int64_t N, int64_t binner,
+                                                      cplx<T>* C, int64_t ldc, int64_t strideC,
+                                                      int64_t sCin, cplx<T> beta) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t z = blockIdx.y;
+  if (idx >= M * N) return;
+  const int64_t row = idx % M, col = idx / M;
+  const cplx<T>* P = W + z * ks * M * N + idx;
+  cplx<T> v{0, 0};
+  for (int64_t k = 0; k < ks; ++k) v = cadd(v, P[k * M * N]);
+  cplx<T>* c = C + (z / binner) * strideC + (z % binner) * sCin + row + col * ldc;
+  if (beta.x != T(0) || beta.y != T(0)) v = cadd(v, cmul(beta, *c));
+  *c = v;
+}
+
+template <typename T>
+int64_t gemm_ws_bytes(const GemmParams<T>& p) {
+  const int64_t bin = p.binner > 1 ? p.binner : 1;
+  if (p.m <= 0 || p.n <= 0 || p.batch <= 0) return 0;
+  if (p.dot_out != nullptr) return ceil_div(p.m, 64) * p.batch * bin * p.n * (int64_t)sizeof(double);
+  if (p.ksplit > 1) return p.ksplit * p.batch * bin * p.m * p.n * (int64_t)sizeof(cplx<T>);
+  return 0;
+}
+
+template <typename T> void launch_gemm_tiles(const GemmParams<T>& p, hipStream_t s);
 
 template <typename T>
 void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
@@ -266,6 +317,40 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
     throw std::runtime_error("launch_gemm: herm_dot needs the dot epilogue, opA = 'N', square A, one product");
   if (p.tri && (p.dot_out != nullptr || p.A2 != nullptr || p.ksplit > 1))
     throw std::runtime_error("launch_gemm: a triangular A takes one product, no dot epilogue, no split-K");
+  if (p.dot_out != nullptr && (ks > 1 || p.lower))
+    throw std::runtime_error("launch_gemm: the dot epilogue takes no split-K and no lower-only tiles");
+  if (gemm_ws_bytes(p) > 0 && p.ws == nullptr)
+    throw std::runtime_error("launch_gemm: the dot epilogue and split-K need a workspace");
+  if (ks > 1) {
+    // the pieces into the workspace, [(b binner + bi) ksplit + piece] x (M x N, ld M)
+    GemmParams<T> q = p;
+    q.C = static_cast<cplx<T>*>(p.ws);
+    q.ldc = p.m;
+    q.sCsplit = p.m * p.n;
+    q.sCin = ks * q.sCsplit;
+    q.strideC = bin * q.sCin;
+    q.beta = cplx<T>{0, 0};
+    launch_gemm_tiles(q, s);
+  } else {
+    launch_gemm_tiles(p, s);
+  }
+  if (p.dot_out != nullptr) {
+    dim3 g((unsigned)ceil_div(p.n, 256), (unsigned)(p.batch * bin));
+    hipLaunchKernelGGL(k_dot_reduce, g, dim3(256), 0, s, static_cast<const double*>(p.ws),
+                       ceil_div(p.m, 64), p.n, bin, p.dot_out, p.dot_stride, p.sDotin);
+    PPH_CHECK_LAUNCH();
+  } else if (ks > 1) {
+    dim3 g((unsigned)ceil_div(p.m * p.n, 256), (unsigned)(p.batch * bin));
+    hipLaunchKernelGGL(k_split_reduce<T>, g, dim3(256), 0, s, static_cast<const cplx<T>*>(p.ws), ks,
+                       p.m, p.n, bin, p.C, p.ldc, p.strideC, p.sCin, p.beta);
+    PPH_CHECK_LAUNCH();
+  }
+}
+
+// the tile kernels of launch_gemm (arguments checked by the caller)
+template <typename T>
+void launch_gemm_tiles(const GemmParams<T>& p, hipStream_t s) {
+  const int64_t ks = p.ksplit > 1 ? p.ksplit : 1, bin = p.binner > 1 ? p.binner : 1;
   // a K step must not straddle the two products
   if (p.A2 != nullptr && p.k1 % GBK != 0) throw std::runtime_error("launch_gemm: k1 must be a multiple of 16");
   const bool ca = (p.opA == 'C'), cb = (p.opB == 'C'), dot = (p.dot_out != nullptr);
@@ -320,5 +405,7 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
 
 template void launch_gemm<double>(const GemmParams<double>&, hipStream_t);
 template void launch_gemm<float>(const GemmParams<float>&, hipStream_t);
+template int64_t gemm_ws_bytes<double>(const GemmParams<double>&);
+template int64_t gemm_ws_bytes<float>(const GemmParams<float>&);
 
 }  // namespace pph

@@ -52,7 +52,9 @@ void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk
                         const double* phit, const cplx<double>* bph, int op0, int nops,
                         const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
                         int64_t batch, double* out0, int64_t out0_b, double* out1,
-                        int64_t out1_op, int64_t out1_b, hipStream_t s);
+                        int64_t out1_op, int64_t out1_b, double* ws, hipStream_t s);
+// bytes of the workspace launch_sparse_dots needs (per-workgroup partials)
+int64_t sparse_dots_ws_bytes(int64_t N, int64_t M, int64_t batch, int nops);
 // Y = D X for k columns (n x k per matrix, ld n) from the listed blocks: row
 // lists (rowptr, bcol) and their transposes (tptr, trow, tblk: for atom a the
 // blocks (c, a) with c > a), phases bph (op 0) and real blocks phit
@@ -85,9 +87,10 @@ struct GemmParams {
   // inner batch: blockIdx.z = (outer * binner + inner) * ksplit + split; operand
   // offsets add inner * sXin to the outer strides (binner 0 or 1 = no inner batch)
   int64_t binner; int64_t sAin, sBin, sCin, sXin, sDotin;
-  // split-K: the K range is cut into ksplit pieces whose products are added
-  // into C atomically (requires beta = 0 and C cleared beforehand)
-  int64_t ksplit;
+  // split-K: the K range is cut into ksplit pieces; each piece's product goes
+  // to its own slice of ws and a second kernel adds the slices in order
+  // (C = sum + beta C).  sCsplit: offset of piece k's C (set by launch_gemm)
+  int64_t ksplit, sCsplit;
   // lower != 0: tiles lying entirely above the diagonal of C are skipped
   int lower;
   // diagonal-dot epilogue: when dot_out != null, C is not written; instead
@@ -110,7 +113,13 @@ struct GemmParams {
   // four_m != 0: four real products per complex step instead of three (no
   // cancellation in the imaginary part; a third more matrix-core work)
   int four_m;
+  // device workspace of gemm_ws_bytes(p) bytes when that is nonzero: the dot
+  // epilogue's per-row-tile partials or split-K's per-piece products, summed
+  // in a fixed order so the results do not depend on scheduling
+  void* ws;
 };
+
+template <typename T> int64_t gemm_ws_bytes(const GemmParams<T>& p);
 
 template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);
 
@@ -123,8 +132,13 @@ struct TridiagWork {
   cplx<T>* W;  int64_t strideW;       // n x nb panel W
   cplx<T>* tau; int64_t strideTau;    // n
   T* d; T* e; int64_t strideDE;       // diagonal / off-diagonal (n each)
-  cplx<T>* ypart; int64_t strideY;    // y = A v accumulator: n
-  cplx<T>* dpart; int64_t strideDp;   // correction dots W^H v, V^H v: 2 nb
+  // per-workgroup partials of the hemv, summed in a fixed order by k_fin:
+  // ypart[b * strideY + slot * ldY + l] (l = local row of the trailing matrix),
+  // slots 0 .. ntmax - 1 row sums of segment ys, ntmax .. 2 ntmax - 1 column
+  // sums of tile row I; dpart[b * strideDp + I * 2 nb + k] the correction dots
+  // W^H x, V^H x of tile row I
+  cplx<T>* ypart; int64_t strideY, ldY, ntmax;
+  cplx<T>* dpart; int64_t strideDp;
   double* scal; int64_t strideScal;   // per-column scalars (tau, scale, alpha2 of the panel)
   double* part; int64_t strideP;      // per-workgroup partials: |x|^2 at [0, n/64 + 1),
   int64_t yvOff;                      //   w^H v (re, im pairs) from part[yvOff]
@@ -135,6 +149,8 @@ struct TridiagWork {
 constexpr int64_t kHemvTimeStride = 8;
 
 template <typename T> void tridiagonalise(const TridiagWork<T>& w, hipStream_t s);
+// hemv tiles of the trailing matrix at column 0 (the most any column has)
+inline int64_t hemv_tiles(int64_t n) { return (n - 1 + 15 + 63) / 64; }
 
 // ---------------------------------------------------------------- two-stage reduction (twostage.hip)
 // Dense -> band of half-bandwidth bw (he2hb: panel QR of bw columns, then the
@@ -229,7 +245,21 @@ struct BackWork {
   int64_t shift = 1;
   // Z already holds the vectors to transform (the Z_T conversion skipped)
   bool z_ready = false;
+  // workspace of back_transform_ws_bytes bytes (split-K pieces of Wt = V^H Z)
+  void* ws = nullptr;
 };
+
+// split-K pieces of the product Wt = V^H Z over mr rows for M vectors of B
+// matrices: enough workgroups to fill the GPU when the output tiles are few
+inline int64_t back_ksplit(int64_t mr, int64_t M, int64_t B) {
+  const int64_t tiles = ((M + 63) / 64) * B;
+  return std::max<int64_t>(1, std::min<int64_t>((mr + 255) / 256, (1024 + tiles - 1) / tiles));
+}
+template <typename T>
+int64_t back_transform_ws_bytes(int64_t n, int64_t B, int64_t nb, int64_t M, int64_t shift) {
+  const int64_t ks = back_ksplit(n - shift, M, B);
+  return ks > 1 ? ks * B * nb * M * (int64_t)sizeof(cplx<T>) : 0;
+}
 
 template <typename T> void back_transform(const BackWork<T>& w, hipStream_t s);
 

@@ -23,7 +23,9 @@ constexpr int kDotAtoms = 16;      // atom rows per workgroup
 
 // grid: x = 256 modes, y = kDotAtoms atom rows, z = q-point.  Thread = mode i,
 // vectors from the mode-contiguous copy Zt (row r of matrix b at
-// b * strideT + r * ldt).
+// b * strideT + r * ldt).  Each workgroup stores its sums as partials
+// part[((b * NOPS + o) * gridDim.y + blockIdx.y) * M + i]; k_dots_reduce adds
+// them over the atom chunks in order.
 template <typename Tz, int NOPS>
 __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__ rowptr,
                                                      const int32_t* __restrict__ bcol, int64_t nblk,
@@ -31,9 +33,7 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
                                                      const cplx<double>* __restrict__ bph, int op0,
                                                      const cplx<Tz>* __restrict__ Zt, int64_t ldt,
                                                      int64_t strideT, int64_t N, int64_t M,
-                                                     double* __restrict__ out0, int64_t out0_b,
-                                                     double* __restrict__ out1, int64_t out1_op,
-                                                     int64_t out1_b) {
+                                                     double* __restrict__ part) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.z;
   if (i >= M) return;
@@ -85,11 +85,30 @@ __global__ void __launch_bounds__(256) k_sparse_dots(const int64_t* __restrict__
     }
   }
 #pragma unroll
-  for (int o = 0; o < NOPS; ++o) {
-    const int op = op0 + o;
-    double* dst = (op == 0) ? out0 + b * out0_b + i : out1 + b * out1_b + (op - 1) * out1_op + i;
-    atomicAdd(dst, acc[o]);
-  }
+  for (int o = 0; o < NOPS; ++o)
+    part[((b * NOPS + o) * (int64_t)gridDim.y + blockIdx.y) * M + i] = acc[o];
+}
+
+// out(op) += sum over the atom chunks of the partials, in order
+__global__ void __launch_bounds__(256) k_dots_reduce(const double* __restrict__ part, int64_t ny,
+                                                     int64_t M, int op0, int nops,
+                                                     double* __restrict__ out0, int64_t out0_b,
+                                                     double* __restrict__ out1, int64_t out1_op,
+                                                     int64_t out1_b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int o = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  if (i >= M) return;
+  const double* P = part + (b * nops + o) * ny * M + i;
+  double acc = 0.0;
+  for (int64_t y = 0; y < ny; ++y) acc += P[y * M];
+  const int op = op0 + o;
+  double* dst = (op == 0) ? out0 + b * out0_b + i : out1 + b * out1_b + (op - 1) * out1_op + i;
+  *dst += acc;
+}
+
+int64_t sparse_dots_ws_bytes(int64_t N, int64_t M, int64_t batch, int nops) {
+  return ceil_div(N, kDotAtoms) * M * batch * nops * (int64_t)sizeof(double);
 }
 
 template <typename Tz>
@@ -97,31 +116,35 @@ void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk
                         const double* phit, const cplx<double>* bph, int op0, int nops,
                         const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
                         int64_t batch, double* out0, int64_t out0_b, double* out1,
-                        int64_t out1_op, int64_t out1_b, hipStream_t s) {
+                        int64_t out1_op, int64_t out1_b, double* ws, hipStream_t s) {
   if (M <= 0 || batch <= 0 || N <= 0) return;
+  if (ws == nullptr) throw std::runtime_error("launch_sparse_dots: missing workspace");
   if (op0 < 0 || op0 + nops > 4) throw std::runtime_error("launch_sparse_dots: operators out of range");
   if ((op0 == 0 && !out0) || (op0 + nops > 1 && !out1))
     throw std::runtime_error("launch_sparse_dots: missing output");
   dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, kDotAtoms), (unsigned)batch);
 #define PPH_DOTS(K)                                                                          \
   hipLaunchKernelGGL((k_sparse_dots<Tz, K>), grid, dim3(256), 0, s, rowptr, bcol, nblk, phit, \
-                     bph, op0, Zt, ldt, strideT, N, M, out0, out0_b, out1, out1_op, out1_b)
+                     bph, op0, Zt, ldt, strideT, N, M, ws)
   if (nops == 1) PPH_DOTS(1);
   else if (nops == 3) PPH_DOTS(3);
   else if (nops == 4) PPH_DOTS(4);
   else throw std::runtime_error("launch_sparse_dots: nops must be 1, 3 or 4");
 #undef PPH_DOTS
   PPH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_dots_reduce, dim3(grid.x, (unsigned)nops, (unsigned)batch), dim3(256), 0, s, ws,
+                     (int64_t)grid.y, M, op0, nops, out0, out0_b, out1, out1_op, out1_b);
+  PPH_CHECK_LAUNCH();
 }
 
 template void launch_sparse_dots<float>(const int64_t*, const int32_t*, int64_t, const double*,
                                         const cplx<double>*, int, int, const cplx<float>*, int64_t,
                                         int64_t, int64_t, int64_t, int64_t, double*, int64_t,
-                                        double*, int64_t, int64_t, hipStream_t);
+                                        double*, int64_t, int64_t, double*, hipStream_t);
 template void launch_sparse_dots<double>(const int64_t*, const int32_t*, int64_t, const double*,
                                          const cplx<double>*, int, int, const cplx<double>*,
                                          int64_t, int64_t, int64_t, int64_t, int64_t, double*,
-                                         int64_t, double*, int64_t, int64_t, hipStream_t);
+                                         int64_t, double*, int64_t, int64_t, double*, hipStream_t);
 
 // Y = D X for k columns of X (n x k per matrix, column-major, ld n): thread =
 // (atom row a, column c).  Row a of D is the listed blocks (a, b), b <= a,

@@ -10,10 +10,12 @@
 //             sums of |x|^2.
 //   k_hemv  : u = S x for the trailing matrix S = A(j+1:, j+1:) and the
 //             unscaled column x = (0, A(j+2:, j)) over a 2-D grid of tile rows
-//             x column segments, accumulated with device atomics, plus the
-//             dots W^H x, V^H x; workgroup (0, 0) forms the reflector (beta,
+//             x column segments, stored as per-workgroup partials (row sums
+//             per segment, column sums per tile row), plus the dots W^H x,
+//             V^H x per tile row; workgroup (0, 0) forms the reflector (beta,
 //             tau, scale = 1/(alpha-beta)) from k_col's partials.
-//   k_fin   : with v = e_0 + scale x: y = S(:, 0) + scale u, the dots
+//   k_fin   : sums the hemv's partials in a fixed order, then with
+//             v = e_0 + scale x: y = S(:, 0) + scale u, the dots
 //             W^H v = conj(W(j+1, :)) + scale W^H x (same for V), then
 //             y - V (W^H v) - W (V^H v), w = tau y; writes v into V and
 //             per-workgroup partials of w^H v.
@@ -21,7 +23,8 @@
 // the reflector.
 // Every cross-workgroup reduction ends at a kernel boundary: producers store
 // per-workgroup partials and the next kernel sums them in a fixed order, so no
-// kernel needs device-scope fences and the sums are deterministic.
+// kernel needs device-scope fences or atomics, and the results repeat bit for
+// bit from run to run.
 // The trailing matrix is kept correct in its lower triangle only (the hemv
 // reads the lower triangle and uses it twice).
 // The final w = tau y + alpha2 v is never written separately: every consumer
@@ -186,10 +189,6 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
   }
   __syncthreads();
 
-  // clear the accumulators k_hemv_sym adds into (y over rows j+1.., dots 2 nb)
-  if (sub == 0 && r < n) w.ypart[b * w.strideY + r] = cplx<T>{0, 0};
-  if (blockIdx.x == 0)
-    for (int k = threadIdx.x; k < 2 * (int)w.nb; k += blockDim.x) w.dpart[b * w.strideDp + k] = cplx<T>{0, 0};
   cplx<T> acc{0, 0};
   if (r < n && sub < jj) {
     // V(r,k) conj(W(j,k)) + W(r,k) conj(V(j,k))
@@ -219,9 +218,11 @@ __global__ void __launch_bounds__(256) k_col(TridiagWork<T> w, int64_t j, int jj
 // row, wave w = columns 16w..16w+15 of every tile; the next tile's elements
 // are loaded while the current one is used.  Row sums stay in registers; the
 // 16 column sums of a wave are reduced across lanes by a butterfly that halves
-// the number of live columns at each step.  Results go to the accumulator
-// with device atomics.  Blocks of segment 0 also form the correction dots
-// W^H x, V^H x for their rows.
+// the number of live columns at each step.  Results are stored as partials:
+// the row sums of the workgroup in slot ys = blockIdx.y, the column sums of
+// tile (I, J) in slot ntmax + I (each (slot, row) is written by exactly one
+// workgroup).  Blocks of segment 0 also store the correction dots W^H x,
+// V^H x of their tile row.
 // Tiles start `sh` rows (and columns) before j + 1, at a multiple of 16: with
 // lda a multiple of 16 every 64-row column segment then covers exactly four
 // 128-byte lines instead of straddling five; local index l = o + sh for
@@ -328,7 +329,8 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
   const cplx<T>* A = w.A + b * w.strideA;
   const cplx<T>* S = A + r0 + r0 * w.lda;
   const cplx<T>* V = w.V + b * w.strideV;
-  cplx<T>* y = w.ypart + b * w.strideY + r0;
+  cplx<T>* yrow = w.ypart + b * w.strideY + (int64_t)blockIdx.y * w.ldY;
+  cplx<T>* ycol = w.ypart + b * w.strideY + (w.ntmax + I) * w.ldY;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const int64_t rI = (int64_t)I * 64 + lane;          // this lane's local row
   const bool rowok = rI >= sh && rI < mL;
@@ -392,18 +394,12 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
     T r1, i1;
     reduce_scatter16(pr, pi, h3, h2, r1, i1);
     const int64_t cJ = (int64_t)J * 64 + mycol;
-    if ((lane & 3) == 0 && cJ >= sh && cJ < mL) {
-      atomicAdd(&y[cJ].x, r1);
-      atomicAdd(&y[cJ].y, i1);
-    }
+    if ((lane & 3) == 0 && cJ >= sh && cJ < mL) ycol[cJ] = cplx<T>{r1, i1};
   }
   red[wv][lane] = rowacc;
   __syncthreads();
-  if (wv == 0 && rowok) {
-    const cplx<T> s = cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane]));
-    atomicAdd(&y[rI].x, s.x);
-    atomicAdd(&y[rI].y, s.y);
-  }
+  if (wv == 0 && rowok)
+    yrow[rI] = cadd(cadd(red[0][lane], red[1][lane]), cadd(red[2][lane], red[3][lane]));
   if (blockIdx.y == 0 && jj > 0) {
     // correction dots W_eff^H x (k < jj), V^H x (jj <= k < 2 jj) over this
     // tile row, 64 at a time (2 jj < 128): wave wv takes k = k0 + wv + 4 i,
@@ -427,11 +423,8 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
       T r1, i1;
       reduce_scatter16(dr, di, h3, h2, r1, i1);
       const int kk = k0 + wv + 4 * ((h5 ? 8 : 0) + (h4 ? 4 : 0) + (h3 ? 2 : 0) + (h2 ? 1 : 0));
-      if ((lane & 3) == 0 && kk < 2 * jj) {
-        cplx<T>* d = w.dpart + b * w.strideDp + kk;
-        atomicAdd(&d->x, r1);
-        atomicAdd(&d->y, i1);
-      }
+      if ((lane & 3) == 0 && kk < 2 * jj)
+        w.dpart[b * w.strideDp + (int64_t)I * 2 * w.nb + kk] = cplx<T>{r1, i1};
     }
   }
 }
@@ -439,9 +432,11 @@ __global__ void __launch_bounds__(256) k_hemv_sym(TridiagWork<T> w, int64_t j, i
 // ---------------------------------------------------------------- k_fin
 // 256 threads = 64 rows x 4 waves; wave s handles the panel terms k = s, s+4, ...
 template <typename T>
-__global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj, int64_t p) {
+__global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj, int64_t p,
+                                             int seg, int tsh) {
   __shared__ cplx<T> sW[64], sV[64];
   __shared__ cplx<T> red[4][64];
+  __shared__ double ured[4][64][2], dred[4][128][2];
   __shared__ double sh[8];
   const int64_t b = blockIdx.z;
   const int64_t n = w.n, m = n - j - 1;
@@ -457,20 +452,65 @@ __global__ void __launch_bounds__(256) k_fin(TridiagWork<T> w, int64_t j, int jj
   // u(r) and x(r) = A(r, j)
   PanelRow<T> prow;
   load_panel_row(w, b, r, sub, jj, p, rr < m && sub < jj, prow);
-  cplx<T> s0{0, 0}, ur{0, 0}, xr{0, 0};
+  cplx<T> s0{0, 0}, xr{0, 0};
   if (sub == 0 && rr < m) {
     s0 = A[r + (j + 1) * w.lda];
-    ur = w.ypart[b * w.strideY + r];
     xr = A[r + j * w.lda];
   }
-
-  // W^H v = conj(W_eff(j+1, k)) + scale (W^H x), V^H v likewise
-  for (int k = threadIdx.x; k < 2 * jj; k += blockDim.x) {
-    const cplx<T> s = w.dpart[b * w.strideDp + k];
-    if (k < jj) sW[k] = cadd(cconj(weff(w, b, j + 1, k, p, scal)), cmul(sc, s));
-    else sV[k - jj] = cadd(cconj(V[(j + 1) + (p + k - jj) * n]), cmul(sc, s));
+  // u(r) from the hemv's partials of local row l = rr + tsh in tile row tr: the
+  // row sums of segments 0 .. tr / seg and the column sums of tile rows
+  // tr .. nt - 1; wave `sub` takes terms sub, sub + 4, ... with eight loads in
+  // flight, summed in FP64 in a fixed order
+  const int64_t nt = (m + tsh + 63) / 64;
+  {
+    const int64_t l = rr + tsh, tr = l / 64;
+    const int64_t nrow = tr / seg + 1, nterm = (rr < m) ? nrow + (nt - tr) : 0;
+    const cplx<T>* P = w.ypart + b * w.strideY + (rr < m ? l : 0);
+    double ux = 0.0, uy = 0.0;
+    for (int64_t t0 = sub; t0 < nterm; t0 += 32) {
+      cplx<T> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t t = min(t0 + 4 * u, nterm - 1);
+        const int64_t slot = (t < nrow) ? t : w.ntmax + tr + (t - nrow);
+        v[u] = P[slot * w.ldY];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (t0 + 4 * u < nterm) { ux += (double)v[u].x; uy += (double)v[u].y; }
+    }
+    ured[sub][lane][0] = ux;
+    ured[sub][lane][1] = uy;
+  }
+  // the correction dots W^H x, V^H x (2 jj <= 128 of them) summed over the
+  // tile rows: dot k = lane, lane + 64; wave `sub` takes tile rows sub,
+  // sub + 4, ..., the same way
+  for (int k = lane; k < 2 * jj; k += 64) {
+    const cplx<T>* D = w.dpart + b * w.strideDp + k;
+    double dx = 0.0, dy = 0.0;
+    for (int64_t I0 = sub; I0 < nt; I0 += 32) {
+      cplx<T> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = D[min(I0 + 4 * u, nt - 1) * 2 * w.nb];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (I0 + 4 * u < nt) { dx += (double)v[u].x; dy += (double)v[u].y; }
+    }
+    dred[sub][k][0] = dx;
+    dred[sub][k][1] = dy;
   }
   __syncthreads();
+  // W^H v = conj(W_eff(j+1, k)) + scale (W^H x), V^H v likewise
+  if (threadIdx.x < 2 * jj) {
+    const int k = threadIdx.x;
+    const cplx<T> sdot{(T)(((dred[0][k][0] + dred[1][k][0]) + dred[2][k][0]) + dred[3][k][0]),
+                       (T)(((dred[0][k][1] + dred[1][k][1]) + dred[2][k][1]) + dred[3][k][1])};
+    if (k < jj) sW[k] = cadd(cconj(weff(w, b, j + 1, k, p, scal)), cmul(sc, sdot));
+    else sV[k - jj] = cadd(cconj(V[(j + 1) + (p + k - jj) * n]), cmul(sc, sdot));
+  }
+  __syncthreads();
+  const cplx<T> ur{(T)(((ured[0][lane][0] + ured[1][lane][0]) + ured[2][lane][0]) + ured[3][lane][0]),
+                   (T)(((ured[0][lane][1] + ured[1][lane][1]) + ured[2][lane][1]) + ured[3][lane][1])};
 
   cplx<T> acc{0, 0};
   if (rr < m && sub < jj) {
@@ -550,6 +590,9 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
   const int64_t n = w.n;
   const unsigned B = (unsigned)w.batch;
   if (w.nb > 64 || w.nb % 16 != 0) throw std::runtime_error("tridiagonalise: nb must be 16..64, multiple of 16");
+  if (n > 1 && (w.ntmax < hemv_tiles(n) || w.ldY < w.ntmax * 64 || w.strideY < 2 * w.ntmax * w.ldY ||
+                w.strideDp < w.ntmax * 2 * w.nb))
+    throw std::runtime_error("tridiagonalise: hemv partial buffers too small");
   hipLaunchKernelGGL(k_reset_scal<T>, dim3(B), dim3(256), 0, s, w);
   PPH_CHECK_LAUNCH();
   // V: explicit Householder vectors, zero above the unit entry, n + 64 columns
@@ -579,7 +622,7 @@ void tridiagonalise(const TridiagWork<T>& w, hipStream_t s) {
       PPH_CHECK_LAUNCH();
       if (timed) PPH_HIP(hipEventRecord(w.hemv_ev[2 * (j / kHemvTimeStride) + 1], s));
       hipLaunchKernelGGL(k_fin<T>, dim3((unsigned)ceil_div(m, 64), 1, B), dim3(256), 0, s,
-                         w, j, jj, p);
+                         w, j, jj, p, seg, sh);
       PPH_CHECK_LAUNCH();
     }
     const int64_t r0 = p + pw;
