@@ -6,16 +6,22 @@ pipeline.
     loss = (w * omega).sum(); loss.backward()           # q.grad, params.grad
 
 Forward: force constants from the parameters (the model is linear in them,
-parphom_amd.synth), then pph_run for frequencies, eigenvectors and the
-handle reused for the adjoint.  Backward (Hellmann-Feynman):
+parphom_amd.synth), then pph_run_device: frequencies and eigenvectors stay in
+GPU memory (torch tensors the library writes), and only the frequencies come
+back to the host.  Backward (Hellmann-Feynman), on the device:
 d lambda_k / dx = <e_k| dD/dx |e_k> for x = a Cartesian q component
 (pph_expect with what = 1..3) or a parameter (pph_expect on a handle built
-from that parameter's basis force constants, dD/d theta = D[basis]).
+from that parameter's basis force constants, dD/d theta = D[basis]), with the
+eigenvectors read where the forward left them.
 Inside clusters of degenerate eigenvalues the derivative of the sorted
 eigenvalues is the sorted spectrum of the projected block Z_c^H dD/dx Z_c
-(degenerate-subspace projection).  d omega / d lambda = K^2 / max(2|omega|,
-1e-3) with K = f32(15.633302) f32(33.35641), the conversion of
-write_output.F90:135 and the floor of write_output.F90:443.
+(degenerate-subspace projection).  Its off-diagonal entries come from the
+same device product by polarisation, B_ab = ((a + b)^H X (a + b) - B_aa -
+B_bb) / 2 + i (B_aa + B_bb - (a + i b)^H X (a + i b)) / 2, on vectors formed
+on the GPU; only the k x k eigenvalue problems are solved on the host.
+d omega / d lambda = K^2 / max(2|omega|, 1e-3) with K = f32(15.633302)
+f32(33.35641), the conversion of write_output.F90:135 and the floor of
+write_output.F90:443.
 Reduced q: dD/dq_red = 2 pi32 (lat^T)^-1 dD/dq_cart, dq_cart as the analytic
 derivative of dynamical_matrix.F90:249-253 defines it.
 """
@@ -87,29 +93,55 @@ class SpringPhonons:
 
 
 def _expect(h, q, Z, what):
-    """Re diag(Z^H X Z) per q; Z (nq, n, nz) complex128 host array."""
+    """Re diag(Z^H X Z) per q on the device: Z (nq, cols, n) complex128 CUDA
+    tensor (each q's vectors column-major, the layout pph_run_device writes),
+    X = D(q) (what 0) or dD/dq_{x,y,z} (what 1..3) of handle h; returns a
+    (nq, cols) float64 CUDA tensor."""
     import ctypes as C
-    nq, n, nz = Z.shape
-    Zc = np.ascontiguousarray(Z.transpose(0, 2, 1))          # column-major n x nz per q
+    nq, cols, n = Z.shape
+    if n != h.n or Z.dtype != torch.complex128 or not Z.is_cuda or not Z.is_contiguous():
+        raise ValueError("_expect: Z must be a contiguous complex128 CUDA tensor (nq, cols, n)")
     qf = np.asfortranarray(np.atleast_2d(np.asarray(q, np.float64)))
-    out = np.zeros((nq, nz))
-    lib._check(lib.load().pph_expect(h._h, nq, lib._ptr(qf), nq, what, Zc.ctypes.data, nz, nz,
-                                     lib._ptr(out), 0), h._h)
+    out = torch.empty((nq, cols), dtype=torch.float64, device=Z.device)
+    lib._check(lib.load().pph_expect(h._h, nq, lib._ptr(qf), nq, what, C.c_void_p(Z.data_ptr()),
+                                     cols, cols,
+                                     C.cast(C.c_void_p(out.data_ptr()), C.POINTER(C.c_double)), 1),
+               h._h)
     return out
 
 
 def _dlam(h, q, Z, lam, what, cluster_rel):
-    """d lambda / dx for every mode, with projected blocks inside clusters."""
-    d = _expect(h, q, Z, what)
-    for iq in range(Z.shape[0]):
-        groups = _clusters(lam[iq], cluster_rel)
-        if not groups:
-            continue
-        X = h.assemble(q[iq:iq + 1], what)[0]
-        for g in groups:
-            Zc = Z[iq][:, g]
-            blk = Zc.conj().T @ (X @ Zc)
-            d[iq, g] = np.sort(np.linalg.eigvalsh(0.5 * (blk + blk.conj().T)))
+    """d lambda / dx for every mode (nq, n) on the host, with the projected
+    blocks inside clusters of degenerate eigenvalues."""
+    d = _expect(h, q, Z, what).cpu().numpy()
+    groups = [_clusters(lam[iq], cluster_rel) for iq in range(Z.shape[0])]
+    qs = [iq for iq in range(Z.shape[0]) if groups[iq]]
+    if not qs:
+        return d
+    # polarisation vectors of every pair inside every cluster, per q
+    pairs = {iq: [(a, b) for g in groups[iq] for ia, a in enumerate(g) for b in g[ia + 1:]]
+             for iq in qs}
+    npol = 2 * max(len(v) for v in pairs.values())
+    P = torch.zeros((len(qs), npol, Z.shape[2]), dtype=torch.complex128, device=Z.device)
+    for r, iq in enumerate(qs):
+        a = torch.tensor([x for x, _ in pairs[iq]], device=Z.device)
+        b = torch.tensor([y for _, y in pairs[iq]], device=Z.device)
+        k = len(pairs[iq])
+        P[r, 0:2 * k:2] = Z[iq, a] + Z[iq, b]
+        P[r, 1:2 * k:2] = Z[iq, a] + 1j * Z[iq, b]
+    pol = _expect(h, np.asarray(q)[qs], P, what).cpu().numpy()
+    for r, iq in enumerate(qs):
+        off = {}
+        for t, (a, b) in enumerate(pairs[iq]):
+            s = d[iq, a] + d[iq, b]
+            off[(a, b)] = 0.5 * (pol[r, 2 * t] - s) + 0.5j * (s - pol[r, 2 * t + 1])
+        for g in groups[iq]:
+            blk = np.diag(d[iq, g]).astype(np.complex128)
+            for ia, a in enumerate(g):
+                for ib in range(ia + 1, len(g)):
+                    blk[ia, ib] = off[(a, g[ib])]
+                    blk[ib, ia] = np.conj(off[(a, g[ib])])
+            d[iq, g] = np.sort(np.linalg.eigvalsh(blk))
     return d
 
 
@@ -118,12 +150,21 @@ class _Frequencies(torch.autograd.Function):
     def forward(ctx, q, params, model):
         qn = q.detach().cpu().numpy().astype(np.float64)
         pn = params.detach().cpu().numpy().astype(np.float64)
+        nq, n = len(qn), model.n
+        dev = torch.device("cuda", 0)
+        out = {"eval_cm": torch.zeros((nq, n), dtype=torch.float64, device=dev),
+               "m": torch.zeros(nq, dtype=torch.int64, device=dev),
+               "nz": torch.zeros(nq, dtype=torch.int64, device=dev),
+               "info": torch.zeros(nq, dtype=torch.int32, device=dev),
+               "evec": torch.empty((nq, n, n), dtype=torch.complex128, device=dev)}
         with model.handle(pn, jobz="V", precision=model.precision) as h:
-            res = h.run(qn)
-        om = np.stack(res["eval_cm"])                          # (nq, n)
-        Z = np.stack(res["evec"])                              # (nq, n, n)
+            h.run_device(qn, out)
+        torch.cuda.synchronize()
+        if int(out["m"].min()) != n or int(out["info"].abs().max()) != 0:
+            raise RuntimeError("frequencies: the eigensolve did not return every mode")
+        om = out["eval_cm"].cpu().numpy()                      # (nq, n)
         ctx.model = model
-        ctx.qn, ctx.pn, ctx.om, ctx.Z = qn, pn, om, Z
+        ctx.qn, ctx.pn, ctx.om, ctx.Z = qn, pn, om, out["evec"]
         return torch.from_numpy(om.copy())
 
     @staticmethod
