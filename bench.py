@@ -96,7 +96,7 @@ def velocity_difference(v, v_ref, om_ref, rel=1e-6):
     """Largest difference of the velocities of two runs on the same q, through
     sums over clusters of eigenvalues closer than rel * max|omega^2| (inside a
     degenerate cluster the individual velocities depend on the basis; their
-    sum does not).  Returns (max abs difference m/s, same / max |v_ref|)."""
+    sum does not).  Returns (max abs difference m/s, max |v_ref|)."""
     lam = np.sign(om_ref) * om_ref ** 2
     tol = rel * np.abs(lam).max()
     groups, cur = [], [0]
@@ -108,7 +108,7 @@ def velocity_difference(v, v_ref, om_ref, rel=1e-6):
             cur = [k]
     groups.append(cur)
     d = max(float(np.abs(v[g].sum(axis=0) - v_ref[g].sum(axis=0)).max()) for g in groups)
-    return d, d / float(np.abs(v_ref).max())
+    return d, float(np.abs(v_ref).max())
 
 
 def split_profile(prof, n):
@@ -604,7 +604,9 @@ def main():
         # through sums over degenerate clusters (gauge-invariant), all modes
         dv = [velocity_difference(r32["vel"][i], r64["vel"][i], r64["ev"][i]) for i in range(k)]
         res["max_abs_dvel_vs_fp64_m_per_s"] = max(x[0] for x in dv)
-        res["max_rel_dvel_vs_fp64"] = max(x[1] for x in dv)
+        # relative to the largest velocity over the compared q-points (at
+        # Gamma every velocity is zero)
+        res["max_rel_dvel_vs_fp64"] = max(x[0] for x in dv) / max(1e-300, max(x[1] for x in dv))
         # whole-path roofline of both paths, against the spec-sheet matrix peaks
         # and against what the library GEMM reaches on this device
         p32, p64 = gemm_peak(torch, torch.float32), gemm_peak(torch, torch.float64)
@@ -623,7 +625,7 @@ def main():
                 "reduction": "two-stage (band 32)" if ro["two_stage"] else "one-stage",
                 "value": a.steps * B * S / ro["dt"], "unit": "q-pts/s",
                 "max_abs_domega_vs_fp64_cm": float(np.abs(ro["ev"][:k] - r64["ev"][:k]).max()),
-                "max_rel_dvel_vs_fp64": max(x[1] for x in dvo),
+                "max_rel_dvel_vs_fp64": max(x[0] for x in dvo) / max(1e-300, max(x[1] for x in dvo)),
                 "phase_ms_one_batch_alone": {kk: round(v, 3) for kk, v in pho.items()}}
 
     if not a.no_cpu:

@@ -172,6 +172,7 @@ struct Lane {
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
   DevBuf gws, sws;         // partials of the GEMM dot epilogue / split-K and of the sparse forms
+  DevBuf meta;             // per-matrix offsets and counts of the batch's outputs
   // two-stage reduction: band, bulge-chasing reflectors and their block
   // factors, panel T / Mx, Gram partials, stall flags
   DevBuf tsAB, tsV2, tsTau2, tsVTb, tsTp, tsG, tsInfo;
@@ -347,6 +348,7 @@ struct EigResult {
   std::vector<int64_t> M;     // per matrix
   std::vector<int64_t> first; // per matrix: index of the first returned mode
   std::vector<int32_t> info;
+  const int* info_dev = nullptr;  // device flags still to be or-ed into info (B ints), or null
   int64_t mcap = 0;           // leading dimension of w / columns of Z
   double* lam = nullptr;      // device, B x mcap (lambda)
   cplx<double>* Z64 = nullptr;  // device, B x n x mcap (when vectors)
@@ -430,13 +432,7 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
   PPH_HIP(hipMemcpyAsync(c1.data(), C1, c1.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
   PPH_HIP(hipMemcpyAsync(c2.data(), C2, c2.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
   PPH_HIP(hipStreamSynchronize(s));
-  const double tnorm_proxy = [&] {
-    double m = 0;
-    for (int64_t b = 0; b < B; ++b)
-      for (int64_t k = 0; k < Mv[b]; ++k) m = std::max(m, std::fabs(lam[b * mcap + k]));
-    return m;
-  }();
-  const double ctol = 1e-7 * std::max(tnorm_proxy, 1e-300);
+
   // Ritz pairs of the low block, kept for the complement correction below
   std::vector<cd> Uh((size_t)B * Kp * Kp, 0.0);
   std::vector<double> thh((size_t)B * Kp, 0.0);
@@ -449,6 +445,10 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
     double* lb = lam.data() + b * mcap;
     const double* nb = nrm.data() + b * mcap;
     std::vector<double> newl(lb, lb + kt);
+    // modes within 1e-7 max|lambda| of this matrix of the last target join its cluster
+    double tnorm_proxy = 0.0;
+    for (int64_t k = 0; k < M; ++k) tnorm_proxy = std::max(tnorm_proxy, std::fabs(lb[k]));
+    const double ctol = 1e-7 * std::max(tnorm_proxy, 1e-300);
     // one Rayleigh-Ritz block over all kt target modes: near-degenerate low
     // modes (the acoustic triplet at Gamma) may be closer to each other than
     // the FP32 residual, where a per-mode second-order correction is not
@@ -525,6 +525,51 @@ static void refine_low_modes(Handle& h, Lane& L, int64_t B, const cplx<double>* 
   }
   PPH_HIP(hipMemcpyAsync(lam_dev, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, s));
   PPH_HIP(hipStreamSynchronize(s));
+}
+
+// refine_low_modes for a batch whose matrices return all n modes (range A),
+// with the selection, the k x k algebra and the correction sums on the GPU
+// (refine.hip): the products with the lowest 16 vectors run for every batch,
+// matrices without modes below the threshold leave the kernel at once.  No
+// complement share: with every mode computed the sums are complete.
+static void refine_low_modes_dev(Handle& h, Lane& L, int64_t B, const cplx<double>* D64,
+                                 cplx<double>* X, double* lam_dev, const double* nrm2_dev,
+                                 int64_t mcap, hipStream_t s) {
+  const int64_t n = h.n;
+  const double K32 = (double)(float)(15.633302f * 33.35641f);
+  const double lam_low = (50.0 / K32) * (50.0 / K32);
+  const int64_t Kp = 16, K = std::min<int64_t>(Kp, n);
+  L.WT.ensure((size_t)B * n * Kp * sizeof(cplx<double>));
+  L.C12.ensure((size_t)B * 2 * mcap * Kp * sizeof(cplx<double>));
+  cplx<double>* WT = L.WT.as<cplx<double>>();
+  cplx<double>* C1 = L.C12.as<cplx<double>>();
+  cplx<double>* C2 = C1 + (size_t)B * mcap * Kp;
+  if (h.use_sparse()) {                      // W_T = D X(:, 0:K) from the listed blocks
+    launch_sparse_spmm(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_tptr.as<int64_t>(),
+                       h.sp_trow.as<int32_t>(), h.sp_tblk.as<int64_t>(), h.sp_nblk,
+                       h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), X, n * mcap, WT, n * Kp,
+                       h.N, K, B, s);
+  } else {
+    GemmParams<double> g{};                 // W_T = D X(:, 0:K)
+    g.opA = 'N'; g.opB = 'N'; g.m = n; g.n = K; g.k = n;
+    g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
+    g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
+    g.B = X; g.ldb = n; g.strideB = n * mcap;
+    g.C = WT; g.ldc = n; g.strideC = n * Kp;
+    g.batch = B;
+    launch_gemm<double>(g, s);
+  }
+  for (int which = 0; which < 2; ++which) {   // C1 = X^H W_T, C2 = X^H X_T
+    GemmParams<double> g{};
+    g.opA = 'C'; g.opB = 'N'; g.m = n; g.n = K; g.k = n;
+    g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
+    g.A = X; g.lda = n; g.strideA = n * mcap;
+    g.B = which == 0 ? WT : X; g.ldb = n; g.strideB = which == 0 ? n * Kp : n * mcap;
+    g.C = which == 0 ? C1 : C2; g.ldc = mcap; g.strideC = mcap * Kp;
+    g.batch = B;
+    launch_gemm<double>(g, s);
+  }
+  launch_low_rr(lam_dev, nrm2_dev, C1, C2, n, mcap, lam_low, B, s);
 }
 
 // The complement share of the second-order correction for the Ritz pairs
@@ -730,17 +775,25 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   te.want_vectors = vectors ? 1 : 0;
   L.tm.begin(PH_TRIEIG, s);
   tri_prepare(te, s);
-  std::vector<int64_t> ilo(B), ihi(B);
-  PPH_HIP(hipMemcpyAsync(ilo.data(), te.ilo, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  PPH_HIP(hipMemcpyAsync(ihi.data(), te.ihi, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  // range A: every matrix returns its n modes from the bottom, so the counts
+  // are known here and the batch runs without a host round trip (a stalled
+  // bulge-chasing pipeline's flags stay on the device until the outputs are
+  // written); ranges V and I read the Sturm-count ranges back
+  const bool all_modes = o.range == 'A';
+  std::vector<int64_t> ilo(B, 0), ihi(B, n);
   std::vector<int> rinfo(B, 0);
-  if (two)
-    PPH_HIP(hipMemcpyAsync(rinfo.data(), ts.info, B * sizeof(int), hipMemcpyDeviceToHost, s));
-  PPH_HIP(hipStreamSynchronize(s));
+  if (!all_modes) {
+    PPH_HIP(hipMemcpyAsync(ilo.data(), te.ilo, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PPH_HIP(hipMemcpyAsync(ihi.data(), te.ihi, B * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (two)
+      PPH_HIP(hipMemcpyAsync(rinfo.data(), ts.info, B * sizeof(int), hipMemcpyDeviceToHost, s));
+    PPH_HIP(hipStreamSynchronize(s));
+  }
   R.M.assign(B, 0);
   R.first.assign(B, 0);
   // a stalled bulge-chasing pipeline reports as a convergence failure
   R.info.assign(rinfo.begin(), rinfo.end());
+  R.info_dev = (all_modes && two) ? ts.info : nullptr;
   int64_t mmax = 0;
   for (int64_t b = 0; b < B; ++b) {
     R.M[b] = std::min(mcap, ihi[b] - ilo[b]);
@@ -855,8 +908,11 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
                               mcap, s);
       launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
-      refine_low_modes(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), R.M, ilo,
-                       mcap, s);
+      if (all_modes)
+        refine_low_modes_dev(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), mcap, s);
+      else
+        refine_low_modes(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), R.M, ilo,
+                         mcap, s);
       L.tm.end(PH_REFINE, s);
       R.Z64 = L.Z64.as<cplx<double>>();
       if (o.range == 'V') {
@@ -1274,8 +1330,11 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
       const EigResult& R = bo.R;
       std::vector<int32_t> infos(B);
       std::vector<int64_t> nzs(B);
+      // per matrix, in doubles: offset and count of the eigenvalues, the
+      // eigenvectors and the velocities to return
+      std::vector<int64_t> meta(6 * B, 0);
+      int64_t mmax = 0, nzmax = 0;
       for (int64_t b = 0; b < B; ++b) {
-        const int64_t iq = q0 + b;
         const int64_t m = R.M[b];
         int32_t info = R.info[b];
         int64_t nz = 0;
@@ -1288,28 +1347,56 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
         infos[b] = info;
         nzs[b] = nz;
         const int64_t f0 = b * R.mcap + R.first[b];      // first returned mode
-        if (m > 0)
-          PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + f0, m * sizeof(double),
-                                 kind, s));
-        if (out->evec && nz > 0)
-          PPH_HIP(hipMemcpyAsync((char*)out->evec + (size_t)iq * n * out->evec_cols * 16,
-                                 R.Z64 + (size_t)f0 * n, (size_t)nz * n * 16, kind, s));
-        if (out->vel && nz > 0)
-          PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + f0 * 3,
-                                 (size_t)nz * 3 * sizeof(double), kind, s));
+        meta[b] = f0;                 meta[B + b] = m;
+        meta[2 * B + b] = f0 * n * 2; meta[3 * B + b] = nz * n * 2;
+        meta[4 * B + b] = f0 * 3;     meta[5 * B + b] = nz * 3;
+        mmax = std::max(mmax, m);
+        nzmax = std::max(nzmax, nz);
       }
+      std::vector<int32_t> dflags;
       if (device_out) {
+        // one gather launch per output array for the whole batch
+        L.meta.ensure(meta.size() * sizeof(int64_t));
+        int64_t* md = L.meta.as<int64_t>();
+        PPH_HIP(hipMemcpyAsync(md, meta.data(), meta.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        launch_gather_rows(bo.om, md, md + B, out->eval_cm + q0 * out->ld_eval, out->ld_eval, mmax, B, s);
+        if (out->evec && nzmax > 0)
+          launch_gather_rows(reinterpret_cast<const double*>(R.Z64), md + 2 * B, md + 3 * B,
+                             reinterpret_cast<double*>(out->evec) + q0 * n * out->evec_cols * 2,
+                             n * out->evec_cols * 2, nzmax * n * 2, B, s);
+        if (out->vel && nzmax > 0)
+          launch_gather_rows(bo.vel, md + 4 * B, md + 5 * B, out->vel + q0 * out->evec_cols * 3,
+                             out->evec_cols * 3, nzmax * 3, B, s);
         PPH_HIP(hipMemcpyAsync(out->m + q0, R.M.data(), B * sizeof(int64_t), hipMemcpyHostToDevice, s));
         PPH_HIP(hipMemcpyAsync(out->info + q0, infos.data(), B * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        if (R.info_dev) launch_or_flags(out->info + q0, R.info_dev, B, s);
         if (out->nz) PPH_HIP(hipMemcpyAsync(out->nz + q0, nzs.data(), B * sizeof(int64_t), hipMemcpyHostToDevice, s));
       } else {
         for (int64_t b = 0; b < B; ++b) {
-          out->m[q0 + b] = R.M[b];
-          out->info[q0 + b] = infos[b];
-          if (out->nz) out->nz[q0 + b] = nzs[b];
+          const int64_t iq = q0 + b, f0 = meta[b], m = meta[B + b], nz = nzs[b];
+          if (m > 0)
+            PPH_HIP(hipMemcpyAsync(out->eval_cm + iq * out->ld_eval, bo.om + f0, m * sizeof(double),
+                                   kind, s));
+          if (out->evec && nz > 0)
+            PPH_HIP(hipMemcpyAsync((char*)out->evec + (size_t)iq * n * out->evec_cols * 16,
+                                   R.Z64 + (size_t)f0 * n, (size_t)nz * n * 16, kind, s));
+          if (out->vel && nz > 0)
+            PPH_HIP(hipMemcpyAsync(out->vel + (size_t)iq * out->evec_cols * 3, bo.vel + f0 * 3,
+                                   (size_t)nz * 3 * sizeof(double), kind, s));
+        }
+        if (R.info_dev) {
+          dflags.assign(B, 0);
+          PPH_HIP(hipMemcpyAsync(dflags.data(), R.info_dev, B * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         }
       }
       PPH_HIP(hipStreamSynchronize(s));      // the host vectors above outlive the copies
+      if (!device_out) {
+        for (int64_t b = 0; b < B; ++b) {
+          out->m[q0 + b] = R.M[b];
+          out->info[q0 + b] = infos[b] | (dflags.empty() ? 0 : dflags[b]);
+          if (out->nz) out->nz[q0 + b] = nzs[b];
+        }
+      }
     }
   };
   if (nl == 1) {
@@ -1490,6 +1577,12 @@ int pph_heevx(int64_t n, int64_t nmat, void* A, const pph_opts* opts, double* w,
                        opts->vl, opts->vu, R);
     } else {
       eig_batch<double>(h, h, B, h.A64.as<cplx<double>>(), nullptr, Z != nullptr, opts->vl, opts->vu, R);
+    }
+    if (R.info_dev) {                        // the flags the batch left on the device
+      std::vector<int32_t> fl(B);
+      PPH_HIP(hipMemcpyAsync(fl.data(), R.info_dev, B * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      PPH_HIP(hipStreamSynchronize(s));
+      for (int64_t b = 0; b < B; ++b) R.info[b] |= fl[b];
     }
     for (int64_t b = 0; b < B; ++b) {
       const int64_t mm = R.M[b];

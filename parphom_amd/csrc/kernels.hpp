@@ -53,6 +53,12 @@ void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk
                         const cplx<Tz>* Zt, int64_t ldt, int64_t strideT, int64_t N, int64_t M,
                         int64_t batch, double* out0, int64_t out0_b, double* out1,
                         int64_t out1_op, int64_t out1_b, double* ws, hipStream_t s);
+// Low-mode Rayleigh-Ritz step and second-order correction of the FP32 path
+// on the device (refine.hip): lam (B x mcap) refined in place for the modes
+// below lam_low (at most 16 per matrix); C1 = X^H D X(:, 0:16), C2 = X^H
+// X(:, 0:16) with leading dimension mcap and matrix stride 16 mcap.
+void launch_low_rr(double* lam, const double* nrm, const cplx<double>* C1, const cplx<double>* C2,
+                   int64_t M, int64_t mcap, double lam_low, int64_t B, hipStream_t s);
 // bytes of the workspace launch_sparse_dots needs (per-workgroup partials)
 int64_t sparse_dots_ws_bytes(int64_t N, int64_t M, int64_t batch, int nops);
 // Y = D X for k columns (n x k per matrix, ld n) from the listed blocks: row
@@ -298,6 +304,12 @@ void launch_cg_direction(cplx<double>* p, const cplx<double>* r, const double* r
 template <typename Ti, typename To>
 void launch_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count, hipStream_t s);
 void launch_fill_zero(void* p, int64_t bytes, hipStream_t s);
+// dst[b * dst_ld + i] = src[off[b] + i], i < cnt[b] <= max_cnt (doubles; off,
+// cnt device arrays of B entries)
+void launch_gather_rows(const double* src, const int64_t* off, const int64_t* cnt, double* dst,
+                        int64_t dst_ld, int64_t max_cnt, int64_t B, hipStream_t s);
+// info[b] |= flags[b], b < B (device arrays)
+void launch_or_flags(int32_t* info, const int* flags, int64_t B, hipStream_t s);
 
 // ---------------------------------------------------------------- density of states (dos.hip)
 // freq, func: nk x nb row-major (freq[k * nb + b]; func null: 1 everywhere, the

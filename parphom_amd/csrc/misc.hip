@@ -255,4 +255,41 @@ void launch_convert(const cplx<Ti>* in, cplx<To>* out, int64_t count, hipStream_
 template void launch_convert<float, double>(const cplx<float>*, cplx<double>*, int64_t, hipStream_t);
 template void launch_convert<double, float>(const cplx<double>*, cplx<float>*, int64_t, hipStream_t);
 
+// Batch outputs into the caller's device arrays in one launch per array:
+// dst[b * dst_ld + i] = src[off[b] + i] for i < cnt[b] (doubles; off and cnt
+// are B entries each), instead of one copy per q-point.
+__global__ void __launch_bounds__(256) k_gather_rows(const double* __restrict__ src,
+                                                     const int64_t* __restrict__ off,
+                                                     const int64_t* __restrict__ cnt,
+                                                     double* __restrict__ dst, int64_t dst_ld) {
+  const int64_t b = blockIdx.y;
+  const int64_t c = cnt[b];
+  const double* sp = src + off[b];
+  double* dp = dst + b * dst_ld;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < c;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dp[i] = sp[i];
+}
+
+void launch_gather_rows(const double* src, const int64_t* off, const int64_t* cnt, double* dst,
+                        int64_t dst_ld, int64_t max_cnt, int64_t B, hipStream_t s) {
+  if (B <= 0 || max_cnt <= 0) return;
+  const int64_t bx = std::min<int64_t>(ceil_div(max_cnt, 256), 1024);
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)bx, (unsigned)B), dim3(256), 0, s, src, off, cnt,
+                     dst, dst_ld);
+  PPH_CHECK_LAUNCH();
+}
+
+// info[b] |= flags[b] (per-matrix flags produced on the device)
+__global__ void k_or_flags(int32_t* info, const int* flags, int64_t B) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b < B) info[b] |= flags[b];
+}
+
+void launch_or_flags(int32_t* info, const int* flags, int64_t B, hipStream_t s) {
+  if (B <= 0) return;
+  hipLaunchKernelGGL(k_or_flags, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, s, info, flags, B);
+  PPH_CHECK_LAUNCH();
+}
+
 }  // namespace pph
