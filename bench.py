@@ -70,13 +70,21 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
         h.run_device(step_q(i), out)
         log(f"[bench] {prec} warmup {i + 1}/{warmup} done")
     torch.cuda.synchronize()
-    h.profile(enable=True, reset=True)
+    g0 = h.graphs()
     t0 = time.perf_counter()
     for i in range(steps):
         h.run_device(step_q(warmup + i), out)
         log(f"[bench] {prec} step {i + 1}/{steps} done")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    g1 = h.graphs()
+    # the phase profile of the same steps, recorded in a second pass (the
+    # library's phase timers run every batch eagerly, the timed pass replays
+    # the captured batch graphs)
+    h.profile(enable=True, reset=True)
+    for i in range(steps):
+        h.run_device(step_q(warmup + i), out)
+    torch.cuda.synchronize()
     prof = h.profile(enable=True, reset=True)
     # one batch alone (untimed; also the accuracy check): the first B q-points
     h.run_device(qs[:B], out)
@@ -85,7 +93,8 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
            "ev": out["eval_cm"][:B].cpu().numpy().copy(),
            "vel": out["vel"][:B].cpu().numpy().copy(),
            "z0": out["evec"][0].cpu().numpy().T.copy(),   # eigenvectors of the first q, columns
-           "two_stage": prof_iso["q2"][1] > 0}
+           "two_stage": prof_iso["q2"][1] > 0,
+           "graph_replays": g1[1] - g0[1], "graph_captures": g1[0] - g0[0]}
     h.close()
     del out
     torch.cuda.empty_cache()
@@ -408,7 +417,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--streams", type=int, default=2,
                     help="batches in flight per pph_run call (one stream each)")
@@ -476,7 +485,8 @@ def main():
                       "q_per_step": B * S, "batch": B, "streams": S,
                       "range": "A", "eigenvectors": True, "group_velocity": True,
                       "precision": "S",
-                      "reduction": "two-stage (band 32)" if two else "one-stage"}}
+                      "reduction": "two-stage (band 32)" if two else "one-stage",
+                      "batch_graphs_replayed_in_timed_steps": r32["graph_replays"]}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
     iso, iso_launch_ms = split_profile(iso32, n)
@@ -544,7 +554,7 @@ def main():
                            "traffic": kernel_traffic("k_q2_apply", a.workload, n, B),
                            "kernel": "k_q2_apply<float, 32, 8>", "flops_per_launch": q2_flops,
                            "avg_launch_us": q2_s * 1e6,
-                           "note": f"timed steps, {S} batches in flight sharing the GPU"}
+                           "note": f"profiled pass over the timed steps' q-points, {S} batches in flight sharing the GPU"}
         iso_q2 = iso32["q2"][0] / max(1, iso32["q2"][1]) / 1e3
         res["roofline_one_batch_alone"] = {
             "bound": "mfma", "achieved": q2_flops / iso_q2 / 1e12, "peak": FP32_PEAK / 1e12,
@@ -579,7 +589,7 @@ def main():
                            "kernel": "k_hemv_sym<float>", "bytes_per_launch": hemv_bytes,
                            "avg_launch_us": hemv_launch_ms * 1e3,
                            "ms_per_batch": round(phases["hemv"], 3),
-                           "note": f"timed steps, {S} batches in flight sharing the GPU"}
+                           "note": f"profiled pass over the timed steps' q-points, {S} batches in flight sharing the GPU"}
         iso_s = iso_launch_ms / 1e3
         res["roofline_one_batch_alone"] = {
             "bound": "hbm", "achieved": hemv_bytes / iso_s / 1e9, "peak": HBM_PEAK / 1e9,
@@ -589,14 +599,15 @@ def main():
     if not a.no_fp64:
         # FP64 workspaces are larger: half the batch per stream keeps both in HBM
         B64 = max(1, B // 2) if S > 1 else B
-        r64 = time_path(lib, torch, st, fc, qs, "D", B64, max(1, a.steps - 1),
-                        0 if a.warmup == 0 else 1, S, a.reduction)
+        r64 = time_path(lib, torch, st, fc, qs, "D", B64, max(1, a.steps - 1), a.warmup, S,
+                        a.reduction)
         qps64 = max(1, a.steps - 1) * B64 * S / r64["dt"]
         k = min(B, B64)
         dw = float(np.abs(ev32[:k] - r64["ev"][:k]).max())
         ph64, _ = split_profile(r64["prof"], n)
         res["fp64_path"] = {"value": qps64, "unit": "q-pts/s", "batch": B64, "streams": S,
                             "reduction": "two-stage (band 32)" if r64["two_stage"] else "one-stage",
+                            "batch_graphs_replayed_in_timed_steps": r64["graph_replays"],
                             "phase_ms_per_batch": {kk: round(v, 3) for kk, v in ph64.items()}}
         res["speedup_vs_fp64"] = qps32 / qps64
         res["max_abs_domega_vs_fp64_cm"] = dw

@@ -59,6 +59,13 @@ int pph_debug_sparse(void* h, int32_t mode, int64_t* nblk);
  * the same tridiagonal eigensolver and back-transformation; no counterpart
  * in the reference (pzheevx reduces in one stage, diagonalize.F90:73-96). */
 int pph_debug_reduction(void* h, int32_t mode, int32_t band);
+/* Batch graphs: with range A and profiling off, pph_run captures a batch of a
+ * shape its lane has just run eagerly as a HIP graph and replays it for the
+ * later batches of that shape (q-points uploaded before each replay, outputs
+ * gathered after it).  mode 1 on (the default), 0 off (every batch eager), -1
+ * unchanged; captures / replays (may be NULL) receive the counts over the
+ * handle's lanes.  No counterpart in the reference. */
+int pph_debug_graphs(void* h, int32_t mode, int64_t* captures, int64_t* replays);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,8 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <complex>
 #include <cmath>
 #include <cstring>
@@ -32,6 +34,11 @@ static thread_local std::string g_last_error;
 // one-stage reduction).
 constexpr int64_t kTwoStageMinN = 512, kTwoStageMaxN = 16384;
 
+// Bumped by every device allocation and free of a DevBuf: a captured batch
+// graph holds the addresses of its buffers and is replayed only while this
+// count is the one it was captured at.
+static std::atomic<uint64_t> g_alloc_gen{0};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -40,7 +47,10 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      g_alloc_gen.fetch_add(1);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -53,6 +63,7 @@ struct DevBuf {
       p = nullptr;
       throw std::bad_alloc();
     }
+    g_alloc_gen.fetch_add(1);
     bytes = b;
   }
   template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
@@ -160,6 +171,8 @@ struct Timers {
   }
 };
 
+struct BatchOut;
+
 // The working state of one batch in flight: its stream, phase timers and
 // device workspaces.  pph_run keeps several lanes busy at once (one host
 // thread each), so that one batch's memory-bound reduction overlaps
@@ -176,10 +189,24 @@ struct Lane {
   // two-stage reduction: band, bulge-chasing reflectors and their block
   // factors, panel T / Mx, Gram partials, stall flags
   DevBuf tsAB, tsV2, tsTau2, tsVTb, tsTp, tsG, tsInfo;
+  // the batch captured as a HIP graph (range A): its shape key, the
+  // allocation count it was captured at, and the host-side results of the
+  // capture (counts, device pointers), which every replay reproduces
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    std::array<int64_t, 7> key{};
+    uint64_t gen = 0;
+  } graph;
+  std::shared_ptr<BatchOut> graph_out;
+  std::array<int64_t, 7> eager_key{};
+  uint64_t eager_gen = ~uint64_t(0);
+  bool graph_failed = false;
+  int64_t graph_captures = 0, graph_replays = 0;
   Lane() = default;
   Lane(const Lane&) = delete;
   Lane& operator=(const Lane&) = delete;
   ~Lane() {
+    if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -204,6 +231,7 @@ struct Handle : Lane {
   int band = -1;            // -1: the process default
   bool two_stage() const;
   int band_width() const;
+  bool graphs = true;      // replay captured batch graphs (range A)
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
   int nchunk = 64;
@@ -945,8 +973,10 @@ struct BatchOut {
   double* vel = nullptr;       // device B x mcap x 3
 };
 
-static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /* B x 3 row-major */,
-                      BatchOut& out) {
+// Everything of one batch after its q-points are in L.qdev: only kernels
+// on the lane's stream (and, outside range A, the host reads of the mode
+// counts), so a range-A batch can be captured as a graph.
+static void compute_batch(Handle& h, Lane& L, int64_t B, BatchOut& out) {
   const int64_t n = h.n, N = h.N;
   hipStream_t s = L.stream;
   const pph_opts& o = h.opts;
@@ -954,9 +984,6 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   const bool vel = o.vel != 0;
   const bool want_vec = vel || o.jobz == 'V';
   const bool sparse = h.use_sparse();
-
-  L.qdev.ensure((size_t)B * 3 * sizeof(double));
-  PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
   // D in FP64: the FP64 path's working matrix; on the FP32 path the
   // refinement's dense products read it, which the block-sparse forms replace
   const bool d64 = !fp32 || !sparse;
@@ -1057,6 +1084,66 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
     out.vel = L.vacc.as<double>() + B * 3 * mcap;
   }
   L.tm.end(PH_TOTAL, s);
+}
+
+// One batch: the q-points to the device, then compute_batch, eagerly or as a
+// HIP graph.  A range-A batch of a shape the lane has just run eagerly (so
+// every buffer is allocated at its final size) is captured once and then
+// replayed for every later batch of that shape while no device buffer has
+// been allocated or freed since the capture; the captured batch's host-side
+// results (counts, device pointers) are those of every replay.  Profiling,
+// ranges I and V, and a failed capture run eagerly.
+static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /* B x 3 row-major */,
+                      BatchOut& out) {
+  hipStream_t s = L.stream;
+  L.qdev.ensure((size_t)B * 3 * sizeof(double));
+  PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+  const bool eligible = h.graphs && !L.graph_failed && h.opts.range == 'A' && !L.tm.on;
+  const std::array<int64_t, 7> key{B, h.nb_panel, h.nb_back, h.use_sparse() ? 1 : 0,
+                                   h.two_stage() ? 1 : 0, h.band_width(), h.n};
+  if (eligible && L.graph.exec && L.graph.key == key && L.graph.gen == g_alloc_gen.load() &&
+      L.graph_out) {
+    PPH_HIP(hipGraphLaunch(L.graph.exec, s));
+    out = *L.graph_out;
+    ++L.graph_replays;
+    return;
+  }
+  const uint64_t gen0 = g_alloc_gen.load();
+  if (eligible && L.eager_key == key && L.eager_gen == gen0) {
+    if (L.graph.exec) {
+      PPH_HIP(hipGraphExecDestroy(L.graph.exec));
+      L.graph.exec = nullptr;
+    }
+    PPH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    bool ok = true;
+    try {
+      compute_batch(h, L, B, out);
+    } catch (...) {
+      ok = false;
+    }
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    ok = ok && e == hipSuccess && g != nullptr;
+    // another lane allocating meanwhile does not make the capture wrong, but
+    // the count it would be replayed against is then unknown: run eagerly
+    const bool same_gen = g_alloc_gen.load() == gen0;
+    if (ok && same_gen) ok = hipGraphInstantiate(&L.graph.exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (ok && same_gen) {
+      L.graph.key = key;
+      L.graph.gen = gen0;
+      L.graph_out = std::make_shared<BatchOut>(out);
+      ++L.graph_captures;
+      PPH_HIP(hipGraphLaunch(L.graph.exec, s));
+      return;
+    }
+    L.graph.exec = nullptr;
+    if (!ok) L.graph_failed = true;      // not capturable: this lane stays eager
+  }
+  compute_batch(h, L, B, out);
+  L.eager_key = key;
+  L.eager_gen = g_alloc_gen.load();
 }
 
 // q-points per device batch when the caller does not fix it: as many as fit
@@ -1318,6 +1405,8 @@ static int run_common(pph_handle* hh, int64_t nq, const double* q, int64_t ldq, 
     PPH_HIP(hipSetDevice(h->device));
     Lane& L = h->lane((size_t)l);
     const hipStream_t s = L.stream;
+    // allocated before the first batch, so a later batch can be captured
+    if (device_out) L.meta.ensure((size_t)6 * bmax * sizeof(int64_t));
     std::vector<double> qrows;
     for (int64_t q0 = l * bmax; q0 < nq; q0 += nl * bmax) {
       const int64_t B = std::min(bmax, nq - q0);
@@ -1831,6 +1920,21 @@ extern "C" int pph_debug_reduction(pph_handle* hh, int32_t mode, int32_t band) {
   }
   h->reduction = mode;
   if (band) h->band = band;
+  return PPH_OK;
+}
+
+extern "C" int pph_debug_graphs(void* hh, int32_t mode, int64_t* captures, int64_t* replays) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return fail(nullptr, "pph_debug_graphs: handle is NULL", PPH_ERR_ARG);
+  if (mode < -1 || mode > 1) return fail(h, "pph_debug_graphs: mode must be -1, 0 or 1", PPH_ERR_ARG);
+  if (mode >= 0) h->graphs = mode == 1;
+  int64_t c = 0, r = 0;
+  for (size_t l = 0; l < h->lanes(); ++l) {
+    c += h->lane(l).graph_captures;
+    r += h->lane(l).graph_replays;
+  }
+  if (captures) *captures = c;
+  if (replays) *replays = r;
   return PPH_OK;
 }
 

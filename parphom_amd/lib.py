@@ -43,7 +43,7 @@ EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expe
             "pph_fc_symmetrize", "pph_destroy",
             "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
-            "pph_debug_sparse", "pph_debug_reduction")
+            "pph_debug_sparse", "pph_debug_reduction", "pph_debug_graphs")
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
           "hemv_launches", "he2hb", "hb2st", "q2")
@@ -82,6 +82,8 @@ def load(path: str = LIB_PATH):
     lib.pph_debug_sparse.restype = C.c_int
     lib.pph_debug_reduction.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.pph_debug_reduction.restype = C.c_int
+    lib.pph_debug_graphs.argtypes = [C.c_void_p, C.c_int32, _i64p, _i64p]
+    lib.pph_debug_graphs.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -266,6 +268,13 @@ class Handle:
         two-stage (dense -> band -> tridiagonal), -1 automatic; band 16 or 32
         (0 keeps the current one)."""
         _check(load().pph_debug_reduction(self._h, mode, band), self._h)
+
+    def graphs(self, mode=-1):
+        """Batch graphs (range A): 1 on (the default), 0 off, -1 unchanged.
+        Returns (captures, replays) summed over the handle's lanes."""
+        c, r = C.c_int64(0), C.c_int64(0)
+        _check(load().pph_debug_graphs(self._h, mode, C.byref(c), C.byref(r)), self._h)
+        return int(c.value), int(r.value)
 
     def sparse(self, mode=1):
         """Products with D and dD in the Rayleigh quotients and velocities:
