@@ -45,55 +45,70 @@ def _device_outputs(nq, n):
 
 
 def tune(lat, pos, types, masses, fc, q, precision="S", jobz="V", vel=True,
-         batches=(16, 32), streams=(1, 2), panels=(32, 48), backs=(64, 128), reps=1,
-         device_outputs=False, handle_factory=None, clock=time.perf_counter, log=None):
-    """Time every (batch, streams, nb_panel, nb_back) candidate and return
-    {"config": best, "qps": best rate, "table": [rows]}.  Each candidate runs
-    one untimed warm-up call and then `reps` timed calls of pph_run over
-    batch * streams q-points (taken cyclically from q): with host outputs, or
-    with outputs left in device buffers (pph_run_device, torch tensors) when
-    device_outputs is set.  Candidates the library rejects are recorded with
-    qps None."""
+         batches=(16, 32), streams=(1, 2), panels=(32, 48), backs=(64, 128), reductions=(-1,),
+         reps=1, device_outputs=False, handle_factory=None, clock=time.perf_counter, log=None):
+    """Time every (batch, streams, reduction, nb_panel, nb_back) candidate and
+    return {"config": best, "qps": best rate, "table": [rows]}.  Each
+    candidate runs one untimed warm-up call and then `reps` timed calls of
+    pph_run over batch * streams q-points (taken cyclically from q): with host
+    outputs, or with outputs left in device buffers (pph_run_device, torch
+    tensors) when device_outputs is set.  reduction: -1 the library's choice,
+    0 one-stage, 1 two-stage.  A candidate that fails (a blocking the library
+    rejects, a batch that does not fit in memory, any library error) is
+    recorded with qps None and its error, and the search goes on."""
     make = handle_factory or _default_handle
     q = np.atleast_2d(np.asarray(q, np.float64))
     table = []
     best = None
+
+    def sync():
+        if device_outputs:
+            import torch
+            torch.cuda.synchronize()
+
     for B, S in itertools.product(batches, streams):
         per_call = B * S
         idx = np.arange(per_call) % len(q)
         qs = q[idx]
-        out = _device_outputs(per_call, 3 * len(pos)) if device_outputs else None
+        rows = [{"batch": B, "streams": S, "reduction": r, "nb_panel": nbp, "nb_back": nbb,
+                 "qps": None} for r, nbp, nbb in itertools.product(reductions, panels, backs)]
+        try:
+            out = _device_outputs(per_call, 3 * len(pos)) if device_outputs else None
+            h = make(lat, pos, types, masses, fc, jobz=jobz, vel=vel, precision=precision,
+                     batch=B, streams=S)
+        except Exception as e:                 # e.g. the batch does not fit in memory
+            for row in rows:
+                row["error"] = str(e)
+            table.extend(rows)
+            continue
         call = (lambda h: h.run_device(qs, out)) if device_outputs else (lambda h: h.run(qs))
-        with make(lat, pos, types, masses, fc, jobz=jobz, vel=vel, precision=precision,
-                  batch=B, streams=S) as h:
-            for nbp, nbb in itertools.product(panels, backs):
-                row = {"batch": B, "streams": S, "nb_panel": nbp, "nb_back": nbb, "qps": None}
-                try:
-                    h.blocking(nbp, nbb)
-                except Exception as e:          # unsupported blocking: keep the row, skip it
-                    row["error"] = str(e)
-                    table.append(row)
-                    continue
-                call(h)                        # warm-up (allocations, first-use set-up)
-                if device_outputs:
-                    import torch
-                    torch.cuda.synchronize()
-                t0 = clock()
-                for _ in range(reps):
-                    call(h)
-                if device_outputs:
-                    torch.cuda.synchronize()
-                dt = clock() - t0
-                row["qps"] = reps * per_call / dt if dt > 0 else float("inf")
+        with h:
+            for row in rows:
                 table.append(row)
+                try:
+                    if row["reduction"] != -1 and hasattr(h, "reduction"):
+                        h.reduction(row["reduction"])
+                    h.blocking(row["nb_panel"], row["nb_back"])
+                    call(h)                    # warm-up (allocations, first-use set-up)
+                    sync()
+                    t0 = clock()
+                    for _ in range(reps):
+                        call(h)
+                    sync()
+                    dt = clock() - t0
+                except Exception as e:         # keep the row, go on with the next candidate
+                    row["error"] = str(e)
+                    continue
+                row["qps"] = reps * per_call / dt if dt > 0 else float("inf")
                 if log:
-                    log(f"[tune] batch {B} streams {S} panel {nbp} back {nbb}: "
-                        f"{row['qps']:.3f} q-pts/s")
+                    log(f"[tune] batch {B} streams {S} reduction {row['reduction']} "
+                        f"panel {row['nb_panel']} back {row['nb_back']}: {row['qps']:.3f} q-pts/s")
                 if best is None or row["qps"] > best["qps"]:
                     best = row
+        del out
     if best is None:
         raise RuntimeError("tune: no candidate ran")
-    cfg = {k: best[k] for k in ("batch", "streams", "nb_panel", "nb_back")}
+    cfg = {k: best[k] for k in ("batch", "streams", "reduction", "nb_panel", "nb_back")}
     return {"config": cfg, "qps": best["qps"], "table": table}
 
 
@@ -105,6 +120,8 @@ def main(argv=None):
     ap.add_argument("--streams", default="1,2")
     ap.add_argument("--panels", default="32,48")
     ap.add_argument("--backs", default="64,128")
+    ap.add_argument("--reductions", default="-1",
+                    help="reductions to try: -1 the library's choice, 0 one-stage, 1 two-stage")
     ap.add_argument("--reps", type=int, default=1)
     ap.add_argument("--device-outputs", action="store_true",
                     help="leave outputs in device buffers (pph_run_device) instead of host arrays")
@@ -114,7 +131,8 @@ def main(argv=None):
     ints = lambda s: tuple(int(x) for x in s.split(","))   # noqa: E731
     res = tune(st.lat, st.pos, st.types, st.masses, fc, q, precision=a.precision,
                batches=ints(a.batches), streams=ints(a.streams), panels=ints(a.panels),
-               backs=ints(a.backs), reps=a.reps, device_outputs=a.device_outputs,
+               backs=ints(a.backs), reductions=ints(a.reductions), reps=a.reps,
+               device_outputs=a.device_outputs,
                log=lambda m: print(m, flush=True))
     res["workload"] = a.workload
     print(json.dumps(res))

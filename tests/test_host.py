@@ -229,12 +229,51 @@ def test_tuner_picks_fastest_candidate_and_records_rejections():
     res = tune.tune(None, None, None, None, None, np.zeros((5, 3)), batches=(16, 32),
                     streams=(1, 2), panels=(32, 40), backs=(64, 128), reps=2,
                     handle_factory=StubHandle, clock=lambda: now[0])
-    assert res["config"] == {"batch": 16, "streams": 2, "nb_panel": 32, "nb_back": 128}
+    assert res["config"] == {"batch": 16, "streams": 2, "reduction": -1, "nb_panel": 32,
+                             "nb_back": 128}
     assert abs(res["qps"] - 100.0) < 1e-9
     rejected = [r for r in res["table"] if r["nb_panel"] == 40]
     assert len(rejected) == 8 and all(r["qps"] is None and "error" in r for r in rejected)
     # one warm-up and two timed calls per accepted candidate
     assert len(calls) == 3 * 8
+
+
+def test_tuner_records_failing_candidates_and_goes_on():
+    """A candidate whose handle cannot be made (a batch that does not fit in
+    memory) or whose run raises is recorded with its error; the search goes
+    on and picks the best of the others."""
+    from parphom_amd import tune
+    now = [0.0]
+
+    class StubHandle:
+        def __init__(self, *a, batch, streams, **k):
+            if batch == 64:
+                raise MemoryError("out of device memory")
+            self.batch, self.streams = batch, streams
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            return False
+
+        def blocking(self, nbp, nbb):
+            self.nbb = nbb
+
+        def run(self, q):
+            if self.streams == 2 and self.nbb == 64:
+                raise RuntimeError("pph_run: out of memory")
+            now[0] += (1.0 if self.nbb == 128 else 2.0) * len(q) / 100.0
+
+    res = tune.tune(None, None, None, None, None, np.zeros((5, 3)), batches=(16, 64),
+                    streams=(1, 2), panels=(32,), backs=(64, 128), reps=1,
+                    handle_factory=StubHandle, clock=lambda: now[0])
+    t = res["table"]
+    assert len(t) == 8
+    assert all(r["qps"] is None and "device memory" in r["error"] for r in t if r["batch"] == 64)
+    failed = [r for r in t if r["batch"] == 16 and r["streams"] == 2 and r["nb_back"] == 64]
+    assert len(failed) == 1 and failed[0]["qps"] is None and "out of memory" in failed[0]["error"]
+    assert res["config"]["nb_back"] == 128 and abs(res["qps"] - 100.0) < 1e-9
 
 
 def test_complement_correction_formula():
