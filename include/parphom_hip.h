@@ -198,6 +198,27 @@ int pph_dos_smearing(int32_t method, int64_t nq, int64_t nbands, const double* f
 int pph_fc_symmetrize(int64_t natom, double* fc, int64_t iterations, int32_t device,
                       int32_t on_device);
 
+/* Spring-model force constants on the GPU, the synthetic producer that
+ * stands in for the finite-displacement LAMMPS producer of
+ * Force_Constant_Generation/get_force_constant.py:227-295 (which writes the
+ * dataset force_read.F90:93-158 reads).  For each listed pair p (atoms
+ * pi[p] != pj[p], minimum-image separation rvec[3p .. 3p+2] in A) of kind
+ * 0..2 (intralayer neighbour shell s: radial spring params[s], tangential
+ * params[3 + s]) or 3 (interlayer: params[6] exp(-(|r| - inter_r0) /
+ * inter_decay), radial):
+ *   fc[i, j] = -(spring block) .* (1 + noise_amp u_ij),
+ * u_ij symmetric in [-1, 1), drawn from (seed, pair) by a counter-based hash;
+ * then fc[i, i] = -sum_{j != i} fc[i, j] (acoustic sum rule).  fc: dense
+ * (natom, natom, 3, 3) C-order float64, eV/A^2.  The model is linear in
+ * params, so params = e_k gives d fc / d params[k].  Host or device pointers
+ * per `on_device` (with device pointers the pairs are not range-checked);
+ * synchronous.  parphom_amd/synth.py builds the pair list and states the
+ * same model on the CPU. */
+int pph_spring_fc(int64_t natom, int64_t npairs, const int64_t* pi, const int64_t* pj,
+                  const double* rvec, const int32_t* kind, const double* params, uint64_t seed,
+                  double noise_amp, double inter_r0, double inter_decay, double* fc,
+                  int32_t device, int32_t on_device);
+
 int pph_destroy(pph_handle* h);
 
 /* Last error text for a handle (or the thread's last error when h is NULL). */

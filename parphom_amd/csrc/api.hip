@@ -1894,6 +1894,53 @@ int pph_fc_symmetrize(int64_t natom, double* fc, int64_t iterations, int32_t dev
   PPH_API_CATCH(nullptr)
 }
 
+int pph_spring_fc(int64_t natom, int64_t npairs, const int64_t* pi, const int64_t* pj,
+                  const double* rvec, const int32_t* kind, const double* params, uint64_t seed,
+                  double noise_amp, double inter_r0, double inter_decay, double* fc,
+                  int32_t device, int32_t on_device) {
+  if (natom < 1 || npairs < 0 || !fc || !params ||
+      (npairs > 0 && (!pi || !pj || !rvec || !kind)) || !(inter_decay > 0.0))
+    return fail(nullptr, "pph_spring_fc: bad arguments", PPH_ERR_ARG);
+  if (!on_device) {
+    // a pair outside the cell or a kind the model does not know would write
+    // outside fc or read outside the parameters
+    for (int64_t p = 0; p < npairs; ++p)
+      if (pi[p] < 0 || pi[p] >= natom || pj[p] < 0 || pj[p] >= natom || pi[p] == pj[p] ||
+          kind[p] < 0 || kind[p] > 3)
+        return fail(nullptr, "pph_spring_fc: pair or kind out of range", PPH_ERR_ARG);
+  }
+  PPH_API_TRY
+  PPH_HIP(hipSetDevice(device));
+  Stream st;
+  SpringParams sp{};
+  for (int k = 0; k < 7; ++k) sp.p[k] = params[k];
+  sp.amp = noise_amp; sp.inter_r0 = inter_r0; sp.inter_decay = inter_decay; sp.seed = seed;
+  const size_t bytes = (size_t)natom * natom * 9 * sizeof(double);
+  DevBuf bfc, bi, bj, br, bk;
+  double* d = fc;
+  const int64_t *di = pi, *dj = pj;
+  const double* dr = rvec;
+  const int32_t* dk = kind;
+  if (!on_device) {
+    bfc.ensure(bytes);
+    d = bfc.as<double>();
+    if (npairs > 0) {
+      bi.ensure(npairs * sizeof(int64_t)); bj.ensure(npairs * sizeof(int64_t));
+      br.ensure(npairs * 3 * sizeof(double)); bk.ensure(npairs * sizeof(int32_t));
+      PPH_HIP(hipMemcpyAsync(bi.p, pi, npairs * sizeof(int64_t), hipMemcpyHostToDevice, st.s));
+      PPH_HIP(hipMemcpyAsync(bj.p, pj, npairs * sizeof(int64_t), hipMemcpyHostToDevice, st.s));
+      PPH_HIP(hipMemcpyAsync(br.p, rvec, npairs * 3 * sizeof(double), hipMemcpyHostToDevice, st.s));
+      PPH_HIP(hipMemcpyAsync(bk.p, kind, npairs * sizeof(int32_t), hipMemcpyHostToDevice, st.s));
+      di = bi.as<int64_t>(); dj = bj.as<int64_t>(); dr = br.as<double>(); dk = bk.as<int32_t>();
+    }
+  }
+  launch_spring_fc(natom, npairs, di, dj, dr, dk, sp, d, st.s);
+  if (!on_device) PPH_HIP(hipMemcpyAsync(fc, d, bytes, hipMemcpyDeviceToHost, st.s));
+  PPH_HIP(hipStreamSynchronize(st.s));
+  return PPH_OK;
+  PPH_API_CATCH(nullptr)
+}
+
 int pph_destroy(pph_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return PPH_OK;

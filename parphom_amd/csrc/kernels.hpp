@@ -59,6 +59,19 @@ void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk
 // X(:, 0:16) with leading dimension mcap and matrix stride 16 mcap.
 void launch_low_rr(double* lam, const double* nrm, const cplx<double>* C1, const cplx<double>* C2,
                    int64_t M, int64_t mcap, double lam_low, int64_t B, hipStream_t s);
+// Spring-model force constants (spring.hip): pairs (pi, pj) with separation
+// rvec (3 per pair) and kind 0..2 (intralayer shell: radial p[kind],
+// tangential p[3 + kind]) or 3 (interlayer: p[6] exp(-(|r| - inter_r0) /
+// inter_decay)); noise amplitude amp, seed of the counter-based noise.  fc:
+// dense (N, N, 3, 3) C-order, cleared first; self blocks by the sum rule.
+struct SpringParams {
+  double p[7];
+  double amp, inter_r0, inter_decay;
+  uint64_t seed;
+};
+void launch_spring_fc(int64_t N, int64_t npairs, const int64_t* pi, const int64_t* pj,
+                      const double* rvec, const int32_t* kind, const SpringParams& sp, double* fc,
+                      hipStream_t s);
 // bytes of the workspace launch_sparse_dots needs (per-workgroup partials)
 int64_t sparse_dots_ws_bytes(int64_t N, int64_t M, int64_t batch, int nops);
 // Y = D X for k columns (n x k per matrix, ld n) from the listed blocks: row

@@ -40,7 +40,7 @@ class PphOut(C.Structure):
 
 EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expect", "pph_heevx",
             "pph_dos_triangulation", "pph_dos_triangulation_noalloc", "pph_dos_smearing",
-            "pph_fc_symmetrize", "pph_destroy",
+            "pph_fc_symmetrize", "pph_spring_fc", "pph_destroy",
             "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
             "pph_debug_sparse", "pph_debug_reduction", "pph_debug_graphs")
@@ -95,10 +95,13 @@ def load(path: str = LIB_PATH):
     lib.pph_dos_smearing.argtypes = [C.c_int32, C.c_int64, C.c_int64, _dp, _dp, C.c_double,
                                      C.c_int64, C.c_int64, _dp, _dp, C.c_int32, C.c_int32]
     lib.pph_fc_symmetrize.argtypes = [C.c_int64, _dp, C.c_int64, C.c_int32, C.c_int32]
+    lib.pph_spring_fc.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, _dp, C.c_uint64, C.c_double, C.c_double,
+                                  C.c_double, C.c_void_p, C.c_int32, C.c_int32]
     for name in ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_heevx",
                  "pph_destroy", "pph_debug_zgemm", "pph_dos_triangulation",
                  "pph_dos_triangulation_noalloc", "pph_dos_smearing",
-                 "pph_fc_symmetrize"):
+                 "pph_fc_symmetrize", "pph_spring_fc"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -415,6 +418,43 @@ def dos_smearing(method, freq, weight, width, E, natom, device=0):
     _check(lib.pph_dos_smearing(int(m), nq, nb, _ptr(freq), _ptr(w), float(width), int(natom),
                                 len(E), _ptr(E), _ptr(dos), device, 0))
     return dos
+
+
+def spring_fc(natom, i, j, rvec, kind, params, seed, noise_amp, inter_r0, inter_decay, out=None,
+              device=0):
+    """Spring-model force constants on the GPU (pph_spring_fc) from a pair
+    list.  Returns a (natom, natom, 3, 3) float64 array, or fills `out`, a
+    contiguous float64 CUDA tensor of that shape, and returns it."""
+    lib = load()
+    i = np.ascontiguousarray(i, np.int64)
+    j = np.ascontiguousarray(j, np.int64)
+    rvec = np.ascontiguousarray(rvec, np.float64).reshape(-1, 3)
+    kind = np.ascontiguousarray(kind, np.int32)
+    params = np.ascontiguousarray(params, np.float64)
+    npairs = len(i)
+    if not (len(j) == len(kind) == len(rvec) == npairs) or params.shape != (7,):
+        raise ValueError("spring_fc: pair arrays of one length and 7 parameters")
+    args = (natom, npairs, i.ctypes.data, j.ctypes.data, rvec.ctypes.data, kind.ctypes.data,
+            _ptr(params), int(seed) & (2 ** 64 - 1), float(noise_amp), float(inter_r0),
+            float(inter_decay))
+    if out is None:
+        fc = np.zeros((natom, natom, 3, 3))
+        _check(lib.pph_spring_fc(*args, fc.ctypes.data, device, 0))
+        return fc
+    import torch
+    if (not isinstance(out, torch.Tensor) or out.dtype != torch.float64 or not out.is_cuda
+            or not out.is_contiguous() or tuple(out.shape) != (natom, natom, 3, 3)):
+        raise ValueError("spring_fc: out must be a contiguous float64 CUDA tensor (N, N, 3, 3)")
+    # device pointers are not range-checked by the library: check the pairs here
+    if npairs and (i.min() < 0 or j.min() < 0 or max(i.max(), j.max()) >= natom or
+                   (i == j).any() or kind.min() < 0 or kind.max() > 3):
+        raise ValueError("spring_fc: pair or kind out of range")
+    dev = out.device
+    ti, tj = torch.from_numpy(i).to(dev), torch.from_numpy(j).to(dev)
+    tr, tk = torch.from_numpy(rvec).to(dev), torch.from_numpy(kind).to(dev)
+    dargs = args[:2] + (ti.data_ptr(), tj.data_ptr(), tr.data_ptr(), tk.data_ptr()) + args[6:]
+    _check(lib.pph_spring_fc(*dargs, out.data_ptr(), dev.index or 0, 1))
+    return out
 
 
 def fc_symmetrize(fc, iterations, device=0):

@@ -19,6 +19,11 @@ k(r) = k0 exp(-(r - 3.35)/0.5) for r < 6 A, the minimum periodic image for
 every pair, +-1% seeded multiplicative noise on each pair block (applied
 symmetrically, so Phi_ij = Phi_ji^T holds exactly), and self blocks from the
 acoustic sum rule, Phi_ii = -sum_{j != i} Phi_ij.
+
+The noise of the pair {i, j} is counter-based, a function of (seed, pair,
+component) alone: splitmix64 of seed * 0x9E3779B97F4A7C15 + (min(i,j) N +
+max(i,j)) * 8 + c for the six upper-triangle components c, mapped to
+[-1, 1).  The library's GPU producer (pph_spring_fc) draws the same numbers.
 """
 from __future__ import annotations
 
@@ -170,40 +175,74 @@ class SpringBasis:
     natom: int
 
 
-def spring_basis(st: Structure, seed: int) -> SpringBasis:
-    """Decompose the spring-model force constants into per-parameter blocks."""
+def spring_pairs(st: Structure):
+    """The pairs the spring model couples: (i, j, rvec, kind) with rvec the
+    minimum-image separation and kind 0, 1, 2 (intralayer neighbour shell) or
+    3 (interlayer, |r| < cutoff); other pairs carry no spring."""
     a = A_GRAPHENE
     shells = (a / math.sqrt(3.0), a, 2.0 * a / math.sqrt(3.0))
     i, j, r = _min_image_pairs(st, INTER_CUTOFF)
     dist = np.linalg.norm(r, axis=1)
+    same = st.layer[i] == st.layer[j]
+    kind = np.full(len(i), -1, np.int32)
+    for s, d0 in enumerate(shells):
+        kind[same & (np.abs(dist - d0) < 1e-3)] = s
+    kind[(~same) & (dist < INTER_CUTOFF)] = 3
+    keep = kind >= 0
+    return i[keep], j[keep], r[keep], kind[keep]
+
+
+def spring_basis(st: Structure, seed: int) -> SpringBasis:
+    """Decompose the spring-model force constants into per-parameter blocks."""
+    i, j, r, kind = spring_pairs(st)
+    dist = np.linalg.norm(r, axis=1)
     rhat = r / dist[:, None]
     rr = rhat[:, :, None] * rhat[:, None, :]
     eye = np.eye(3)[None]
-    same = st.layer[i] == st.layer[j]
     blocks = np.zeros((len(i), 7, 3, 3))
-    for s, d0 in enumerate(shells):
-        sel = same & (np.abs(dist - d0) < 1e-3)
+    for s in range(3):
+        sel = kind == s
         blocks[sel, s] = rr[sel]
         blocks[sel, 3 + s] = (eye - rr)[sel]
-    inter = (~same) & (dist < INTER_CUTOFF)
+    inter = kind == 3
     kfac = np.exp(-(dist - INTERLAYER) / INTER_DECAY)
     blocks[inter, 6] = kfac[inter, None, None] * rr[inter]
-    keep = np.abs(blocks).reshape(len(i), -1).max(axis=1) > 0
-    i, j, blocks = i[keep], j[keep], blocks[keep]
     # symmetric multiplicative noise, identical for (i, j) and (j, i)
-    rng = np.random.default_rng(seed)
     N = st.natom
-    lo = np.minimum(i, j)
-    hi = np.maximum(i, j)
-    key = lo * N + hi
-    ukeys, inv = np.unique(key, return_inverse=True)
-    u = rng.uniform(-1.0, 1.0, size=(len(ukeys), 6))
-    tri = np.zeros((len(ukeys), 3, 3))
-    iu = np.triu_indices(3)
-    tri[:, iu[0], iu[1]] = u
-    tri = tri + np.triu(tri, 1).transpose(0, 2, 1)
-    noise = 1.0 + NOISE * tri[inv]
+    noise = 1.0 + NOISE * pair_noise(seed, i, j, N)
     return SpringBasis(i=i, j=j, blocks=blocks, noise=noise, natom=N)
+
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+# upper-triangle component c of a 3x3 block: (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+_TRIU = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """The splitmix64 finaliser on uint64 (wrapping arithmetic)."""
+    x = np.asarray(x, np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def pair_noise(seed: int, i, j, N: int) -> np.ndarray:
+    """(npairs, 3, 3) symmetric noise in [-1, 1) of the pairs {i, j}: the same
+    numbers for (i, j) and (j, i)."""
+    i = np.asarray(i, np.int64)
+    j = np.asarray(j, np.int64)
+    key = (np.minimum(i, j) * N + np.maximum(i, j)).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        base = np.uint64(seed) * _GOLDEN + key * np.uint64(8)
+    out = np.zeros((len(i), 3, 3))
+    for c, (a, b) in enumerate(_TRIU):
+        h = splitmix64(base + np.uint64(c))
+        u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)   # [0, 1)
+        out[:, a, b] = 2.0 * u - 1.0
+        out[:, b, a] = out[:, a, b]
+    return out
 
 
 def force_constants(st: Structure, seed: int, params=None, basis: SpringBasis = None,
@@ -213,7 +252,11 @@ def force_constants(st: Structure, seed: int, params=None, basis: SpringBasis = 
         basis = spring_basis(st, seed)
     p = default_params() if params is None else np.asarray(params, np.float64)
     N = basis.natom
-    pair = -np.einsum("k,pkab->pab", p, basis.blocks) * basis.noise
+    # the parameter sum in a fixed order (k = 0 .. 6), as the GPU producer adds it
+    acc = np.zeros((len(basis.i), 3, 3))
+    for k in range(len(p)):
+        acc = acc + p[k] * basis.blocks[:, k]
+    pair = -acc * basis.noise
     fc = np.zeros((N, N, 3, 3), dtype=np.float64)
     fc[basis.i, basis.j] = pair
     self_blk = np.zeros((N, 3, 3))
