@@ -45,7 +45,8 @@ def device_outputs(torch, B, n):
     }
 
 
-def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1):
+def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1,
+              tile="64x64"):
     """Each step is one pph_run_device call over streams x B q-points: the
     library keeps `streams` batches of B in flight (one stream and host thread
     each).  Returns a dict: the timed seconds, the phase profile of the timed
@@ -58,6 +59,8 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
                    batch=B, streams=streams)
     nblk = h.sparse(1)               # blocks the quadratic forms use (0 = dense products)
     h.reduction(reduction)           # -1: the library's choice
+    if not (prec == "D" and tile == "128x128"):     # the 128 x 128 tile is FP32 only
+        h.gemm_tile(tile)
     per_step = B * streams
     out = device_outputs(torch, per_step, n)
     nq = len(qs)
@@ -94,6 +97,8 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
            "vel": out["vel"][:B].cpu().numpy().copy(),
            "z0": out["evec"][0].cpu().numpy().T.copy(),   # eigenvectors of the first q, columns
            "two_stage": prof_iso["q2"][1] > 0,
+           # the one-launch small-matrix reduction records no hemv launches
+           "small": prof_iso["q2"][1] == 0 and prof_iso["hemv_launches"][1] == 0,
            "graph_replays": g1[1] - g0[1], "graph_captures": g1[0] - g0[0]}
     h.close()
     del out
@@ -423,9 +428,12 @@ def main():
                     help="batches in flight per pph_run call (one stream each); the default "
                          "and the batch sizes are the measured best of both paths at n = 4764")
     ap.add_argument("--workload", default="tbg_11_12")
-    ap.add_argument("--reduction", type=int, default=-1, choices=(-1, 0, 1),
-                    help="reduction to the tridiagonal: 0 one-stage, 1 two-stage, "
+    ap.add_argument("--reduction", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="reduction to the tridiagonal: 0 one-stage, 1 two-stage, 2 the "
+                         "one-launch small-matrix kernel (n <= 128 FP32, 96 FP64), "
                          "-1 the library's choice")
+    ap.add_argument("--tile", default="64x64", choices=("64x64", "128x64", "64x128", "128x128"),
+                    help="GEMM tile shape of the eigensolver (the autotuner's tile dimension)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -472,10 +480,10 @@ def main():
         f"(inputs built in {time.perf_counter() - t0:.1f}s)")
 
     B, S = a.batch, a.streams
-    r32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S, a.reduction)
+    r32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S, a.reduction, a.tile)
     dt32, prof32, iso32, ev32, z32, nblk = (r32["dt"], r32["prof"], r32["iso"], r32["ev"],
                                             r32["z0"], r32["nblk"])
-    two = r32["two_stage"]
+    two, small = r32["two_stage"], r32["small"]
     qps32 = a.steps * B * S / dt32
     res = {"metric": "q-pts/s at 3N=4764 eigpairs+v_g: FP32+refine vs FP64, max|dw|<=1e-3 cm^-1",
            "value": qps32, "unit": "q-pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -486,7 +494,9 @@ def main():
                       "q_per_step": B * S, "batch": B, "streams": S,
                       "range": "A", "eigenvectors": True, "group_velocity": True,
                       "precision": "S",
-                      "reduction": "two-stage (band 32)" if two else "one-stage",
+                      "reduction": ("two-stage (band 32)" if two else
+                                    "small-matrix kernel" if small else "one-stage"),
+                      "gemm_tile": a.tile,
                       "batch_graphs_replayed_in_timed_steps": r32["graph_replays"]}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
@@ -579,6 +589,20 @@ def main():
             "bound": "mfma", "achieved": he_flops / he_s / 1e12, "peak": FP32_PEAK / 1e12,
             "unit": "TFLOP/s", "frac": he_flops / he_s / FP32_PEAK,
             "kernel": "he2hb (k_panel_qr + k_zgemm panel products), one batch alone"}
+    elif small:
+        # the one-launch reduction of small matrices (k_small_hetrd, one
+        # workgroup per matrix, the matrix in LDS): 16/3 n^3 flops per matrix
+        # (the textbook count of a Hermitian reduction) on the vector units, in
+        # the working precision (FP32 here); one launch per batch, rated on the
+        # batch run alone
+        sm_flops = B * 16.0 / 3.0 * float(n) ** 3
+        sm_s = iso["tridiag"] / 1e3
+        res["roofline"] = {"bound": "fp32 vector", "achieved": sm_flops / sm_s / 1e12,
+                           "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                           "frac": sm_flops / sm_s / FP32_PEAK, "traffic": None,
+                           "kernel": "k_small_hetrd<float>", "flops_per_launch": sm_flops,
+                           "avg_launch_us": sm_s * 1e6,
+                           "note": "one batch alone on the GPU (HIP-event pair on the lane's stream)"}
     else:
         # roofline of the dominant kernel, the hemv of the one-stage reduction
         # (k_hemv_sym, one launch per column j, HBM-bound): algorithmic bytes
@@ -611,13 +635,14 @@ def main():
         # 11.97 q-points/s against 11.09 at 16 x 2 and 11.65 at 32 x 2)
         B64 = max(1, (3 * B) // 4) if S > 1 else B
         r64 = time_path(lib, torch, st, fc, qs, "D", B64, max(1, a.steps - 1), a.warmup, S,
-                        a.reduction)
+                        a.reduction, a.tile)
         qps64 = max(1, a.steps - 1) * B64 * S / r64["dt"]
         k = min(B, B64)
         dw = float(np.abs(ev32[:k] - r64["ev"][:k]).max())
         ph64, _ = split_profile(r64["prof"], n)
         res["fp64_path"] = {"value": qps64, "unit": "q-pts/s", "batch": B64, "streams": S,
-                            "reduction": "two-stage (band 32)" if r64["two_stage"] else "one-stage",
+                            "reduction": ("two-stage (band 32)" if r64["two_stage"] else
+                                          "small-matrix kernel" if r64["small"] else "one-stage"),
                             "batch_graphs_replayed_in_timed_steps": r64["graph_replays"],
                             "phase_ms_per_batch": {kk: round(v, 3) for kk, v in ph64.items()}}
         res["speedup_vs_fp64"] = qps32 / qps64
@@ -639,12 +664,14 @@ def main():
             "gemm_peak_spec_tflops": {"fp32": FP32_PEAK / 1e12, "fp64": FP64_PEAK / 1e12}}
         if not a.no_compare:
             # the FP32 path with the other reduction, same batches and lanes
-            other = 0 if two else 1
-            ro = time_path(lib, torch, st, fc, qs, "S", B, a.steps, min(a.warmup, 1), S, other)
+            other = 0 if (two or small) else 1
+            ro = time_path(lib, torch, st, fc, qs, "S", B, a.steps, min(a.warmup, 1), S, other,
+                           a.tile)
             pho, _ = split_profile(ro["iso"], n)
             dvo = [velocity_difference(ro["vel"][i], r64["vel"][i], r64["ev"][i]) for i in range(k)]
             res["fp32_other_reduction"] = {
-                "reduction": "two-stage (band 32)" if ro["two_stage"] else "one-stage",
+                "reduction": ("two-stage (band 32)" if ro["two_stage"] else
+                              "small-matrix kernel" if ro["small"] else "one-stage"),
                 "value": a.steps * B * S / ro["dt"], "unit": "q-pts/s",
                 "max_abs_domega_vs_fp64_cm": float(np.abs(ro["ev"][:k] - r64["ev"][:k]).max()),
                 "max_rel_dvel_vs_fp64": max(x[0] for x in dvo) / max(1e-300, max(x[1] for x in dvo)),

@@ -53,7 +53,10 @@ int pph_debug_blocking(void* h, int32_t nb_panel, int32_t nb_back);
 int pph_debug_sparse(void* h, int32_t mode, int64_t* nblk);
 /* Reduction to the tridiagonal: mode 0 one-stage (blocked LATRD/HETRD), 1
  * two-stage (dense -> band of half-bandwidth `band` on the matrix cores, then
- * bulge chasing), -1 automatic; band 16 or 32 (0 keeps the current one).
+ * bulge chasing), 2 the one-launch reduction of small matrices (one
+ * workgroup per matrix, the matrix in LDS; orders up to 96 in FP64 and 128 in
+ * FP32, larger orders take the one-stage path), -1 automatic (the small
+ * kernel where the order fits it); band 16 or 32 (0 keeps the current one).
  * With h == NULL the setting becomes the default of every handle whose own
  * mode is -1 (also the temporary ones pph_heevx makes).  Both reductions feed
  * the same tridiagonal eigensolver and back-transformation; no counterpart
@@ -66,6 +69,15 @@ int pph_debug_reduction(void* h, int32_t mode, int32_t band);
  * unchanged; captures / replays (may be NULL) receive the counts over the
  * handle's lanes.  No counterpart in the reference. */
 int pph_debug_graphs(void* h, int32_t mode, int64_t* captures, int64_t* replays);
+/* Tile shape of the matrix-core GEMMs of the eigensolver (reduction updates,
+ * band reduction, back-transformation), the autotuner's tile dimension: 1
+ * 64x64 (the default), 2 128x64, 3 64x128, 4 128x128 (FP32 only), 0 the
+ * process default.  With h == NULL the setting becomes the process default
+ * (also for pph_debug_zgemm and the temporary handles of pph_heevx).
+ * Products with the dot epilogue and outputs at most 32 columns wide keep
+ * their own tiles.  No counterpart in the reference (PBLAS chooses its own
+ * blocking inside pzheevx and pzgemm). */
+int pph_debug_gemm_tile(void* h, int32_t tile);
 
 #ifdef __cplusplus
 }

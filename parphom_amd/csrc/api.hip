@@ -194,11 +194,11 @@ struct Lane {
   // capture (counts, device pointers), which every replay reproduces
   struct Graph {
     hipGraphExec_t exec = nullptr;
-    std::array<int64_t, 7> key{};
+    std::array<int64_t, 8> key{};
     uint64_t gen = 0;
   } graph;
   std::shared_ptr<BatchOut> graph_out;
-  std::array<int64_t, 7> eager_key{};
+  std::array<int64_t, 8> eager_key{};
   uint64_t eager_gen = ~uint64_t(0);
   bool graph_failed = false;
   int64_t graph_captures = 0, graph_replays = 0;
@@ -230,8 +230,14 @@ struct Handle : Lane {
   int reduction = -1;
   int band = -1;            // -1: the process default
   bool two_stage() const;
+  // the one-launch reduction of small matrices (mode 2, or automatic) where
+  // the order fits its LDS-resident kernel
+  template <typename T> bool small_reduction() const;
+  // the reduction a batch of this handle runs: 0 one-stage, 1 two-stage, 2 small
+  int reduction_code() const;
   int band_width() const;
   bool graphs = true;      // replay captured batch graphs (range A)
+  int gemm_tile = 0;       // GEMM tile shape of the eigensolver (kGemmTile*; 0: process default)
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
   int nchunk = 64;
@@ -265,6 +271,17 @@ bool Handle::two_stage() const {
   const int r = reduction >= 0 ? reduction : g_reduction_default;
   if (r >= 0) return r == 1;
   return opts.precision == 'S' && n >= kTwoStageMinN && n <= kTwoStageMaxN;
+}
+
+template <typename T> bool Handle::small_reduction() const {
+  const int r = reduction >= 0 ? reduction : g_reduction_default;
+  return (r == -1 || r == 2) && n <= kSmallMaxN<T>;
+}
+
+int Handle::reduction_code() const {
+  if (two_stage()) return 1;
+  const bool small = opts.precision == 'S' ? small_reduction<float>() : small_reduction<double>();
+  return small ? 2 : 0;
 }
 
 // ---------------------------------------------------------------- small dense FP64 (host)
@@ -705,6 +722,7 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   const pph_opts& o = h.opts;
   const bool fp32 = std::is_same<T, float>::value;
   const bool vectors = want_vec || fp32;      // the FP32 path refines with its vectors
+  const gemm_tile_scope tile_scope(h.gemm_tile);
   const int nb = h.nb_panel;
   const int64_t mcap = range_mcap(o, n);
   R.mcap = mcap;
@@ -727,8 +745,11 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
   tw.tau = L.tau.as<cplx<T>>(); tw.strideTau = n;
   tw.d = L.dgl.as<T>(); tw.e = L.egl.as<T>(); tw.strideDE = n;
   TwoStageWork<T> ts{};
+  const bool small = !two && h.small_reduction<T>();
   L.tm.begin(PH_TRID, s);
-  if (two) {
+  if (small) {
+    tridiagonalise_small<T>(tw, n + kVpad, s);
+  } else if (two) {
     constexpr int64_t kChunks = 16;
     L.tsAB.ensure((size_t)B * n * 2 * bwd * sizeof(cplx<T>));
     L.tsV2.ensure((size_t)B * n * kmax2 * bwd * sizeof(cplx<T>));
@@ -1099,8 +1120,9 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   L.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
   const bool eligible = h.graphs && !L.graph_failed && h.opts.range == 'A' && !L.tm.on;
-  const std::array<int64_t, 7> key{B, h.nb_panel, h.nb_back, h.use_sparse() ? 1 : 0,
-                                   h.two_stage() ? 1 : 0, h.band_width(), h.n};
+  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, h.use_sparse() ? 1 : 0,
+                                   h.reduction_code(), h.band_width(), h.n,
+                                   h.gemm_tile ? h.gemm_tile : gemm_current_tile()};
   if (eligible && L.graph.exec && L.graph.key == key && L.graph.gen == g_alloc_gen.load() &&
       L.graph_out) {
     PPH_HIP(hipGraphLaunch(L.graph.exec, s));
@@ -1955,8 +1977,8 @@ int pph_destroy(pph_handle* hh) {
 
 // ---------------------------------------------------------------- measurement hook
 extern "C" int pph_debug_reduction(pph_handle* hh, int32_t mode, int32_t band) {
-  if (mode < -1 || mode > 1)
-    return fail(nullptr, "pph_debug_reduction: mode must be -1, 0 or 1", PPH_ERR_ARG);
+  if (mode < -1 || mode > 2)
+    return fail(nullptr, "pph_debug_reduction: mode must be -1, 0, 1 or 2", PPH_ERR_ARG);
   if (band != 0 && band != 16 && band != 32)
     return fail(nullptr, "pph_debug_reduction: band must be 0 (keep), 16 or 32", PPH_ERR_ARG);
   Handle* h = reinterpret_cast<Handle*>(hh);
@@ -1982,6 +2004,21 @@ extern "C" int pph_debug_graphs(void* hh, int32_t mode, int64_t* captures, int64
   }
   if (captures) *captures = c;
   if (replays) *replays = r;
+  return PPH_OK;
+}
+
+extern "C" int pph_debug_gemm_tile(void* hh, int32_t tile) {
+  if (tile < 0 || tile >= kGemmTileCount)
+    return fail(nullptr, "pph_debug_gemm_tile: tile must be 0 (default), 1 (64x64), 2 (128x64), "
+                         "3 (64x128) or 4 (128x128)", PPH_ERR_ARG);
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) {
+    set_gemm_tile_default(tile ? tile : (int)kGemmTile64x64);
+    return PPH_OK;
+  }
+  if (tile == kGemmTile128x128 && h->opts.precision == 'D')
+    return fail(h, "pph_debug_gemm_tile: the 128x128 tile is FP32 only", PPH_ERR_ARG);
+  h->gemm_tile = tile;
   return PPH_OK;
 }
 

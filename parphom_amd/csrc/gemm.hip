@@ -347,6 +347,42 @@ void launch_gemm(const GemmParams<T>& p, hipStream_t s) {
   }
 }
 
+// ---------------------------------------------------------------- tile shapes
+// The shape a product with tile 0 takes: the calling thread's scope, else the
+// process default (set by pph_debug_gemm_tile with no handle), else 64 x 64.
+static int g_gemm_tile_default = kGemmTile64x64;
+static thread_local int t_gemm_tile = kGemmTileDefault;
+
+void set_gemm_tile_default(int tile) { g_gemm_tile_default = tile; }
+int gemm_current_tile() { return t_gemm_tile != kGemmTileDefault ? t_gemm_tile : g_gemm_tile_default; }
+gemm_tile_scope::gemm_tile_scope(int tile) : saved(t_gemm_tile) { t_gemm_tile = tile; }
+gemm_tile_scope::~gemm_tile_scope() { t_gemm_tile = saved; }
+
+// (32 WM) x (32 WN) tiles, three or four real products per complex step
+template <typename T, bool CA, bool CB, int WM, int WN>
+static void launch_shape(const GemmParams<T>& p, int64_t zdim, hipStream_t s) {
+  const dim3 g((unsigned)ceil_div(p.m, 32 * WM), (unsigned)ceil_div(p.n, 32 * WN), (unsigned)zdim);
+  if (p.four_m)
+    hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, WM, WN, false>), g, dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, WM, WN>), g, dim3(256), 0, s, p);
+  PPH_CHECK_LAUNCH();
+}
+
+template <typename T, bool CA, bool CB>
+static void launch_tiled(const GemmParams<T>& p, int tile, int64_t zdim, hipStream_t s) {
+  switch (tile) {
+    case kGemmTile128x64: launch_shape<T, CA, CB, 4, 2>(p, zdim, s); break;
+    case kGemmTile64x128: launch_shape<T, CA, CB, 2, 4>(p, zdim, s); break;
+    case kGemmTile128x128:
+      // FP64 accumulators of a 128 x 128 tile do not fit the register file
+      if constexpr (sizeof(T) == 8) throw std::runtime_error("launch_gemm: the 128 x 128 tile is FP32 only");
+      else launch_shape<T, CA, CB, 4, 4>(p, zdim, s);
+      break;
+    default: launch_shape<T, CA, CB, 2, 2>(p, zdim, s); break;
+  }
+}
+
 // the tile kernels of launch_gemm (arguments checked by the caller)
 template <typename T>
 void launch_gemm_tiles(const GemmParams<T>& p, hipStream_t s) {
@@ -373,34 +409,21 @@ void launch_gemm_tiles(const GemmParams<T>& p, hipStream_t s) {
     PPH_GEMM_CASE32(true, true)
 #undef PPH_GEMM_CASE32
   }
-  dim3 grid((unsigned)ceil_div(p.m, 64), (unsigned)ceil_div(p.n, 64),
-            (unsigned)(p.batch * bin * ks));
-#define PPH_GEMM_CASE(CA, CB, DT)                                                  \
-  if (ca == CA && cb == CB && dot == DT) {                                         \
-    hipLaunchKernelGGL((k_zgemm<T, CA, CB, DT>), grid, dim3(256), 0, s, p);        \
-    PPH_CHECK_LAUNCH();                                                            \
-    return;                                                                        \
+  const int64_t zdim = p.batch * bin * ks;
+  if (!dot) {
+    const int tile = p.tile != kGemmTileDefault ? p.tile : gemm_current_tile();
+    if (tile < kGemmTile64x64 || tile >= kGemmTileCount)
+      throw std::runtime_error("launch_gemm: unknown tile shape " + std::to_string(tile));
+    if (!ca && !cb) return launch_tiled<T, false, false>(p, tile, zdim, s);
+    if (ca && !cb) return launch_tiled<T, true, false>(p, tile, zdim, s);
+    if (!ca && cb) return launch_tiled<T, false, true>(p, tile, zdim, s);
+    return launch_tiled<T, true, true>(p, tile, zdim, s);
   }
-  if (p.four_m && !dot) {
-#define PPH_GEMM_CASE4(CA, CB)                                                     \
-    if (ca == CA && cb == CB) {                                                    \
-      hipLaunchKernelGGL((k_zgemm<T, CA, CB, false, 2, 2, false>), grid, dim3(256), 0, s, p); \
-      PPH_CHECK_LAUNCH();                                                          \
-      return;                                                                      \
-    }
-    PPH_GEMM_CASE4(false, false)
-    PPH_GEMM_CASE4(true, false)
-    PPH_GEMM_CASE4(false, true)
-    PPH_GEMM_CASE4(true, true)
-#undef PPH_GEMM_CASE4
-  }
-  PPH_GEMM_CASE(false, false, false)
-  PPH_GEMM_CASE(true, false, false)
-  PPH_GEMM_CASE(false, true, false)
-  PPH_GEMM_CASE(true, true, false)
-  PPH_GEMM_CASE(false, false, true)
-#undef PPH_GEMM_CASE
-  throw std::runtime_error("launch_gemm: unsupported op combination");
+  // the dot epilogue: 64 x 64 tiles (its partial slots are per 64-row tile)
+  if (ca || cb) throw std::runtime_error("launch_gemm: the dot epilogue takes op 'N' for both operands");
+  const dim3 grid((unsigned)ceil_div(p.m, 64), (unsigned)ceil_div(p.n, 64), (unsigned)zdim);
+  hipLaunchKernelGGL((k_zgemm<T, false, false, true>), grid, dim3(256), 0, s, p);
+  PPH_CHECK_LAUNCH();
 }
 
 template void launch_gemm<double>(const GemmParams<double>&, hipStream_t);

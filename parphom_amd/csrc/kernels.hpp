@@ -132,6 +132,10 @@ struct GemmParams {
   // four_m != 0: four real products per complex step instead of three (no
   // cancellation in the imaginary part; a third more matrix-core work)
   int four_m;
+  // tile shape of a product without the dot epilogue (kGemmTile*): 0 takes
+  // the calling thread's current shape (gemm_tile_scope), which defaults to
+  // 64 x 64
+  int tile;
   // device workspace of gemm_ws_bytes(p) bytes when that is nonzero: the dot
   // epilogue's per-row-tile partials or split-K's per-piece products, summed
   // in a fixed order so the results do not depend on scheduling
@@ -141,6 +145,21 @@ struct GemmParams {
 template <typename T> int64_t gemm_ws_bytes(const GemmParams<T>& p);
 
 template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);
+// GEMM tile shapes (rows x columns of C per 256-thread workgroup); products
+// with the dot epilogue and outputs at most 32 columns wide keep their own
+// (64 x 64 and 64 x 32)
+enum { kGemmTileDefault = 0, kGemmTile64x64 = 1, kGemmTile128x64 = 2, kGemmTile64x128 = 3,
+       kGemmTile128x128 = 4, kGemmTileCount = 5 };
+// The shape GEMMs with tile 0 take on the calling host thread while the scope
+// lives (each lane runs its batches on its own thread); restored afterwards.
+struct gemm_tile_scope {
+  explicit gemm_tile_scope(int tile);
+  ~gemm_tile_scope();
+  int saved;
+};
+int gemm_current_tile();
+// the shape of threads without a scope (initially 64 x 64)
+void set_gemm_tile_default(int tile);
 
 // ---------------------------------------------------------------- tridiagonalisation
 template <typename T>
@@ -170,6 +189,14 @@ constexpr int64_t kHemvTimeStride = 8;
 template <typename T> void tridiagonalise(const TridiagWork<T>& w, hipStream_t s);
 // hemv tiles of the trailing matrix at column 0 (the most any column has)
 inline int64_t hemv_tiles(int64_t n) { return (n - 1 + 15 + 63) / 64; }
+
+// ---------------------------------------------------------------- small matrices (small.hip)
+// The whole reduction of a matrix of order n <= kSmallMaxN in one launch, one
+// workgroup per matrix with the matrix in LDS.  Uses A (lower triangle read),
+// V (vcols columns of n rows per matrix written, zero past column n - 2), tau,
+// d, e of w; the panel, partial and scalar buffers are not touched.
+template <typename T> constexpr int64_t kSmallMaxN = sizeof(T) == 8 ? 96 : 128;
+template <typename T> void tridiagonalise_small(const TridiagWork<T>& w, int64_t vcols, hipStream_t s);
 
 // ---------------------------------------------------------------- two-stage reduction (twostage.hip)
 // Dense -> band of half-bandwidth bw (he2hb: panel QR of bw columns, then the

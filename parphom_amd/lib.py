@@ -43,7 +43,10 @@ EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expe
             "pph_fc_symmetrize", "pph_spring_fc", "pph_destroy",
             "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
-            "pph_debug_sparse", "pph_debug_reduction", "pph_debug_graphs")
+            "pph_debug_sparse", "pph_debug_reduction", "pph_debug_graphs", "pph_debug_gemm_tile")
+
+# GEMM tile shapes of pph_debug_gemm_tile (0: the process default)
+GEMM_TILES = {"64x64": 1, "128x64": 2, "64x128": 3, "128x128": 4}
 
 PHASES = ("assemble", "tridiag", "tri_eig", "back_transform", "refine", "velocity", "total",
           "hemv_launches", "he2hb", "hb2st", "q2")
@@ -84,6 +87,8 @@ def load(path: str = LIB_PATH):
     lib.pph_debug_reduction.restype = C.c_int
     lib.pph_debug_graphs.argtypes = [C.c_void_p, C.c_int32, _i64p, _i64p]
     lib.pph_debug_graphs.restype = C.c_int
+    lib.pph_debug_gemm_tile.argtypes = [C.c_void_p, C.c_int32]
+    lib.pph_debug_gemm_tile.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -268,9 +273,16 @@ class Handle:
 
     def reduction(self, mode=1, band=0):
         """Reduction to the tridiagonal on this handle: 0 one-stage, 1
-        two-stage (dense -> band -> tridiagonal), -1 automatic; band 16 or 32
-        (0 keeps the current one)."""
+        two-stage (dense -> band -> tridiagonal), 2 the one-launch reduction of
+        small matrices (n <= 96 FP64, 128 FP32; one-stage above), -1
+        automatic; band 16 or 32 (0 keeps the current one)."""
         _check(load().pph_debug_reduction(self._h, mode, band), self._h)
+
+    def gemm_tile(self, tile=0):
+        """Tile shape of the eigensolver's matrix-core GEMMs on this handle:
+        a GEMM_TILES name or code (0: the process default)."""
+        code = GEMM_TILES[tile] if isinstance(tile, str) else int(tile)
+        _check(load().pph_debug_gemm_tile(self._h, code), self._h)
 
     def graphs(self, mode=-1):
         """Batch graphs (range A): 1 on (the default), 0 off, -1 unchanged.
@@ -305,10 +317,18 @@ class Handle:
         return out.transpose(0, 2, 1)
 
 
+def set_default_gemm_tile(tile="64x64"):
+    """Process default of the GEMM tile shape (a GEMM_TILES name or code;
+    also used by debug_zgemm and the temporary handles of heevx)."""
+    code = GEMM_TILES[tile] if isinstance(tile, str) else int(tile)
+    _check(load().pph_debug_gemm_tile(None, code))
+
+
 def set_default_reduction(mode=-1, band=0):
     """Process default of the reduction for handles left on automatic (also
-    the temporary handles of heevx): 0 one-stage, 1 two-stage, -1 the
-    library's choice."""
+    the temporary handles of heevx): 0 one-stage, 1 two-stage, 2 the
+    one-launch small-matrix reduction where the order fits, -1 the library's
+    choice."""
     _check(load().pph_debug_reduction(None, mode, band))
 
 

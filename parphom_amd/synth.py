@@ -314,6 +314,7 @@ def q_random(nq: int, seed: int) -> np.ndarray:
 # ---------------------------------------------------------------- workloads
 WORKLOADS = {
     # name: (m, n, fc seed, q builder)
+    "tbg_1_2": (1, 2, 11, lambda: q_grid(64)),
     "tbg_3_4": (3, 4, 1003, lambda: q_path_gkmg()),
     "tbg_8_9": (8, 9, 2008, lambda: q_grid_shifted(16, 2)),
     "tbg_5_6": (5, 6, 3005, lambda: q_grid(64)),

@@ -290,7 +290,9 @@ def test_run_two_streams(gpu_available, prec):
 def test_profile_before_first_two_lane_run(gpu_available):
     """Phase timers switched on before any batch has run: the second lane,
     created by pph_run, must time its batches too (every batch counted once,
-    every eighth hemv launch of each batch timed)."""
+    every eighth hemv launch of each batch timed; the one-stage reduction is
+    forced, since at n = 84 the library picks the small-matrix kernel, which
+    has no hemv launches)."""
     lib = _lib()
     st, fc = U.structure(1, 2, 11)
     from parphom_amd import synth
@@ -298,6 +300,7 @@ def test_profile_before_first_two_lane_run(gpu_available):
     n = 3 * st.natom
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
                     batch=2, streams=2) as h:
+        h.reduction(0)
         h.profile(enable=True, reset=True)
         res = h.run(qs)
         prof = h.profile(enable=False)

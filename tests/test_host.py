@@ -229,8 +229,8 @@ def test_tuner_picks_fastest_candidate_and_records_rejections():
     res = tune.tune(None, None, None, None, None, np.zeros((5, 3)), batches=(16, 32),
                     streams=(1, 2), panels=(32, 40), backs=(64, 128), reps=2,
                     handle_factory=StubHandle, clock=lambda: now[0])
-    assert res["config"] == {"batch": 16, "streams": 2, "reduction": -1, "nb_panel": 32,
-                             "nb_back": 128}
+    assert res["config"] == {"batch": 16, "streams": 2, "reduction": -1, "tile": "64x64",
+                             "nb_panel": 32, "nb_back": 128}
     assert abs(res["qps"] - 100.0) < 1e-9
     rejected = [r for r in res["table"] if r["nb_panel"] == 40]
     assert len(rejected) == 8 and all(r["qps"] is None and "error" in r for r in rejected)
@@ -274,6 +274,51 @@ def test_tuner_records_failing_candidates_and_goes_on():
     failed = [r for r in t if r["batch"] == 16 and r["streams"] == 2 and r["nb_back"] == 64]
     assert len(failed) == 1 and failed[0]["qps"] is None and "out of memory" in failed[0]["error"]
     assert res["config"]["nb_back"] == 128 and abs(res["qps"] - 100.0) < 1e-9
+
+
+def test_tuner_searches_tile_shapes_and_reductions():
+    """The tile shape and the reduction are search dimensions: each candidate
+    sets them on its handle (tune.apply) before its calls, a shape the
+    handle rejects is recorded with its error, and the fastest shape wins."""
+    from parphom_amd import tune
+    now = [0.0]
+    seen = []
+
+    class StubHandle:
+        def __init__(self, *a, batch, streams, **k):
+            self.tile, self.red = None, -1
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            return False
+
+        def reduction(self, mode):
+            self.red = mode
+
+        def gemm_tile(self, tile):
+            if tile == "128x128":
+                raise ValueError("the 128x128 tile is FP32 only")
+            self.tile = tile
+
+        def blocking(self, nbp, nbb):
+            pass
+
+        def run(self, q):
+            seen.append((self.red, self.tile))
+            cost = {"64x64": 3.0, "128x64": 2.0, "64x128": 2.5}[self.tile] + (0.5 if self.red == 0 else 0)
+            now[0] += cost * len(q) / 100.0
+
+    res = tune.tune(None, None, None, None, None, np.zeros((4, 3)), batches=(8,), streams=(1,),
+                    panels=(32,), backs=(128,), reductions=(0, 1),
+                    tiles=("64x64", "128x64", "64x128", "128x128"), handle_factory=StubHandle,
+                    clock=lambda: now[0])
+    assert res["config"]["tile"] == "128x64" and res["config"]["reduction"] == 1
+    assert len(res["table"]) == 8
+    bad = [r for r in res["table"] if r["tile"] == "128x128"]
+    assert len(bad) == 2 and all(r["qps"] is None and "FP32 only" in r["error"] for r in bad)
+    assert set(seen) == {(r, t) for r in (0, 1) for t in ("64x64", "128x64", "64x128")}
 
 
 def test_complement_correction_formula():
