@@ -887,19 +887,33 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
     L.tm.begin(PH_BACK, s);
     if (two) {
       // Z = Q1 (Q2 Z_T): the bulge-chasing reflectors first, then the he2hb
-      // panels (their reflectors start bw rows below their column)
-      launch_zt_to_z<T>(te.zt, ldzt, te.strideZt, bw.Z, bw.ldz, bw.strideZ, n, mmax, B, s);
+      // panels (their reflectors start bw rows below their column).  On the
+      // FP32 path Q2 runs on a row-major copy, made in place from the
+      // row-major FP64 Z_T (same element size): a window of rows is then
+      // contiguous in HBM (1 KB per row and workgroup instead of 256-byte
+      // pieces of 128 columns); one transposition gives Q1 its column-major Z.
+      const bool rows = fp32 && n * ldzt * (int64_t)sizeof(cplx<T>) <= 0x7FFFFFFF;
       L.tsVTb.ensure((size_t)B * G2 * kmax2 * 2 * bwd * bwd * sizeof(cplx<T>) * 2);
       Q2Work<T> q2{};
       q2.n = n; q2.batch = B; q2.bw = bwd; q2.kmax = kmax2; q2.m = mmax;
       q2.V2 = ts.V2; q2.tau2 = ts.tau2; q2.strideV2 = ts.strideV2; q2.strideTau2 = ts.strideTau2;
       q2.VTb = L.tsVTb.as<cplx<T>>(); q2.strideVTb = G2 * kmax2 * 2 * bwd * bwd;
       q2.Vb = q2.VTb + B * q2.strideVTb;
-      q2.Z = bw.Z; q2.ldz = bw.ldz; q2.strideZ = bw.strideZ;
+      if (rows) {
+        launch_zt_to_c64_rows(te.zt, ldzt, te.strideZt, n, mmax, B, s);
+        q2.Z = reinterpret_cast<cplx<T>*>(te.zt); q2.ldz = ldzt; q2.strideZ = te.strideZt;
+        q2.rowmajor = 1;
+      } else {
+        launch_zt_to_z<T>(te.zt, ldzt, te.strideZt, bw.Z, bw.ldz, bw.strideZ, n, mmax, B, s);
+        q2.Z = bw.Z; q2.ldz = bw.ldz; q2.strideZ = bw.strideZ;
+      }
       prepare_q2<T>(q2, s);
       L.tm.begin(PH_Q2, s);
       apply_q2<T>(q2, s);
       L.tm.end(PH_Q2, s);
+      if (rows)
+        launch_zr_to_z(reinterpret_cast<const cplx<float>*>(te.zt), ldzt, te.strideZt,
+                       reinterpret_cast<cplx<float>*>(bw.Z), bw.ldz, bw.strideZ, n, mmax, B, s);
       bw.shift = bwd;
       bw.z_ready = true;
     }

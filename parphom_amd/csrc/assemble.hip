@@ -133,6 +133,7 @@ struct AsmArgs {
   const double* cd_dc;      // 3 x 3: crystal step for each Cartesian direction (row a)
   double cd_norm;           // |delta|
   int64_t N, nb, mode;
+  int64_t qchunk;           // q-points per workgroup along grid z (blockIdx.z * qchunk ...)
   int64_t ldD;              // leading dimension of D64 / D32 (>= n; dD uses n)
   const uint8_t* nzflag;    // N x N: 0 where the block of Phi_sym is zero (may be null)
   cplx<double>* D64;
@@ -185,7 +186,8 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs A) {
 
   const bool want_d = (A.dD64 != nullptr) || (A.dD32 != nullptr);
   const size_t mstride = (size_t)n * n;
-  for (int64_t iq = 0; iq < A.nb; ++iq) {
+  const int64_t q0 = (int64_t)blockIdx.z * A.qchunk, q1 = min(A.nb, q0 + A.qchunk);
+  for (int64_t iq = q0; iq < q1; ++iq) {
     const double* q = A.q + 3 * iq;
     if (i < N && !nz) {
       sp0r[tid] = sp0i[tid] = 0.0;
@@ -406,7 +408,13 @@ void launch_assemble(const AsmParams& P, hipStream_t s) {
   A.ldD = P.ldD > 0 ? P.ldD : 3 * P.N;
   A.nzflag = P.nzflag;
   A.D64 = P.D64; A.D32 = P.D32; A.dD64 = P.dD64; A.dD32 = P.dD32;
-  dim3 grid((unsigned)ceil_div(P.N, 256), (unsigned)P.N);
+  // one workgroup per (256 atoms i, atom j); small cells also split the batch's
+  // q-points over grid z so that the launch has a few thousand workgroups
+  const int64_t wg = ceil_div(P.N, 256) * P.N;
+  int64_t zs = std::min<int64_t>(std::max<int64_t>(P.nb, 1), std::max<int64_t>(1, 4096 / wg));
+  A.qchunk = ceil_div(std::max<int64_t>(P.nb, 1), zs);
+  zs = ceil_div(std::max<int64_t>(P.nb, 1), A.qchunk);
+  dim3 grid((unsigned)ceil_div(P.N, 256), (unsigned)P.N, (unsigned)zs);
   if (A.mode == 1) hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, s, A);
   else hipLaunchKernelGGL(k_assemble<0>, grid, dim3(256), 0, s, A);
   PPH_CHECK_LAUNCH();

@@ -32,6 +32,52 @@ __global__ void __launch_bounds__(256) k_zt_to_z(const double* __restrict__ zt, 
   }
 }
 
+// zt[i ldzt + k] (FP64, real) -> complex64 (value, 0) in the same 8 bytes
+__global__ void __launch_bounds__(256) k_zt_to_c64_rows(double* __restrict__ zt, int64_t ldzt,
+                                                        int64_t strideZt, int64_t n, int64_t m) {
+  const int64_t b = blockIdx.z, i = blockIdx.y;
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || k >= m) return;
+  double* p = zt + b * strideZt + i * ldzt + k;
+  const double v = *p;
+  *reinterpret_cast<cplx<float>*>(p) = cplx<float>{(float)v, 0.f};
+}
+
+// Z(i, k) = Zr(i, k): row-major to column-major through a 32 x 32 LDS tile
+__global__ void __launch_bounds__(256) k_zr_to_z(const cplx<float>* __restrict__ Zr, int64_t ldr,
+                                                 int64_t strideZr, cplx<float>* __restrict__ Z,
+                                                 int64_t ldz, int64_t strideZ, int64_t n, int64_t m) {
+  __shared__ cplx<float> tile[32][33];
+  const int64_t b = blockIdx.z;
+  const int64_t k0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = i0 + r, k = k0 + tx;
+    tile[r][tx] = (i < n && k < m) ? Zr[b * strideZr + i * ldr + k] : cplx<float>{0, 0};
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t k = k0 + r, i = i0 + tx;
+    if (i < n && k < m) Z[b * strideZ + i + k * ldz] = tile[tx][r];
+  }
+}
+
+void launch_zt_to_c64_rows(double* zt, int64_t ldzt, int64_t strideZt, int64_t n, int64_t m,
+                           int64_t batch, hipStream_t s) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid((unsigned)ceil_div(m, 256), (unsigned)n, (unsigned)batch);
+  hipLaunchKernelGGL(k_zt_to_c64_rows, grid, dim3(256), 0, s, zt, ldzt, strideZt, n, m);
+  PPH_CHECK_LAUNCH();
+}
+
+void launch_zr_to_z(const cplx<float>* Zr, int64_t ldr, int64_t strideZr, cplx<float>* Z, int64_t ldz,
+                    int64_t strideZ, int64_t n, int64_t m, int64_t batch, hipStream_t s) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid((unsigned)ceil_div(m, 32), (unsigned)ceil_div(n, 32), (unsigned)batch);
+  hipLaunchKernelGGL(k_zr_to_z, grid, dim3(256), 0, s, Zr, ldr, strideZr, Z, ldz, strideZ, n, m);
+  PPH_CHECK_LAUNCH();
+}
+
 // T_p from G_p = V_p^H V_p: T(k,k) = tau_k, T(0:k, k) = -tau_k T(0:k, 0:k) G(0:k, k).
 // One workgroup (nb threads) per (block, matrix); entries beyond the block width are 0.
 template <typename T>

@@ -241,12 +241,20 @@ struct Q2Work {
   const cplx<T>* V2; const cplx<T>* tau2; int64_t strideV2, strideTau2;
   cplx<T>* VTb; cplx<T>* Vb; int64_t strideVTb;    // VT and V of every block, same layout
   cplx<T>* Z; int64_t ldz, strideZ;
+  int rowmajor;          // Z(row, c) at Z[row ldz + c] instead of Z[row + c ldz]
 };
 // V and VT of every block (k_q2_prep), then Z <- Q2 Z (k_q2_apply)
 template <typename T> void prepare_q2(const Q2Work<T>& w, hipStream_t s);
 template <typename T> void apply_q2(const Q2Work<T>& w, hipStream_t s);
 template <typename T>
 void launch_zt_to_z(const double* zt, int64_t ldzt, int64_t strideZt, cplx<T>* Z, int64_t ldz,
+                    int64_t strideZ, int64_t n, int64_t m, int64_t batch, hipStream_t s);
+// In place: the row-major FP64 tridiagonal eigenvectors zt (n rows of ldzt)
+// become the row-major complex64 Z(i, k) = zt[i ldzt + k] in the same bytes
+void launch_zt_to_c64_rows(double* zt, int64_t ldzt, int64_t strideZt, int64_t n, int64_t m,
+                           int64_t batch, hipStream_t s);
+// Z (column-major, ldz) = Zr (row-major, ldr), complex64
+void launch_zr_to_z(const cplx<float>* Zr, int64_t ldr, int64_t strideZr, cplx<float>* Z, int64_t ldz,
                     int64_t strideZ, int64_t n, int64_t m, int64_t batch, hipStream_t s);
 
 // ---------------------------------------------------------------- tridiagonal eigensolver (FP64)

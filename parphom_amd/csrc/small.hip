@@ -17,7 +17,7 @@
 // off-diagonal beta_j), tau_j, and V with column j = v_j explicitly (zero
 // above row j + 1, one at row j + 1) plus zero columns up to the V stride.
 // Every sum runs in a fixed order (row partials over fixed column classes,
-// block sums in a fixed tree), so results repeat bit for bit.
+// wave sums in a fixed butterfly), so results repeat bit for bit.
 //
 // This is the reduction half of pzheevx (diagonalize.F90:73-96) for orders up
 // to kSmallMaxN (the matrix in LDS: 96 for complex128, 128 for complex64).
@@ -35,8 +35,6 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_hetrd(TridiagWork<T> w,
   constexpr int NP = 4;                       // column classes of the mat-vec partials
   __shared__ cplx<T> S[NM * NM];              // column-major, S(r, c) = S[r + c NM]
   __shared__ cplx<T> vv[NM], ww[NM], pp[NP][NM];
-  __shared__ double sh[NT / WAVE];
-  __shared__ double rfs[4];                   // tau (re, im), scale (re, im)
   const int64_t b = blockIdx.x;
   const int n = (int)w.n;
   const int tid = threadIdx.x;
@@ -59,32 +57,35 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_hetrd(TridiagWork<T> w,
   for (int64_t k = tid; k < (vcols - (n - 1)) * n; k += NT) V[(int64_t)(n - 1) * n + k] = cplx<T>{0, 0};
   __syncthreads();
 
+  // Per column four barriers: wave 0 forms the reflector (its norm and the
+  // dot p^H v are wave sums, m < 128 entries), all waves form the mat-vec
+  // partials and the rank-2 update.
+  Reflector rf{};
   for (int j = 0; j < n - 1; ++j) {
     const int o = j + 1, m = n - o;           // the trailing matrix S(o:, o:) has order m
-    double x2 = 0.0;
-    for (int r = o + 1 + tid; r < n; r += NT) x2 += (double)cabs2(S[r + j * NM]);
-    x2 = block_sum(x2, sh);
-    if (tid == 0) {
+    if (tid < WAVE) {
+      double x2 = 0.0;
+      for (int r = o + 1 + tid; r < n; r += WAVE) x2 += (double)cabs2(S[r + j * NM]);
+      x2 = wave_sum(x2);
       const cplx<T> al = S[o + j * NM];
-      const Reflector rf = make_reflector(x2, (double)al.x, (double)al.y);
-      d[j] = S[j + j * NM].x;
-      e[j] = (T)rf.beta;
-      tau[j] = cplx<T>{(T)rf.tr, (T)rf.ti};
-      rfs[0] = rf.tr; rfs[1] = rf.ti; rfs[2] = rf.sr; rfs[3] = rf.si;
+      rf = make_reflector(x2, (double)al.x, (double)al.y);
+      if (tid == 0) {
+        d[j] = S[j + j * NM].x;
+        e[j] = (T)rf.beta;
+        tau[j] = cplx<T>{(T)rf.tr, (T)rf.ti};
+      }
+      // v = (1, scale x) into LDS and into V(:, j), zero above row j + 1
+      const cplx<T> sc{(T)rf.sr, (T)rf.si};
+      for (int r = tid; r < n; r += WAVE) {
+        cplx<T> v{0, 0};
+        if (r == o) v = cplx<T>{1, 0};
+        else if (r > o) v = cmul(sc, S[r + j * NM]);
+        if (r >= o) vv[r - o] = v;
+        V[r + (int64_t)j * n] = v;
+      }
     }
     __syncthreads();
-    const cplx<T> sc{(T)rfs[2], (T)rfs[3]};
-    const cplx<T> tv{(T)rfs[0], (T)rfs[1]};
-    // v = (1, scale x) into LDS and into V(:, j), zero above row j + 1
-    for (int r = tid; r < n; r += NT) {
-      cplx<T> v{0, 0};
-      if (r == o) v = cplx<T>{1, 0};
-      else if (r > o) v = cmul(sc, S[r + j * NM]);
-      if (r >= o) vv[r - o] = v;
-      V[r + (int64_t)j * n] = v;
-    }
-    __syncthreads();
-    // p = S v: NP partial sums per row, partial h over the columns c = h mod NP
+    // S v: NP partial sums per row, partial h over the columns c = h mod NP
     for (int u = tid; u < NP * m; u += NT) {
       const int r = u % m, h = u / m;
       const cplx<T>* col = S + (o + r) + o * NM;
@@ -93,23 +94,26 @@ __global__ void __launch_bounds__(kSmallThreads) k_small_hetrd(TridiagWork<T> w,
       pp[h][r] = acc;
     }
     __syncthreads();
-    // p = tau (sum of the partials), and p^H v
-    double yr = 0.0, yi = 0.0;
-    for (int r = tid; r < m; r += NT) {
-      cplx<T> y = pp[0][r];
+    if (tid < WAVE) {
+      // p = tau (sum of the partials), p^H v, then w = p + alpha2 v with
+      // alpha2 = -tau/2 (p^H v)
+      const cplx<T> tv{(T)rf.tr, (T)rf.ti};
+      double yr = 0.0, yi = 0.0;
+      for (int r = tid; r < m; r += WAVE) {
+        cplx<T> y = pp[0][r];
 #pragma unroll
-      for (int h = 1; h < NP; ++h) y = cadd(y, pp[h][r]);
-      y = cmul(tv, y);
-      ww[r] = y;
-      const cplx<T> v = vv[r];
-      yr += (double)y.x * v.x + (double)y.y * v.y;
-      yi += (double)y.x * v.y - (double)y.y * v.x;
+        for (int h = 1; h < NP; ++h) y = cadd(y, pp[h][r]);
+        y = cmul(tv, y);
+        ww[r] = y;
+        const cplx<T> v = vv[r];
+        yr += (double)y.x * v.x + (double)y.y * v.y;
+        yi += (double)y.x * v.y - (double)y.y * v.x;
+      }
+      yr = wave_sum(yr);
+      yi = wave_sum(yi);
+      const cplx<T> a2{(T)(-0.5 * (rf.tr * yr - rf.ti * yi)), (T)(-0.5 * (rf.tr * yi + rf.ti * yr))};
+      for (int r = tid; r < m; r += WAVE) ww[r] = cadd(ww[r], cmul(a2, vv[r]));
     }
-    yr = block_sum(yr, sh);
-    yi = block_sum(yi, sh);
-    // w = p + alpha2 v with alpha2 = -tau/2 (p^H v)
-    const cplx<T> a2{(T)(-0.5 * (rfs[0] * yr - rfs[1] * yi)), (T)(-0.5 * (rfs[0] * yi + rfs[1] * yr))};
-    for (int r = tid; r < m; r += NT) ww[r] = cadd(ww[r], cmul(a2, vv[r]));
     __syncthreads();
     // S -= v w^H + w v^H (the diagonal kept real)
     for (int k = tid; k < m * m; k += NT) {
@@ -137,9 +141,13 @@ void tridiagonalise_small(const TridiagWork<T>& w, int64_t vcols, hipStream_t s)
   if (vcols < n || w.strideV < n * vcols)
     throw std::runtime_error("tridiagonalise_small: V holds fewer than n columns per matrix");
   const dim3 grid((unsigned)w.batch), block(kSmallThreads);
+  // the LDS matrix sized to the order (smaller tiles leave room for more
+  // workgroups per CU)
   if (n <= 32) hipLaunchKernelGGL((k_small_hetrd<T, 32>), grid, block, 0, s, w, vcols);
   else if (n <= 64) hipLaunchKernelGGL((k_small_hetrd<T, 64>), grid, block, 0, s, w, vcols);
-  else hipLaunchKernelGGL((k_small_hetrd<T, kSmallMaxN<T>>), grid, block, 0, s, w, vcols);
+  else if (n <= 96) hipLaunchKernelGGL((k_small_hetrd<T, 96>), grid, block, 0, s, w, vcols);
+  else if constexpr (kSmallMaxN<T> > 96)
+    hipLaunchKernelGGL((k_small_hetrd<T, kSmallMaxN<T>>), grid, block, 0, s, w, vcols);
   PPH_CHECK_LAUNCH();
 }
 

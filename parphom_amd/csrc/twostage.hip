@@ -746,42 +746,96 @@ __device__ __forceinline__ void tile3m(FA&& A, FB&& B, int k0, int k1, cplx<T> o
   for (int r = 0; r < 4; ++r) out[r] = cplx<T>{p1[r] - p2[r], p3[r] - p1[r] - p2[r]};
 }
 
+// Buffer loads and stores of one complex element through a raw buffer
+// resource: an offset past the resource reads as zero and drops the store,
+// so lanes outside the matrix need no branch.
+typedef unsigned int q2u2 __attribute__((ext_vector_type(2)));
+typedef unsigned int q2u4 __attribute__((ext_vector_type(4)));
+constexpr unsigned kQ2Off = 0xFFFFFFFFu;         // an offset no resource covers
+// bits of the resource's last word for a raw (untyped) buffer
+constexpr int kRawBufferWord3 = 0x00020000;
+__device__ __forceinline__ cplx<float> q2_load(__amdgpu_buffer_rsrc_t r, unsigned off, float) {
+  return __builtin_bit_cast(cplx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ cplx<double> q2_load(__amdgpu_buffer_rsrc_t r, unsigned off, double) {
+  return __builtin_bit_cast(cplx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ void q2_store(cplx<float> v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(q2u2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void q2_store(cplx<double> v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(q2u4, v), r, off, 0, 0);
+}
+
 // Z <- Q2 Z for a strip of 16 NWV columns of one matrix per workgroup; wave w
 // owns columns 16 w .. 16 w + 15 of the strip.  The window of 2bw rows the
-// current block acts on lives in LDS as two halves of bw rows (slot (k + h) & 1
-// holds logical half h), so consecutive blocks of a group share a half and
-// each row of the strip is loaded and stored once per group.  Per block, on
-// the matrix cores and for the wave's own columns only (no cross-wave
-// dependency):
+// current block acts on lives in LDS as two halves of bw rows (block k: slot
+// k & 1 holds logical half 0, the other slot logical half 1), so consecutive
+// blocks of a group share a half and each row of the strip is loaded and
+// stored once per group.  Per block, on the matrix cores and for the wave's
+// own columns only (no cross-wave dependency):
 //   W = V^H Zw   (bw x 16, K = 2bw; reflectors i0.. meet rows i0 .. i0 + bw + 15)
 //   Zw -= VT W   (2bw x 16, K = bw; VT(rr, i) = 0 for i <= rr - bw)
 // with four real MFMAs per complex step (the three-product form's cancellation
-// in the imaginary part costs the FP32 eigenvectors a factor of accuracy)
-// V and VT of a block are shared by the waves: two barriers per block.  The
-// next block's V and VT and the wave's next half of Z are loaded into
-// registers while the current block computes.
-template <typename T, int BW, int NWV>
+// in the imaginary part costs the FP32 eigenvectors a factor of accuracy).
+// V and VT of a block are shared by the waves: two barriers per block.
+// The rows leave and enter the window while the matrix cores work: the next
+// block's V and VT and the rows of its new half are loaded into registers
+// during a block; at the start of the next block the half its predecessor
+// finished is read from its slot, and its stores to HBM and the writes of the
+// new rows into that slot are issued between the MFMAs of the first half of W
+// (K over logical half 0, which is already in place), before the second half
+// of W reads them.  Rows and columns outside the matrix go through the buffer
+// unit's range check (see q2_load).
+template <typename T, int BW, int NWV, bool ROWMAJOR>
 __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
   constexpr int NT = WAVE * NWV;
   constexpr int SW = 16 * NWV;
   constexpr int VE = (2 * BW * BW + NT - 1) / NT;   // V (and VT) elements per thread
   constexpr int ZE = BW * 16 / WAVE;                // elements of the wave's Z half per lane
+  constexpr int ES = (int)sizeof(cplx<T>);
   // rows padded by 128 bytes: the four rows one MFMA operand load spans then
   // fall in alternate halves of the LDS banks
-  constexpr int PC = 128 / (int)sizeof(cplx<T>);
+  constexpr int PC = 128 / ES;
   __shared__ cplx<T> Zw[2][BW][SW + PC];
   __shared__ cplx<T> Vs[2 * BW][BW + PC];           // V(rr, i)
   __shared__ cplx<T> VTs[BW][2 * BW + PC];          // VT(rr, i) at VTs[i][rr]
   __shared__ cplx<T> Ws[BW][SW + PC];
+  const int64_t strip = blockIdx.x;
   const int64_t n = w.n, b = blockIdx.y;
   const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
   const int cw = 16 * wave;                         // the wave's first column in the strip
-  const int64_t col0 = (int64_t)blockIdx.x * SW + cw;
-  cplx<T>* Z = w.Z + b * w.strideZ;
+  const int64_t col0 = strip * SW + cw;
   const cplx<T>* Vb = w.Vb + b * w.strideVTb;
   const cplx<T>* VTb = w.VTb + b * w.strideVTb;
   const int64_t G = ceil_div_dev(n - 1, BW);
-  cplx<T> pv[VE], pvt[VE], pz[ZE];
+  // the wave's columns of Z as a buffer resource (none when the wave lies
+  // past the last column); element (row, c) at byte (row + c ldz) ES, or
+  // (row ldz + c) ES for row-major Z (the host keeps the range below 2^31)
+  const int64_t ncol = max<int64_t>(0, min<int64_t>(16, w.m - col0));
+  const int64_t rbytes = ncol <= 0 ? 0
+                         : ROWMAJOR ? ((n - 1) * w.ldz + ncol) * ES : ((ncol - 1) * w.ldz + n) * ES;
+  const __amdgpu_buffer_rsrc_t zr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(w.Z + b * w.strideZ + (ncol > 0 ? (ROWMAJOR ? col0 : col0 * w.ldz) : 0)), (short)0,
+      (int)rbytes, kRawBufferWord3);
+  // Window moves, element q of a lane: column-major Z, lane -> row lane % bw
+  // and column lane / bw + (64 / bw) q (each access reads whole 128-byte runs
+  // of a column); row-major Z, lane -> column lane % 16 and row lane / 16 + 4 q
+  // (16 lanes read the wave's 128 bytes of a row, the workgroup's waves the
+  // strip's 1 KB; in LDS the four rows of an access fall in alternate bank
+  // halves)
+  const int zr_col = ROWMAJOR ? lane % 16 : lane / BW;
+  const int zr_row = ROWMAJOR ? lane / 16 : lane % BW;
+  auto zrow = [&](int q) { return ROWMAJOR ? zr_row + 4 * q : zr_row; };
+  auto zcol = [&](int q) { return ROWMAJOR ? zr_col : zr_col + (WAVE / BW) * q; };
+  auto zoff = [&](int64_t row0, int q, bool valid) -> unsigned {
+    const int64_t row = row0 + zrow(q);
+    const int64_t cc = zcol(q);
+    if (!(valid && row < n && cc < ncol)) return kQ2Off;
+    return (unsigned)((ROWMAJOR ? row * w.ldz + cc : row + cc * w.ldz) * ES);
+  };
+  auto zslot = [&](int slot, int q) -> cplx<T>& { return Zw[slot][zrow(q)][cw + zcol(q)]; };
+  cplx<T> pv[VE], pvt[VE], pz[ZE], zst[ZE];
   auto fetch_v = [&](int64_t g, int64_t k) {
     const int64_t blk = (g * w.kmax + k) * (int64_t)(2 * BW * BW);
 #pragma unroll
@@ -802,26 +856,9 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
       }
     }
   };
-  // the wave's 16 columns of a half: lane -> row lane % bw, columns lane / bw + (64 / bw) q
   auto fetch_z = [&](int64_t row0) {
 #pragma unroll
-    for (int q = 0; q < ZE; ++q) {
-      const int rr = lane % BW, cc = lane / BW + (WAVE / BW) * q;
-      const int64_t row = row0 + rr, col = col0 + cc;
-      pz[q] = (row < n && col < w.m) ? Z[row + col * w.ldz] : cplx<T>{0, 0};
-    }
-  };
-  auto put_z = [&](int slot) {
-#pragma unroll
-    for (int q = 0; q < ZE; ++q) Zw[slot][lane % BW][cw + lane / BW + (WAVE / BW) * q] = pz[q];
-  };
-  auto store_z = [&](int64_t row0, int slot) {
-#pragma unroll
-    for (int q = 0; q < ZE; ++q) {
-      const int rr = lane % BW, cc = lane / BW + (WAVE / BW) * q;
-      const int64_t row = row0 + rr, col = col0 + cc;
-      if (row < n && col < w.m) Z[row + col * w.ldz] = Zw[slot][rr][cw + cc];
-    }
+    for (int q = 0; q < ZE; ++q) pz[q] = q2_load(zr, zoff(row0, q, true), T(0));
   };
   auto wave_sync = []() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -831,46 +868,91 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
   for (int64_t g = G - 1; g >= 0; --g) {
     const int64_t j0 = g * BW;
     const int64_t Kg = hb2st_tasks(n, BW, j0);
+    // logical half 0 of block 0 into slot 0; its half 1 waits in registers
+    // for the first block's window move
     fetch_z(j0 + 1);
-    put_z(0);
+#pragma unroll
+    for (int q = 0; q < ZE; ++q) zslot(0, q) = pz[q];
     fetch_z(j0 + 1 + BW);
-    put_z(1);
     fetch_v(g, 0);
     for (int64_t k = 0; k < Kg; ++k) {
       const int64_t row0 = j0 + 1 + k * BW;
-      const int s0 = (int)(k & 1);                   // slot of logical half 0
+      const int s0 = (int)(k & 1), s1 = s0 ^ 1;      // slots of logical halves 0 and 1
       __syncthreads();                               // every wave is done with the previous V, VT
       put_v();
       __syncthreads();
+      // the rows the previous block finished (slot s1; none before block 1)
+#pragma unroll
+      for (int q = 0; q < ZE; ++q) zst[q] = zslot(s1, q);
+      cplx<T> pzn[ZE];
+#pragma unroll
+      for (int q = 0; q < ZE; ++q) pzn[q] = pz[q];
       if (k + 1 < Kg) {
         fetch_v(g, k + 1);
-        fetch_z(row0 + 2 * BW);
+        fetch_z(row0 + 2 * BW);                      // the next block's new half
       }
       // W = V^H Zw on the wave's columns: all row tiles of W in one K loop
-      // (their K ranges overlap and share the Zw operand), fully unrolled, so
-      // the LDS loads of later steps are issued ahead of the MFMAs
+      // (their K ranges overlap and share the Zw operand), fully unrolled.
+      // The LDS operands of step kk + 4 are read before the MFMAs of step kk
+      // (scheduling barriers keep them there), so their latency hides behind
+      // the matrix cores.  The first NMS steps also carry the window move
+      // (stores of the finished rows, the new rows into slot s1); logical
+      // half 1 is first read by the loads issued at step bw - 4, after all
+      // of them.
       {
         using acc_t = typename mfma<T>::acc;
         constexpr int NRT = BW / 16;
-        acc_t p1[NRT], p2[NRT], p3[NRT];
-#pragma unroll
-        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; p3[t] = p1[t]; }
-#pragma unroll
-        for (int kk = 0; kk < 2 * BW; kk += 4) {
+        constexpr int KS = 2 * BW;
+        constexpr int NMS = (BW / 4 - 1) < 4 ? (BW / 4 - 1) : 4;   // steps with a window move
+        constexpr int MV = (ZE + NMS - 1) / NMS;     // window elements moved per such step
+        // reflectors 16 t .. 16 t + 15 meet rows 16 t .. 16 t + bw + 15
+        auto live = [](int t, int kk) { return kk >= 16 * t && kk < 16 * t + 16 + BW; };
+        auto ld_z = [&](int kk) {
           const int kr = kk + lane / 16;
-          const cplx<T> bz = Zw[(s0 + kr / BW) & 1][kr % BW][cw + lane % 16];
+          return Zw[kr < BW ? s0 : s1][kr % BW][cw + lane % 16];
+        };
+        auto ld_v = [&](int kk, int t) { return Vs[kk + lane / 16][16 * t + lane % 16]; };
+        acc_t p1[NRT], p2[NRT];
+#pragma unroll
+        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; }
+        cplx<T> bz = ld_z(0), v[NRT];
+#pragma unroll
+        for (int t = 0; t < NRT; ++t) v[t] = live(t, 0) ? ld_v(0, t) : cplx<T>{0, 0};
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+          cplx<T> bzn{0, 0}, vn[NRT];
+#pragma unroll
+          for (int t = 0; t < NRT; ++t) vn[t] = cplx<T>{0, 0};
+          if (kk + 4 < KS) {
+            bzn = ld_z(kk + 4);
+#pragma unroll
+            for (int t = 0; t < NRT; ++t)
+              if (live(t, kk + 4)) vn[t] = ld_v(kk + 4, t);
+          }
+          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int t = 0; t < NRT; ++t) {
-            // reflectors 16 t .. 16 t + 15 meet rows 16 t .. 16 t + bw + 15;
             // conj(v) z with four real products (p1 = Re, p2 = Im): the
             // three-product form loses digits of Im to cancellation
-            if (kk < 16 * t || kk >= 16 * t + 16 + BW) continue;
-            const cplx<T> v = Vs[kr][16 * t + lane % 16];
-            p1[t] = mfma<T>::op(v.x, bz.x, p1[t]);
-            p1[t] = mfma<T>::op(v.y, bz.y, p1[t]);
-            p2[t] = mfma<T>::op(v.x, bz.y, p2[t]);
-            p2[t] = mfma<T>::op(-v.y, bz.x, p2[t]);
+            if (!live(t, kk)) continue;
+            p1[t] = mfma<T>::op(v[t].x, bz.x, p1[t]);
+            p1[t] = mfma<T>::op(v[t].y, bz.y, p1[t]);
+            p2[t] = mfma<T>::op(v[t].x, bz.y, p2[t]);
+            p2[t] = mfma<T>::op(-v[t].y, bz.x, p2[t]);
           }
+          if (kk < 4 * NMS) {
+#pragma unroll
+            for (int u = 0; u < MV; ++u) {
+              const int q = (kk / 4) * MV + u;
+              if (q >= ZE) continue;
+              q2_store(zst[q], zr, zoff(row0 - BW, q, k > 0));
+              zslot(s1, q) = pzn[q];
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          bz = bzn;
+#pragma unroll
+          for (int t = 0; t < NRT; ++t) v[t] = vn[t];
         }
 #pragma unroll
         for (int t = 0; t < NRT; ++t)
@@ -879,45 +961,66 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
             Ws[16 * t + mfma<T>::row(lane, r)][cw + lane % 16] = cplx<T>{p1[t][r], p2[t][r]};
       }
       wave_sync();
-      // Zw -= VT W on the wave's columns: the row tiles share the W operand
+      // Zw -= VT W on the wave's columns: the row tiles share the W operand;
+      // operands of the next step read ahead as above
       {
         using acc_t = typename mfma<T>::acc;
         constexpr int NRT = 2 * BW / 16;
-        acc_t p1[NRT], p2[NRT], p3[NRT];
+        // VT(rr, i) = 0 for i <= rr - bw
+        auto live = [](int t, int kk) { return kk + 3 >= 16 * t - BW + 1; };
+        auto ld_w = [&](int kk) { return Ws[kk + lane / 16][cw + lane % 16]; };
+        auto ld_a = [&](int kk, int t) { return VTs[kk + lane / 16][16 * t + lane % 16]; };
+        acc_t p1[NRT], p2[NRT];
 #pragma unroll
-        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; p3[t] = p1[t]; }
+        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; }
+        cplx<T> wv = ld_w(0), a[NRT];
+#pragma unroll
+        for (int t = 0; t < NRT; ++t) a[t] = live(t, 0) ? ld_a(0, t) : cplx<T>{0, 0};
 #pragma unroll
         for (int kk = 0; kk < BW; kk += 4) {
-          const int kr = kk + lane / 16;
-          const cplx<T> wv = Ws[kr][cw + lane % 16];
+          cplx<T> wn{0, 0}, an[NRT];
+#pragma unroll
+          for (int t = 0; t < NRT; ++t) an[t] = cplx<T>{0, 0};
+          if (kk + 4 < BW) {
+            wn = ld_w(kk + 4);
+#pragma unroll
+            for (int t = 0; t < NRT; ++t)
+              if (live(t, kk + 4)) an[t] = ld_a(kk + 4, t);
+          }
+          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int t = 0; t < NRT; ++t) {
-            // VT(rr, i) = 0 for i <= rr - bw
-            if (kk + 3 < 16 * t - BW + 1) continue;
-            const cplx<T> a = VTs[kr][16 * t + lane % 16];
-            p1[t] = mfma<T>::op(a.x, wv.x, p1[t]);
-            p1[t] = mfma<T>::op(-a.y, wv.y, p1[t]);
-            p2[t] = mfma<T>::op(a.x, wv.y, p2[t]);
-            p2[t] = mfma<T>::op(a.y, wv.x, p2[t]);
+            if (!live(t, kk)) continue;
+            p1[t] = mfma<T>::op(a[t].x, wv.x, p1[t]);
+            p1[t] = mfma<T>::op(-a[t].y, wv.y, p1[t]);
+            p2[t] = mfma<T>::op(a[t].x, wv.y, p2[t]);
+            p2[t] = mfma<T>::op(a[t].y, wv.x, p2[t]);
           }
+          __builtin_amdgcn_sched_barrier(0);
+          wv = wn;
+#pragma unroll
+          for (int t = 0; t < NRT; ++t) a[t] = an[t];
         }
 #pragma unroll
         for (int t = 0; t < NRT; ++t)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int rr = 16 * t + mfma<T>::row(lane, r);
-            cplx<T>& zz = Zw[(s0 + rr / BW) & 1][rr % BW][cw + lane % 16];
+            cplx<T>& zz = Zw[rr < BW ? s0 : s1][rr % BW][cw + lane % 16];
             zz = csub(zz, cplx<T>{p1[t][r], p2[t][r]});
           }
       }
       wave_sync();
-      // logical half 0 is finished for this group: store it; the half after
-      // the window takes its slot
-      store_z(row0, s0);
-      if (k + 1 < Kg) put_z(s0);
-      else store_z(row0 + BW, s0 ^ 1);
-      wave_sync();
     }
+    // the group's last window: both halves are finished
+    const int64_t rowL = j0 + 1 + (Kg - 1) * BW;
+    const int sL = (int)((Kg - 1) & 1);
+#pragma unroll
+    for (int q = 0; q < ZE; ++q) {
+      q2_store(zslot(sL, q), zr, zoff(rowL, q, true));
+      q2_store(zslot(sL ^ 1, q), zr, zoff(rowL + BW, q, true));
+    }
+    wave_sync();
   }
 }
 
@@ -940,9 +1043,18 @@ void apply_q2(const Q2Work<T>& w, hipStream_t s) {
   // columns per workgroup: 16 per wave, 8 waves for FP32 and 2 for FP64 (LDS)
   constexpr int NWV = sizeof(T) == 4 ? 8 : 2;
   const dim3 grid((unsigned)ceil_div(w.m, 16 * NWV), (unsigned)B);
-  if (w.bw == 32) hipLaunchKernelGGL((k_q2_apply<T, 32, NWV>), grid, dim3(WAVE * NWV), 0, s, w);
-  else if (w.bw == 16) hipLaunchKernelGGL((k_q2_apply<T, 16, NWV>), grid, dim3(WAVE * NWV), 0, s, w);
-  else throw std::runtime_error("apply_q2: band width must be 16 or 32");
+  if (w.bw != 32 && w.bw != 16) throw std::runtime_error("apply_q2: band width must be 16 or 32");
+  if (w.rowmajor) {
+    if (w.ldz < w.m || n * w.ldz * (int64_t)sizeof(cplx<T>) > 0x7FFFFFFF)
+      throw std::runtime_error("apply_q2: row-major Z needs m <= ldz and n ldz elements below 2^31 bytes");
+    if (w.bw == 32) hipLaunchKernelGGL((k_q2_apply<T, 32, NWV, true>), grid, dim3(WAVE * NWV), 0, s, w);
+    else hipLaunchKernelGGL((k_q2_apply<T, 16, NWV, true>), grid, dim3(WAVE * NWV), 0, s, w);
+  } else {
+    if ((16 * w.ldz) * (int64_t)sizeof(cplx<T>) > 0x7FFFFFFF)
+      throw std::runtime_error("apply_q2: 16 columns of Z must stay below 2^31 bytes");
+    if (w.bw == 32) hipLaunchKernelGGL((k_q2_apply<T, 32, NWV, false>), grid, dim3(WAVE * NWV), 0, s, w);
+    else hipLaunchKernelGGL((k_q2_apply<T, 16, NWV, false>), grid, dim3(WAVE * NWV), 0, s, w);
+  }
   PPH_CHECK_LAUNCH();
 }
 
