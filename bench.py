@@ -205,6 +205,49 @@ def cpu_baseline(st, fc, q, seconds_cap=60.0):
     return dt, r
 
 
+def scalapack_baseline(st, fc, q, ranks=16, nb=32, timeout=600):
+    """parphom.x's per-q work on the host's cores: the oracle's assembly of
+    D(q) and dD/dq (numpy, timed), then MKL ScaLAPACK pzheevx('V', 'A', 'U')
+    and the six velocity pzgemm calls with the reference's argument choices
+    (oracle/_bin/scalapack_baseline under MPICH, one rank per core, one
+    thread each, mb = nb = 32 as in the examples' inp.dat).  Returns None
+    when the baseline or mpiexec is not there or the run fails."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_bin", "scalapack_baseline")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    from oracle import phonon as O
+    S = O.System(st.lat, st.pos, st.types, st.masses, fc)
+    t0 = time.perf_counter()
+    mats = [S.D(q)] + [S.D(q, a) for a in (1, 2, 3)]
+    t_asm = time.perf_counter() - t0
+    n = mats[0].shape[0]
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            for M in mats:
+                f.write(np.asfortranarray(M, dtype=np.complex128).tobytes(order="F"))
+        del mats
+        env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
+        r = subprocess.run([mpiexec, "-n", str(ranks), exe, path, str(n), "1", str(nb)],
+                           capture_output=True, text=True, timeout=timeout, env=env)
+        line = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode != 0 or not line:
+            log(f"[bench] scalapack baseline failed (rc {r.returncode}): {r.stderr[-400:]}")
+            return None
+        out = json.loads(line[-1])
+    except (OSError, subprocess.SubprocessError, ValueError) as e:
+        log(f"[bench] scalapack baseline failed: {e}")
+        return None
+    finally:
+        os.unlink(path)
+    out["asm_s_per_q"] = t_asm
+    out["s_per_q"] = t_asm + out["eig_s_per_q"] + out["vel_s_per_q"]
+    return out
+
+
 def dos_workload(nbands=4764, mesh=18, seed=7):
     """Transport-DOS shaped workload (the reference's
     Utilities/Examples/transport_dos.py: 18 x 18 full grid, linear
@@ -569,13 +612,13 @@ def main():
                            "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
                            "frac": q2_flops / iso_q2 / FP32_PEAK,
                            "traffic": kernel_traffic("k_q2_apply", a.workload, n, B),
-                           "kernel": "k_q2_apply<float, 32, 8>", "flops_per_launch": q2_flops,
+                           "kernel": "k_q2_apply<float, 32, 8, true>", "flops_per_launch": q2_flops,
                            "avg_launch_us": iso_q2 * 1e6,
                            "note": "one batch alone on the GPU (HIP-event pair on the lane's stream)"}
         res["roofline_shared"] = {
             "bound": "mfma", "achieved": q2_flops / q2_s / 1e12, "peak": FP32_PEAK / 1e12,
             "unit": "TFLOP/s", "frac": q2_flops / q2_s / FP32_PEAK,
-            "kernel": "k_q2_apply<float, 32, 8>", "avg_launch_us": q2_s * 1e6,
+            "kernel": "k_q2_apply<float, 32, 8, true>", "avg_launch_us": q2_s * 1e6,
             "aggregate_tflops": S * q2_flops / q2_s / 1e12,
             "note": f"profiled pass over the timed steps' q-points, {S} batches in flight sharing "
                     "the GPU; per-launch rate (aggregate = S x this while the launches overlap)"}
@@ -679,10 +722,26 @@ def main():
 
     if not a.no_cpu:
         cdt, r = cpu_baseline(st, fc, qs[0])
-        res["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "q-pts/s",
-                               "cores": os.cpu_count(), "kind": "port",
-                               "sample": f"1 q-point of {a.workload} (assembly + zheevx all "
-                                         f"vectors + 3 velocity GEMMs, numpy/OpenBLAS)"}
+        numpy_baseline = {"value": 1.0 / cdt, "unit": "q-pts/s", "cores": os.cpu_count(),
+                          "kind": "port",
+                          "sample": f"1 q-point of {a.workload} (assembly + zheevx all "
+                                    f"vectors + 3 velocity GEMMs, numpy/OpenBLAS)"}
+        ranks = max(1, min(16, os.cpu_count() or 1))
+        sl = scalapack_baseline(st, fc, qs[0], ranks=ranks)
+        if sl is not None:
+            res["cpu_baseline"] = {
+                "value": 1.0 / sl["s_per_q"], "unit": "q-pts/s", "cores": sl["ranks"],
+                "kind": "port",
+                "sample": f"1 q-point of {a.workload}: numpy assembly of D and dD "
+                          f"({sl['asm_s_per_q']:.1f} s), MKL ScaLAPACK pzheevx('V','A','U') "
+                          f"({sl['eig_s_per_q']:.1f} s) and 6 pzgemm for the velocities "
+                          f"({sl['vel_s_per_q']:.1f} s), {sl['ranks']} MPI ranks x 1 thread "
+                          f"on a {sl['nprow']} x {sl['npcol']} grid, blocks of {sl['nb']}"}
+            res["cpu_baseline_numpy"] = numpy_baseline
+            lam0 = float(sl["lowest_lambda"][0])
+            res["scalapack_lowest_lambda_vs_oracle"] = abs(lam0 - float(r["lam"][0]))
+        else:
+            res["cpu_baseline"] = numpy_baseline
         res["max_abs_domega_vs_oracle_cm"] = float(np.abs(ev32[0] - r["eval_cm"]).max())
         # eigenvector residuals of the FP32 path against the oracle's FP64 D(q0):
         # |D z - lambda z| / |D|_2 for 256 sampled modes, lambda from the
