@@ -46,7 +46,7 @@ def device_outputs(torch, B, n):
 
 
 def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1,
-              tile="64x64"):
+              tile="64x64", prof_steps=3):
     """Each step is one pph_run_device call over streams x B q-points: the
     library keeps `streams` batches of B in flight (one stream and host thread
     each).  Returns a dict: the timed seconds, the phase profile of the timed
@@ -81,11 +81,11 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     g1 = h.graphs()
-    # the phase profile of the same steps, recorded in a second pass (the
-    # library's phase timers run every batch eagerly, the timed pass replays
-    # the captured batch graphs)
+    # the phase profile of the first prof_steps of the same steps, recorded
+    # in a second pass (the library's phase timers run every batch eagerly,
+    # the timed pass replays the captured batch graphs)
     h.profile(enable=True, reset=True)
-    for i in range(steps):
+    for i in range(min(steps, prof_steps)):
         h.run_device(step_q(warmup + i), out)
     torch.cuda.synchronize()
     prof = h.profile(enable=True, reset=True)
@@ -677,9 +677,12 @@ def main():
         # (at n = 4764 and three lanes, 24 per lane is the FP64 path's best:
         # 11.97 q-points/s against 11.09 at 16 x 2 and 11.65 at 32 x 2)
         B64 = max(1, (3 * B) // 4) if S > 1 else B
-        r64 = time_path(lib, torch, st, fc, qs, "D", B64, max(1, a.steps - 1), a.warmup, S,
+        # at most 6 timed steps (and 2 warm-up steps) of the FP64 path, so the
+        # default run stays within a few minutes whatever --steps is
+        steps64 = min(max(1, a.steps - 1), 6)
+        r64 = time_path(lib, torch, st, fc, qs, "D", B64, steps64, min(a.warmup, 2), S,
                         a.reduction, a.tile)
-        qps64 = max(1, a.steps - 1) * B64 * S / r64["dt"]
+        qps64 = steps64 * B64 * S / r64["dt"]
         k = min(B, B64)
         dw = float(np.abs(ev32[:k] - r64["ev"][:k]).max())
         ph64, _ = split_profile(r64["prof"], n)
@@ -708,27 +711,26 @@ def main():
         if not a.no_compare:
             # the FP32 path with the other reduction, same batches and lanes
             other = 0 if (two or small) else 1
-            ro = time_path(lib, torch, st, fc, qs, "S", B, a.steps, min(a.warmup, 1), S, other,
+            steps_o = min(a.steps, 4)
+            ro = time_path(lib, torch, st, fc, qs, "S", B, steps_o, min(a.warmup, 1), S, other,
                            a.tile)
             pho, _ = split_profile(ro["iso"], n)
             dvo = [velocity_difference(ro["vel"][i], r64["vel"][i], r64["ev"][i]) for i in range(k)]
             res["fp32_other_reduction"] = {
                 "reduction": ("two-stage (band 32)" if ro["two_stage"] else
                               "small-matrix kernel" if ro["small"] else "one-stage"),
-                "value": a.steps * B * S / ro["dt"], "unit": "q-pts/s",
+                "value": steps_o * B * S / ro["dt"], "unit": "q-pts/s",
                 "max_abs_domega_vs_fp64_cm": float(np.abs(ro["ev"][:k] - r64["ev"][:k]).max()),
                 "max_rel_dvel_vs_fp64": max(x[0] for x in dvo) / max(1e-300, max(x[1] for x in dvo)),
                 "phase_ms_one_batch_alone": {kk: round(v, 3) for kk, v in pho.items()}}
 
     if not a.no_cpu:
-        cdt, r = cpu_baseline(st, fc, qs[0])
-        numpy_baseline = {"value": 1.0 / cdt, "unit": "q-pts/s", "cores": os.cpu_count(),
-                          "kind": "port",
-                          "sample": f"1 q-point of {a.workload} (assembly + zheevx all "
-                                    f"vectors + 3 velocity GEMMs, numpy/OpenBLAS)"}
         ranks = max(1, min(16, os.cpu_count() or 1))
         sl = scalapack_baseline(st, fc, qs[0], ranks=ranks)
         if sl is not None:
+            # the oracle's eigenvalues for the accuracy checks below (values only)
+            from oracle import phonon as O
+            r = O.System(st.lat, st.pos, st.types, st.masses, fc).solve(qs[0], jobz="N")
             res["cpu_baseline"] = {
                 "value": 1.0 / sl["s_per_q"], "unit": "q-pts/s", "cores": sl["ranks"],
                 "kind": "port",
@@ -737,11 +739,15 @@ def main():
                           f"({sl['eig_s_per_q']:.1f} s) and 6 pzgemm for the velocities "
                           f"({sl['vel_s_per_q']:.1f} s), {sl['ranks']} MPI ranks x 1 thread "
                           f"on a {sl['nprow']} x {sl['npcol']} grid, blocks of {sl['nb']}"}
-            res["cpu_baseline_numpy"] = numpy_baseline
             lam0 = float(sl["lowest_lambda"][0])
             res["scalapack_lowest_lambda_vs_oracle"] = abs(lam0 - float(r["lam"][0]))
         else:
-            res["cpu_baseline"] = numpy_baseline
+            # no MPI / ScaLAPACK: the numpy/LAPACK oracle is the baseline
+            cdt, r = cpu_baseline(st, fc, qs[0])
+            res["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "q-pts/s", "cores": os.cpu_count(),
+                                   "kind": "port",
+                                   "sample": f"1 q-point of {a.workload} (assembly + zheevx all "
+                                             f"vectors + 3 velocity GEMMs, numpy/OpenBLAS)"}
         res["max_abs_domega_vs_oracle_cm"] = float(np.abs(ev32[0] - r["eval_cm"]).max())
         # eigenvector residuals of the FP32 path against the oracle's FP64 D(q0):
         # |D z - lambda z| / |D|_2 for 256 sampled modes, lambda from the
