@@ -78,6 +78,15 @@ int pph_debug_graphs(void* h, int32_t mode, int64_t* captures, int64_t* replays)
  * their own tiles.  No counterpart in the reference (PBLAS chooses its own
  * blocking inside pzheevx and pzgemm). */
 int pph_debug_gemm_tile(void* h, int32_t tile);
+/* The FP32 path's Rayleigh-Ritz step over clusters of close eigenvalues
+ * (range A): 1 on (the default), 0 off.  Runs of consecutive modes whose FP64
+ * Rayleigh quotients lie within 1e-5 max|lambda| (at most 8 modes a cluster,
+ * 256 mode pairs a matrix) get the FP64 Ritz values, vectors and velocity
+ * forms of their FP32 vectors' span.  With h == NULL the setting becomes the
+ * process default (handles created later, the temporary handles of
+ * pph_heevx).  No counterpart in the reference
+ * (pzheevx computes in FP64 throughout, where such pairs do not mix). */
+int pph_debug_cluster_rr(void* h, int32_t mode);
 
 #ifdef __cplusplus
 }

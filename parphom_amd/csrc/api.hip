@@ -38,6 +38,8 @@ constexpr int64_t kTwoStageMinN = 512, kTwoStageMaxN = 16384;
 // graph holds the addresses of its buffers and is replayed only while this
 // count is the one it was captured at.
 static std::atomic<uint64_t> g_alloc_gen{0};
+// process default of Handle::cluster_rr (pph_debug_cluster_rr with no handle)
+static std::atomic<int> g_cluster_rr_default{1};
 
 struct DevBuf {
   void* p = nullptr;
@@ -184,6 +186,7 @@ struct Lane {
   DevBuf tdd, tee, te2, tilo, tihi, tinfo, tw, tzt, tfac, tinf;
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
+  DevBuf cmeta, cvec, cform;  // cluster Rayleigh-Ritz: cluster lists, combination vectors, their forms
   DevBuf gws, sws;         // partials of the GEMM dot epilogue / split-K and of the sparse forms
   DevBuf meta;             // per-matrix offsets and counts of the batch's outputs
   // two-stage reduction: band, bulge-chasing reflectors and their block
@@ -237,6 +240,8 @@ struct Handle : Lane {
   int reduction_code() const;
   int band_width() const;
   bool graphs = true;      // replay captured batch graphs (range A)
+  // FP32 path: Rayleigh-Ritz over clusters of close eigenvalues (range A)
+  bool cluster_rr = g_cluster_rr_default.load() != 0;
   int gemm_tile = 0;       // GEMM tile shape of the eigensolver (kGemmTile*; 0: process default)
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
@@ -971,6 +976,50 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
                               mcap, s);
       launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
+      if (all_modes && h.cluster_rr && mmax > 1) {
+        // Rayleigh-Ritz in FP64 over each cluster of close quotients, whose
+        // FP32 vectors mix inside the cluster; the forms of the pairs'
+        // combination vectors give the off-diagonal entries
+        constexpr int64_t P = kClusterSlots;
+        L.cmeta.ensure((size_t)B * kClusterMeta * sizeof(int32_t));
+        L.cvec.ensure((size_t)B * n * P * sizeof(cplx<double>));
+        L.cform.ensure((size_t)B * 4 * P * sizeof(double));
+        int32_t* meta = L.cmeta.as<int32_t>();
+        cplx<double>* cv = L.cvec.as<cplx<double>>();
+        double* numc = L.cform.as<double>();
+        double* vc = numc + B * P;
+        const bool v = forms && h.opts.vel != 0;
+        launch_cluster_find(te.w, mmax, mcap, meta, B, s);
+        launch_fill_zero(numc, B * 4 * P * (int64_t)sizeof(double), s);
+        if (forms) {
+          launch_cluster_gather(meta, L.Zt.as<cplx<float>>(), mcap, 1, n * mcap, cv, P, 1, n, B, s);
+          L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, P, B, v ? 4 : 1));
+          launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                                     h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1, cv,
+                                     P, n * P, h.N, P, B, numc, P, v ? vc : nullptr, P, 3 * P,
+                                     L.sws.as<double>(), s);
+        } else {
+          launch_cluster_gather(meta, L.Z.as<cplx<float>>(), 1, n, n * mcap, cv, 1, n, n, B, s);
+          GemmParams<double> g{};
+          g.opA = 'N'; g.opB = 'N';
+          g.m = n; g.n = P; g.k = n;
+          g.alpha = cplx<double>{1, 0};
+          g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
+          g.B = cv; g.ldb = n; g.strideB = n * P;
+          g.batch = B;
+          g.dot_out = numc; g.dot_stride = P;
+          g.herm_dot = 1;
+          g.dotX = cv; g.lddx = n; g.strideX = n * P;
+          L.gws.ensure((size_t)gemm_ws_bytes(g));
+          g.ws = L.gws.p;
+          launch_gemm<double>(g, s);
+        }
+        // the dense path's velocity products read the FP32 vectors: they
+        // get the Ritz vectors too
+        launch_cluster_rr(meta, L.Z64.as<cplx<double>>(), forms ? nullptr : L.Z.as<cplx<float>>(), n,
+                          mcap, te.w, L.den.as<double>(), L.num.as<double>(), numc,
+                          v ? L.vacc.as<double>() : nullptr, vc, B, s);
+      }
       if (all_modes)
         refine_low_modes_dev(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), mcap, s);
       else
@@ -1134,7 +1183,7 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   L.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
   const bool eligible = h.graphs && !L.graph_failed && h.opts.range == 'A' && !L.tm.on;
-  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, h.use_sparse() ? 1 : 0,
+  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, (h.use_sparse() ? 1 : 0) | (h.cluster_rr ? 2 : 0),
                                    h.reduction_code(), h.band_width(), h.n,
                                    h.gemm_tile ? h.gemm_tile : gemm_current_tile()};
   if (eligible && L.graph.exec && L.graph.key == key && L.graph.gen == g_alloc_gen.load() &&
@@ -2033,6 +2082,17 @@ extern "C" int pph_debug_gemm_tile(void* hh, int32_t tile) {
   if (tile == kGemmTile128x128 && h->opts.precision == 'D')
     return fail(h, "pph_debug_gemm_tile: the 128x128 tile is FP32 only", PPH_ERR_ARG);
   h->gemm_tile = tile;
+  return PPH_OK;
+}
+
+extern "C" int pph_debug_cluster_rr(void* hh, int32_t mode) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (mode < 0 || mode > 1) return fail(h, "pph_debug_cluster_rr: mode must be 0 or 1", PPH_ERR_ARG);
+  if (!h) {
+    g_cluster_rr_default.store(mode);
+    return PPH_OK;
+  }
+  h->cluster_rr = mode == 1;
   return PPH_OK;
 }
 

@@ -59,6 +59,38 @@ void launch_sparse_dots(const int64_t* rowptr, const int32_t* bcol, int64_t nblk
 // X(:, 0:16) with leading dimension mcap and matrix stride 16 mcap.
 void launch_low_rr(double* lam, const double* nrm, const cplx<double>* C1, const cplx<double>* C2,
                    int64_t M, int64_t mcap, double lam_low, int64_t B, hipStream_t s);
+// Rayleigh-Ritz over clusters of close eigenvalues of the FP32 path
+// (refine.hip).  Clusters: runs of consecutive modes whose FP64 Rayleigh
+// quotients differ by at most kClusterTol max|lambda|, at most kClusterMax
+// modes each; a matrix holds at most kClusterSlots combination vectors
+// (2 per pair of modes of a cluster), which bounds its clusters.
+constexpr int kClusterMax = 8;
+constexpr int kClusterSlots = 512;
+constexpr int kClusterCount = kClusterSlots / 2;
+constexpr double kClusterTol = 1e-5;
+// per-matrix cluster list: [0] clusters, [1] slots used, then per cluster
+// (first mode, size, first slot), then per slot (mode i, 2 mode j + kind)
+constexpr int64_t kClusterMeta = 2 + 3 * kClusterCount + 2 * kClusterSlots;
+// meta (B x kClusterMeta) from lam (B x mcap, modes 0..M-1)
+void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int32_t* meta, int64_t B,
+                         hipStream_t s);
+// the combination vectors x_i + x_j (kind 0) and x_i + i x_j (kind 1) of every
+// slot in FP64, zero for unused slots: vector p of matrix b, row r at
+// dst[b * n * kClusterSlots + r * drs + p * dps]; the modes from the FP32
+// vectors, element (r, k) of matrix b at src[b * sstride + r * srs + k * sks]
+void launch_cluster_gather(const int32_t* meta, const cplx<float>* src, int64_t srs, int64_t sks,
+                           int64_t sstride, cplx<double>* dst, int64_t drs, int64_t dps, int64_t n,
+                           int64_t B, hipStream_t s);
+// The k x k problems of the clusters: H and the velocity forms from the
+// diagonal forms (num, vacc: raw forms (b * 3 + a) * mcap + k) and the slot
+// forms (numc: b * kClusterSlots + p; vc: (b * 3 + a) * kClusterSlots + p),
+// G = X^H X from Z64.  Writes the Ritz values to lam, 1 to den, the Ritz
+// forms to vacc (when given), the Ritz vectors to Z64 (n x mcap per matrix,
+// ld n) and, when given, to the FP32 copy Z32 (same layout).
+void launch_cluster_rr(const int32_t* meta, cplx<double>* Z64, cplx<float>* Z32, int64_t n,
+                       int64_t mcap, double* lam, double* den, const double* num,
+                       const double* numc, double* vacc, const double* vc, int64_t B,
+                       hipStream_t s);
 // Spring-model force constants (spring.hip): pairs (pi, pj) with separation
 // rvec (3 per pair) and kind 0..2 (intralayer shell: radial p[kind],
 // tangential p[3 + kind]) or 3 (interlayer: p[6] exp(-(|r| - inter_r0) /
