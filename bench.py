@@ -46,7 +46,7 @@ def device_outputs(torch, B, n):
 
 
 def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1,
-              tile="64x64", prof_steps=3):
+              tile="64x64", prof_steps=3, vec_refine=-1):
     """Each step is one pph_run_device call over streams x B q-points: the
     library keeps `streams` batches of B in flight (one stream and host thread
     each).  Returns a dict: the timed seconds, the phase profile of the timed
@@ -61,6 +61,8 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
     h.reduction(reduction)           # -1: the library's choice
     if not (prec == "D" and tile == "128x128"):     # the 128 x 128 tile is FP32 only
         h.gemm_tile(tile)
+    if vec_refine >= 0:              # FP32 path: FP64 correction steps of the vectors
+        h.vector_refinement(vec_refine)
     per_step = B * streams
     out = device_outputs(torch, per_step, n)
     nq = len(qs)
@@ -477,6 +479,9 @@ def main():
                          "-1 the library's choice")
     ap.add_argument("--tile", default="64x64", choices=("64x64", "128x64", "64x128", "128x128"),
                     help="GEMM tile shape of the eigensolver (the autotuner's tile dimension)")
+    ap.add_argument("--vec-refine", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="FP64 residual-correction steps of the FP32 path's vectors "
+                         "(-1: the library default, 2)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -523,7 +528,8 @@ def main():
         f"(inputs built in {time.perf_counter() - t0:.1f}s)")
 
     B, S = a.batch, a.streams
-    r32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S, a.reduction, a.tile)
+    r32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S, a.reduction, a.tile,
+                    vec_refine=a.vec_refine)
     dt32, prof32, iso32, ev32, z32, nblk = (r32["dt"], r32["prof"], r32["iso"], r32["ev"],
                                             r32["z0"], r32["nblk"])
     two, small = r32["two_stage"], r32["small"]
@@ -540,6 +546,7 @@ def main():
                       "reduction": ("two-stage (band 32)" if two else
                                     "small-matrix kernel" if small else "one-stage"),
                       "gemm_tile": a.tile,
+                      "vector_correction_steps": a.vec_refine if a.vec_refine >= 0 else 2,
                       "batch_graphs_replayed_in_timed_steps": r32["graph_replays"]}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}
@@ -565,16 +572,23 @@ def main():
     # flops) and the dot with the row's 3-vector (24), then per operator
     # Re(p c) weighted and accumulated (4).  On the FP32 path one launch forms
     # the Rayleigh numerators and the three velocity forms (76 flops per block
-    # and mode, in FP64 on the vector units); its phase also holds the
-    # transpose of the vectors to the mode-contiguous layout.
+    # and mode, in FP64 on the vector units): of the FP32 vectors, or, after
+    # the FP64 correction of the vectors (the default), of the corrected FP64
+    # vectors, in chunks of 8 matrices; its phase also holds the transposes
+    # of the vectors to the mode-contiguous layout.
     vel_s = phases["velocity"] / 1e3
-    if nblk > 0:
+    vsteps = a.vec_refine if a.vec_refine >= 0 else 2
+    if vel_s <= 0:
+        vel_rl = None
+    elif nblk > 0:
         vel_flops = B * n * 76.0 * nblk
         vel_rl = {"bound": "fp64 vector", "achieved": vel_flops / vel_s / 1e12,
                   "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                   "frac": vel_flops / vel_s / FP64_PEAK, "traffic": None,
-                  "kernel": "k_sparse_dots<float, 4> (Rayleigh numerators and velocity forms "
-                            "on %d nonzero force-constant blocks) + k_transpose" % nblk}
+                  "kernel": ("k_sparse_dots<double, 4> (Rayleigh numerators and velocity forms "
+                             "of the corrected vectors" if vsteps > 0 else
+                             "k_sparse_dots<float, 4> (Rayleigh numerators and velocity forms")
+                            + " on %d nonzero force-constant blocks) + k_transpose" % nblk}
     else:
         vel_flops = B * 12.0 * n ** 3
         vel_rl = {"bound": "mfma", "achieved": vel_flops / vel_s / 1e12,
@@ -700,6 +714,12 @@ def main():
         # relative to the largest velocity over the compared q-points (at
         # Gamma every velocity is zero)
         res["max_rel_dvel_vs_fp64"] = max(x[0] for x in dv) / max(1e-300, max(x[1] for x in dv))
+        # per mode: only modes closer than 1e-9 max|omega^2| (degenerate to
+        # FP64 precision) are summed
+        dvm = [velocity_difference(r32["vel"][i], r64["vel"][i], r64["ev"][i], rel=1e-9)
+               for i in range(k)]
+        res["max_rel_dvel_vs_fp64_per_mode"] = (max(x[0] for x in dvm) /
+                                               max(1e-300, max(x[1] for x in dvm)))
         # whole-path roofline of both paths, against the spec-sheet matrix peaks
         # and against what the library GEMM reaches on this device
         p32, p64 = gemm_peak(torch, torch.float32), gemm_peak(torch, torch.float64)
@@ -716,6 +736,17 @@ def main():
                            a.tile)
             pho, _ = split_profile(ro["iso"], n)
             dvo = [velocity_difference(ro["vel"][i], r64["vel"][i], r64["ev"][i]) for i in range(k)]
+            # the FP32 path without the FP64 correction of its vectors
+            steps_v = min(a.steps, 4)
+            rv = time_path(lib, torch, st, fc, qs, "S", B, steps_v, min(a.warmup, 1), S,
+                           a.reduction, a.tile, vec_refine=0)
+            dvv = [velocity_difference(rv["vel"][i], r64["vel"][i], r64["ev"][i], rel=1e-9)
+                   for i in range(k)]
+            res["fp32_without_vector_correction"] = {
+                "value": steps_v * B * S / rv["dt"], "unit": "q-pts/s",
+                "max_abs_domega_vs_fp64_cm": float(np.abs(rv["ev"][:k] - r64["ev"][:k]).max()),
+                "max_rel_dvel_vs_fp64_per_mode": (max(x[0] for x in dvv) /
+                                                  max(1e-300, max(x[1] for x in dvv)))}
             res["fp32_other_reduction"] = {
                 "reduction": ("two-stage (band 32)" if ro["two_stage"] else
                               "small-matrix kernel" if ro["small"] else "one-stage"),

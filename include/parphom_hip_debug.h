@@ -87,6 +87,16 @@ int pph_debug_gemm_tile(void* h, int32_t tile);
  * pph_heevx).  No counterpart in the reference
  * (pzheevx computes in FP64 throughout, where such pairs do not mix). */
 int pph_debug_cluster_rr(void* h, int32_t mode);
+/* FP64 residual-correction steps of the FP32 path's eigenvectors (range A,
+ * with eigenvectors): 0, 1 or 2 (the default).  A step computes the
+ * residual R = D X - X Theta in FP64 (block-sparse or dense D), M = X^H R and
+ * X <- X + X E with E_ij = M_ij / (theta_j - theta_i) for pairs farther apart
+ * than the cluster tolerance and E_ii = (1 - |x_i|^2) / 2, the two products
+ * on the FP32 matrix cores; the group velocities then come from the
+ * corrected vectors.  Each step squares the vectors' error.  With h == NULL
+ * the setting becomes the process default.  No counterpart in the reference
+ * (pzheevx returns FP64 vectors). */
+int pph_debug_vector_refinement(void* h, int32_t steps);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,8 @@ constexpr int64_t kTwoStageMinN = 512, kTwoStageMaxN = 16384;
 static std::atomic<uint64_t> g_alloc_gen{0};
 // process default of Handle::cluster_rr (pph_debug_cluster_rr with no handle)
 static std::atomic<int> g_cluster_rr_default{1};
+// process default of Handle::vec_refine (pph_debug_vector_refinement with no handle)
+static std::atomic<int> g_vec_refine_default{2};
 
 struct DevBuf {
   void* p = nullptr;
@@ -187,6 +189,7 @@ struct Lane {
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
   DevBuf cmeta, cvec, cform;  // cluster Rayleigh-Ritz: cluster lists, combination vectors, their forms
+  DevBuf vxt, vtol;        // vector correction: FP64 mode-contiguous vectors of a chunk, pair tolerances
   DevBuf gws, sws;         // partials of the GEMM dot epilogue / split-K and of the sparse forms
   DevBuf meta;             // per-matrix offsets and counts of the batch's outputs
   // two-stage reduction: band, bulge-chasing reflectors and their block
@@ -242,6 +245,8 @@ struct Handle : Lane {
   bool graphs = true;      // replay captured batch graphs (range A)
   // FP32 path: Rayleigh-Ritz over clusters of close eigenvalues (range A)
   bool cluster_rr = g_cluster_rr_default.load() != 0;
+  // FP32 path with eigenvectors: FP64 residual correction steps of the vectors (range A)
+  int vec_refine = g_vec_refine_default.load();
   int gemm_tile = 0;       // GEMM tile shape of the eigensolver (kGemmTile*; 0: process default)
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
@@ -717,6 +722,205 @@ static void complement_correction(Handle& h, Lane& L, int64_t B, const cplx<doub
   for (auto& c : corr) c = -c;
 }
 
+// FP64 residual correction of the FP32 path's vectors (vecref.hip), range A:
+// h.vec_refine steps of X <- X + X E with E from M = X^H (D X - X Theta)
+// (the first with full products, later ones over a band of close modes),
+// on the FP64 vectors Z64 and their FP32 copy L.Z (both updated), values
+// theta (the refined quotients, kept) and |x|^2 in L.den (refreshed).  R32
+// and Em are scratch of B n mcap complex64 each, the caller's dead buffers
+// (the reduced matrices and the tridiagonal vectors).  The residuals and the
+// velocity forms run over chunks of kVecRefChunk matrices (one FP64
+// mode-contiguous copy of a chunk); the products over the whole batch.  With
+// velocities and block-sparse forms, vacc gets the forms of the corrected
+// vectors.
+constexpr int64_t kVecRefChunk = 8;
+
+// Rayleigh-Ritz over the clusters of L.cmeta (refine.hip): the forms of the
+// pairs' combination vectors (block-sparse, or the GEMM's Hermitian-dot
+// epilogue with D64) give the off-diagonal entries of X^H D X and, with v,
+// of X^H dD X; the diagonal ones are L.num and L.vacc.  The vectors come from
+// src (element (r, k) of matrix b at b n mcap + r srs + k sks): the FP32 ones
+// or the corrected FP64 ones.  Writes the Ritz values to theta, the Ritz
+// vectors to Z64 and L.Z, the Ritz velocity forms to L.vacc (with v).
+// The combination vectors go to `scratch` (a dead buffer of scratch_bytes)
+// when it is large enough, else to L.cvec.
+template <typename Ts>
+static void cluster_rayleigh_ritz(Handle& h, Lane& L, int64_t B, const cplx<double>* D64,
+                                  const cplx<Ts>* src, int64_t srs, int64_t sks, bool v,
+                                  double* theta, int64_t mcap, void* scratch,
+                                  size_t scratch_bytes, hipStream_t s) {
+  const int64_t n = h.n;
+  const bool forms = h.use_sparse();
+  constexpr int64_t P = kClusterSlots;
+  const size_t cv_bytes = (size_t)B * n * P * sizeof(cplx<double>);
+  if (scratch_bytes < cv_bytes) L.cvec.ensure(cv_bytes);
+  L.cform.ensure((size_t)B * 4 * P * sizeof(double));
+  const int32_t* meta = L.cmeta.as<int32_t>();
+  cplx<double>* cv = scratch_bytes >= cv_bytes ? static_cast<cplx<double>*>(scratch)
+                                               : L.cvec.as<cplx<double>>();
+  double* numc = L.cform.as<double>();
+  double* vc = numc + B * P;
+  v = v && forms;
+  launch_fill_zero(numc, B * 4 * P * (int64_t)sizeof(double), s);
+  if (forms) {
+    launch_cluster_gather<Ts>(meta, src, srs, sks, n * mcap, cv, P, 1, n, B, s);
+    L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, P, B, v ? 4 : 1));
+    launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                               h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1, cv, P,
+                               n * P, h.N, P, B, numc, P, v ? vc : nullptr, P, 3 * P,
+                               L.sws.as<double>(), s);
+  } else {
+    launch_cluster_gather<Ts>(meta, src, srs, sks, n * mcap, cv, 1, n, n, B, s);
+    GemmParams<double> g{};
+    g.opA = 'N'; g.opB = 'N';
+    g.m = n; g.n = P; g.k = n;
+    g.alpha = cplx<double>{1, 0};
+    g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
+    g.B = cv; g.ldb = n; g.strideB = n * P;
+    g.batch = B;
+    g.dot_out = numc; g.dot_stride = P;
+    g.herm_dot = 1;
+    g.dotX = cv; g.lddx = n; g.strideX = n * P;
+    L.gws.ensure((size_t)gemm_ws_bytes(g));
+    g.ws = L.gws.p;
+    launch_gemm<double>(g, s);
+  }
+  launch_cluster_rr(meta, L.Z64.as<cplx<double>>(), L.Z.as<cplx<float>>(), n, mcap, theta,
+                    L.den.as<double>(), L.num.as<double>(), numc, v ? L.vacc.as<double>() : nullptr,
+                    vc, B, s);
+}
+
+static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D64,
+                            cplx<float>* R32, cplx<float>* Em, double* theta, int64_t mmax,
+                            int64_t mcap, bool want_num, hipStream_t s) {
+  const int64_t n = h.n;
+  const bool forms = h.use_sparse();
+  if (!forms && !D64) throw std::runtime_error("correct_vectors: no FP64 matrices");
+  const int64_t Bc = std::min<int64_t>(B, kVecRefChunk);
+  // the FP64 mode-contiguous copy of a chunk: in L.Zt (the FP32
+  // mode-contiguous vectors, read only before this step) when it is large
+  // enough, else in a buffer of its own
+  const size_t xt_bytes = (size_t)Bc * n * mcap * sizeof(cplx<double>);
+  if (L.Zt.bytes < xt_bytes) L.vxt.ensure(xt_bytes);
+  L.vtol.ensure((size_t)B * sizeof(double));
+  cplx<double>* X64 = L.Z64.as<cplx<double>>();
+  cplx<float>* X32 = L.Z.as<cplx<float>>();
+  cplx<double>* Xt = L.Zt.bytes >= xt_bytes ? L.Zt.as<cplx<double>>() : L.vxt.as<cplx<double>>();
+  double* den = L.den.as<double>();
+  const int64_t sz = n * mcap;              // per-matrix stride of the n x mcap arrays
+  launch_lam_tol(theta, mmax, mcap, kClusterTol, L.vtol.as<double>(), B, s);
+  for (int step = 0; step < h.vec_refine; ++step) {
+    // R = D X - X Theta in FP64, rounded to FP32
+    for (int64_t b0 = 0; b0 < B; b0 += Bc) {
+      const int64_t bc = std::min(Bc, B - b0);
+      if (forms) {
+        launch_transpose<double>(X64 + b0 * sz, n, mmax, n, sz, Xt, mcap, sz, bc, s);
+        launch_sparse_residual(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(),
+                               h.sp_tptr.as<int64_t>(), h.sp_trow.as<int32_t>(),
+                               h.sp_tblk.as<int64_t>(), h.sp_nblk, h.sp_phit.as<double>(),
+                               L.bv.as<cplx<double>>() + b0 * h.sp_nblk * 4, Xt, mcap, sz,
+                               theta + b0 * mcap, mcap, h.N, mmax, R32 + b0 * sz, mcap, sz, bc, s);
+      } else {
+        GemmParams<double> g{};
+        g.opA = 'N'; g.opB = 'N';
+        g.m = n; g.n = mmax; g.k = n;
+        g.alpha = cplx<double>{1, 0}; g.beta = cplx<double>{0, 0};
+        g.A = D64 + b0 * h.ldA * n; g.lda = h.ldA; g.strideA = h.ldA * n;
+        g.B = X64 + b0 * sz; g.ldb = n; g.strideB = sz;
+        g.C = Xt; g.ldc = n; g.strideC = sz;
+        g.batch = bc;
+        // an FP64 product: the 128 x 128 tile of the FP32 ones does not apply
+        if (gemm_current_tile() == kGemmTile128x128) g.tile = kGemmTile64x64;
+        launch_gemm<double>(g, s);
+        launch_dense_residual(Xt, X64 + b0 * sz, theta + b0 * mcap, mcap, n, mmax, sz, R32 + b0 * sz,
+                              bc, s);
+      }
+    }
+    // after the first step only pairs of close modes carry a noticeable
+    // error: a band of kVecRefBand modes on either side of the diagonal
+    if (step > 0 && mmax > 4 * kVecRefBand) {
+      const int64_t smb = mcap * (2 * kVecRefBand + 1);
+      if (forms)
+        launch_band_gram(X32, n, sz, R32, mcap, 1, sz, true, n, mmax, Em, smb, B, s);
+      else
+        launch_band_gram(X32, n, sz, R32, 1, n, sz, false, n, mmax, Em, smb, B, s);
+      launch_band_coeffs(Em, smb, mmax, theta, den, mcap, L.vtol.as<double>(), B, s);
+      launch_band_apply(X32, n, sz, Em, smb, n, mmax, R32, sz, B, s);
+      launch_vecref_update(X64, X32, R32, n * mmax, sz, B, s);
+      launch_colnorm2<double>(X64, n, mmax, n, sz, B, den, mcap, s);
+      continue;
+    }
+    // M = X^H R on the matrix cores (R stored as R^H, mode-contiguous, on
+    // the block-sparse route)
+    {
+      GemmParams<float> g{};
+      g.opA = 'C'; g.opB = forms ? 'C' : 'N';
+      g.m = mmax; g.n = mmax; g.k = n;
+      g.alpha = cplx<float>{1, 0}; g.beta = cplx<float>{0, 0};
+      g.A = X32; g.lda = n; g.strideA = sz;
+      g.B = R32; g.ldb = forms ? mcap : n; g.strideB = sz;
+      g.C = Em; g.ldc = mcap; g.strideC = mcap * mcap;
+      g.batch = B;
+      launch_gemm<float>(g, s);
+    }
+    launch_vecref_coeffs(Em, mcap, mcap * mcap, mmax, theta, den, mcap, L.vtol.as<double>(), B, s);
+    // W = X E (into R32), then X += W and the FP32 copy
+    {
+      GemmParams<float> g{};
+      g.opA = 'N'; g.opB = 'N';
+      g.m = n; g.n = mmax; g.k = mmax;
+      g.alpha = cplx<float>{1, 0}; g.beta = cplx<float>{0, 0};
+      g.A = X32; g.lda = n; g.strideA = sz;
+      g.B = Em; g.ldb = mcap; g.strideB = mcap * mcap;
+      g.C = R32; g.ldc = n; g.strideC = sz;
+      g.batch = B;
+      launch_gemm<float>(g, s);
+    }
+    launch_vecref_update(X64, X32, R32, n * mmax, sz, B, s);
+    launch_colnorm2<double>(X64, n, mmax, n, sz, B, den, mcap, s);
+  }
+  // the forms of the corrected vectors: with velocities and block-sparse
+  // forms, the velocity forms; with want_num, the Rayleigh numerators (the
+  // diagonal of X^H D X for a cluster step that follows)
+  const bool v = forms && h.opts.vel != 0;
+  if (want_num) launch_fill_zero(L.num.p, B * mcap * (int64_t)sizeof(double), s);
+  if (forms && (v || want_num)) {
+    // timed as the velocity phase (inside the refinement phase)
+    if (v) L.tm.begin(PH_VEL, s);
+    const int op0 = want_num ? 0 : 1, nops = (want_num ? 1 : 0) + (v ? 3 : 0);
+    if (v) {
+      L.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
+      launch_fill_zero(L.vacc.p, B * mcap * 3 * (int64_t)sizeof(double), s);
+    }
+    L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, mmax, Bc, nops));
+    for (int64_t b0 = 0; b0 < B; b0 += Bc) {
+      const int64_t bc = std::min(Bc, B - b0);
+      launch_transpose<double>(X64 + b0 * sz, n, mmax, n, sz, Xt, mcap, sz, bc, s);
+      launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
+                                 h.sp_phit.as<double>(), L.bv.as<cplx<double>>() + b0 * h.sp_nblk * 4,
+                                 op0, nops, Xt, mcap, sz, h.N, mmax, bc,
+                                 want_num ? L.num.as<double>() + b0 * mcap : nullptr, mcap,
+                                 v ? L.vacc.as<double>() + b0 * 3 * mcap : nullptr, mcap, 3 * mcap,
+                                 L.sws.as<double>(), s);
+    }
+    if (v) L.tm.end(PH_VEL, s);
+  } else if (want_num) {
+    GemmParams<double> g{};
+    g.opA = 'N'; g.opB = 'N';
+    g.m = n; g.n = mmax; g.k = n;
+    g.alpha = cplx<double>{1, 0};
+    g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
+    g.B = X64; g.ldb = n; g.strideB = sz;
+    g.batch = B;
+    g.dot_out = L.num.as<double>(); g.dot_stride = mcap;
+    g.herm_dot = 1;
+    g.dotX = X64; g.lddx = n; g.strideX = sz;
+    L.gws.ensure((size_t)gemm_ws_bytes(g));
+    g.ws = L.gws.p;
+    launch_gemm<double>(g, s);
+  }
+}
+
 // A: B matrices (n x n, working precision, destroyed); D64: FP64 matrices for
 // the refinement (FP32 path only).  want_vec: caller needs eigenvectors.
 template <typename T>
@@ -933,13 +1137,16 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       L.den.ensure((size_t)B * mcap * sizeof(double));
       launch_fill_zero(L.num.p, B * mcap * sizeof(double), s);
       const bool forms = h.use_sparse();
+      // the FP64 residual correction of the vectors (range A) forms the
+      // velocity forms of the corrected vectors itself
+      const bool vref = all_modes && h.vec_refine > 0 && mmax > 1;
       if (forms) {
         // Rayleigh numerators from the nonzero blocks of D, with the FP32
         // vectors in a mode-contiguous copy (exact in FP64); with velocities
-        // requested, the three velocity forms come from the same pass over
-        // the vectors (one launch, timed as the velocity phase; the
-        // velocity step then only scales them)
-        const bool v = h.opts.vel != 0;
+        // requested (and no vector correction to follow), the three velocity
+        // forms come from the same pass over the vectors (one launch, timed
+        // as the velocity phase; the velocity step then only scales them)
+        const bool v = h.opts.vel != 0 && !vref;
         if (v) {
           L.vacc.ensure((size_t)B * mcap * 6 * sizeof(double));
           launch_fill_zero(L.vacc.p, B * mcap * 3 * sizeof(double), s);
@@ -976,49 +1183,32 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       launch_colnorm2<double>(L.Z64.as<cplx<double>>(), n, mmax, n, n * mcap, B, L.den.as<double>(),
                               mcap, s);
       launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
-      if (all_modes && h.cluster_rr && mmax > 1) {
-        // Rayleigh-Ritz in FP64 over each cluster of close quotients, whose
-        // FP32 vectors mix inside the cluster; the forms of the pairs'
-        // combination vectors give the off-diagonal entries
-        constexpr int64_t P = kClusterSlots;
+      // Rayleigh-Ritz in FP64 over each cluster of close quotients, whose
+      // FP32 vectors mix inside the cluster: on the FP32 vectors, or, when
+      // the vectors are corrected, on the corrected ones (an accurate span)
+      const bool crr = all_modes && h.cluster_rr && mmax > 1;
+      if (crr) {
         L.cmeta.ensure((size_t)B * kClusterMeta * sizeof(int32_t));
-        L.cvec.ensure((size_t)B * n * P * sizeof(cplx<double>));
-        L.cform.ensure((size_t)B * 4 * P * sizeof(double));
-        int32_t* meta = L.cmeta.as<int32_t>();
-        cplx<double>* cv = L.cvec.as<cplx<double>>();
-        double* numc = L.cform.as<double>();
-        double* vc = numc + B * P;
-        const bool v = forms && h.opts.vel != 0;
-        launch_cluster_find(te.w, mmax, mcap, meta, B, s);
-        launch_fill_zero(numc, B * 4 * P * (int64_t)sizeof(double), s);
-        if (forms) {
-          launch_cluster_gather(meta, L.Zt.as<cplx<float>>(), mcap, 1, n * mcap, cv, P, 1, n, B, s);
-          L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, P, B, v ? 4 : 1));
-          launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                                     h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1, cv,
-                                     P, n * P, h.N, P, B, numc, P, v ? vc : nullptr, P, 3 * P,
-                                     L.sws.as<double>(), s);
-        } else {
-          launch_cluster_gather(meta, L.Z.as<cplx<float>>(), 1, n, n * mcap, cv, 1, n, n, B, s);
-          GemmParams<double> g{};
-          g.opA = 'N'; g.opB = 'N';
-          g.m = n; g.n = P; g.k = n;
-          g.alpha = cplx<double>{1, 0};
-          g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
-          g.B = cv; g.ldb = n; g.strideB = n * P;
-          g.batch = B;
-          g.dot_out = numc; g.dot_stride = P;
-          g.herm_dot = 1;
-          g.dotX = cv; g.lddx = n; g.strideX = n * P;
-          L.gws.ensure((size_t)gemm_ws_bytes(g));
-          g.ws = L.gws.p;
-          launch_gemm<double>(g, s);
-        }
-        // the dense path's velocity products read the FP32 vectors: they
-        // get the Ritz vectors too
-        launch_cluster_rr(meta, L.Z64.as<cplx<double>>(), forms ? nullptr : L.Z.as<cplx<float>>(), n,
-                          mcap, te.w, L.den.as<double>(), L.num.as<double>(), numc,
-                          v ? L.vacc.as<double>() : nullptr, vc, B, s);
+        launch_cluster_find(te.w, mmax, mcap, L.cmeta.as<int32_t>(), B, s);
+      }
+      // the reduced matrices A are dead here: scratch for the steps below
+      const size_t a_bytes = (size_t)B * h.ldA * n * sizeof(cplx<T>);
+      if (crr && !vref) {
+        if (forms)
+          cluster_rayleigh_ritz<float>(h, L, B, D64, L.Zt.as<cplx<float>>(), mcap, 1,
+                                       h.opts.vel != 0, te.w, mcap, A, a_bytes, s);
+        else
+          cluster_rayleigh_ritz<float>(h, L, B, D64, L.Z.as<cplx<float>>(), 1, n, false, te.w, mcap,
+                                       A, a_bytes, s);
+      }
+      if (vref) {
+        correct_vectors(h, L, B, D64, reinterpret_cast<cplx<float>*>(A),
+                        reinterpret_cast<cplx<float>*>(te.zt), te.w, mmax, mcap, true, s);
+        // the Rayleigh quotients of the corrected vectors
+        launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
+        if (crr)
+          cluster_rayleigh_ritz<double>(h, L, B, D64, L.Z64.as<cplx<double>>(), 1, n,
+                                        forms && h.opts.vel != 0, te.w, mcap, A, a_bytes, s);
       }
       if (all_modes)
         refine_low_modes_dev(h, L, B, D64, L.Z64.as<cplx<double>>(), te.w, L.den.as<double>(), mcap, s);
@@ -1183,7 +1373,7 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   L.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
   const bool eligible = h.graphs && !L.graph_failed && h.opts.range == 'A' && !L.tm.on;
-  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, (h.use_sparse() ? 1 : 0) | (h.cluster_rr ? 2 : 0),
+  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, (h.use_sparse() ? 1 : 0) | (h.cluster_rr ? 2 : 0) | (h.vec_refine << 2),
                                    h.reduction_code(), h.band_width(), h.n,
                                    h.gemm_tile ? h.gemm_tile : gemm_current_tile()};
   if (eligible && L.graph.exec && L.graph.key == key && L.graph.gen == g_alloc_gen.load() &&
@@ -2093,6 +2283,18 @@ extern "C" int pph_debug_cluster_rr(void* hh, int32_t mode) {
     return PPH_OK;
   }
   h->cluster_rr = mode == 1;
+  return PPH_OK;
+}
+
+extern "C" int pph_debug_vector_refinement(void* hh, int32_t steps) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (steps < 0 || steps > 2)
+    return fail(h, "pph_debug_vector_refinement: steps must be 0, 1 or 2", PPH_ERR_ARG);
+  if (!h) {
+    g_vec_refine_default.store(steps);
+    return PPH_OK;
+  }
+  h->vec_refine = steps;
   return PPH_OK;
 }
 

@@ -76,9 +76,11 @@ void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int32_t* me
                          hipStream_t s);
 // the combination vectors x_i + x_j (kind 0) and x_i + i x_j (kind 1) of every
 // slot in FP64, zero for unused slots: vector p of matrix b, row r at
-// dst[b * n * kClusterSlots + r * drs + p * dps]; the modes from the FP32
-// vectors, element (r, k) of matrix b at src[b * sstride + r * srs + k * sks]
-void launch_cluster_gather(const int32_t* meta, const cplx<float>* src, int64_t srs, int64_t sks,
+// dst[b * n * kClusterSlots + r * drs + p * dps]; the modes from the vectors
+// (FP32 or FP64), element (r, k) of matrix b at src[b * sstride + r * srs +
+// k * sks]
+template <typename Ts>
+void launch_cluster_gather(const int32_t* meta, const cplx<Ts>* src, int64_t srs, int64_t sks,
                            int64_t sstride, cplx<double>* dst, int64_t drs, int64_t dps, int64_t n,
                            int64_t B, hipStream_t s);
 // The k x k problems of the clusters: H and the velocity forms from the
@@ -91,6 +93,42 @@ void launch_cluster_rr(const int32_t* meta, cplx<double>* Z64, cplx<float>* Z32,
                        int64_t mcap, double* lam, double* den, const double* num,
                        const double* numc, double* vacc, const double* vc, int64_t B,
                        hipStream_t s);
+// FP64 residual correction of the vectors (vecref.hip).  Rc = R^H
+// mode-contiguous in FP32 (row r of matrix b at b * strideR + r * ldr) with R
+// = D X - X Theta from the listed blocks; Xt the FP64 vectors mode-contiguous.
+void launch_sparse_residual(const int64_t* rowptr, const int32_t* bcol, const int64_t* tptr,
+                            const int32_t* trow, const int64_t* tblk, int64_t nblk,
+                            const double* phit, const cplx<double>* bph, const cplx<double>* Xt,
+                            int64_t ldt, int64_t strideT, const double* theta, int64_t ldth,
+                            int64_t N, int64_t M, cplx<float>* Rc, int64_t ldr, int64_t strideR,
+                            int64_t B, hipStream_t s);
+// R = Y - X Theta in FP32 (n x M per matrix, column-major, ld n, stride)
+void launch_dense_residual(const cplx<double>* Y, const cplx<double>* X, const double* theta,
+                           int64_t ldth, int64_t n, int64_t M, int64_t stride, cplx<float>* R,
+                           int64_t B, hipStream_t s);
+// E from M = X^H R in place (M x M, ld ldm): M_ij / (theta_j - theta_i) for
+// pairs farther apart than tol[b], else 0; (1 - nrm2_i) / 2 on the diagonal
+void launch_vecref_coeffs(cplx<float>* Mx, int64_t ldm, int64_t strideM, int64_t M,
+                          const double* theta, const double* nrm2, int64_t ldth, const double* tol,
+                          int64_t B, hipStream_t s);
+// X64 += W, X32 = X64 rounded (count elements per matrix at stride)
+void launch_vecref_update(cplx<double>* X64, cplx<float>* X32, const cplx<float>* W, int64_t count,
+                          int64_t stride, int64_t B, hipStream_t s);
+// the banded steps: kVecRefBand modes on either side of the diagonal.
+// Mb[b * smb + j * (2W + 1) + d] holds M_ij (then E_ij), i = j + d - W.
+constexpr int kVecRefBand = 16;
+void launch_band_gram(const cplx<float>* X, int64_t ldx, int64_t sx, const cplx<float>* R,
+                      int64_t rrs, int64_t rjs, int64_t sr, bool rconj, int64_t n, int64_t M,
+                      cplx<float>* Mb, int64_t smb, int64_t B, hipStream_t s);
+void launch_band_coeffs(cplx<float>* Mb, int64_t smb, int64_t M, const double* theta,
+                        const double* nrm2, int64_t ldth, const double* tol, int64_t B, hipStream_t s);
+// Wo = X E over the band (n x M per matrix, column-major, ld n, stride sw)
+void launch_band_apply(const cplx<float>* X, int64_t ldx, int64_t sx, const cplx<float>* Eb,
+                       int64_t smb, int64_t n, int64_t M, cplx<float>* Wo, int64_t sw, int64_t B,
+                       hipStream_t s);
+// tol[b] = scale max_k |lam(b, k)|
+void launch_lam_tol(const double* lam, int64_t M, int64_t ld, double scale, double* tol, int64_t B,
+                    hipStream_t s);
 // Spring-model force constants (spring.hip): pairs (pi, pj) with separation
 // rvec (3 per pair) and kind 0..2 (intralayer shell: radial p[kind],
 // tangential p[3 + kind]) or 3 (interlayer: p[6] exp(-(|r| - inter_r0) /

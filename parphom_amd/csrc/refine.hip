@@ -271,8 +271,9 @@ __global__ void __launch_bounds__(256) k_cluster_find(const double* __restrict__
 }
 
 // thread = (row r, slot p), slot_fast: p varies fastest (mode-contiguous output)
+template <typename Ts>
 __global__ void __launch_bounds__(256) k_cluster_gather(const int32_t* __restrict__ meta,
-                                                        const cplx<float>* __restrict__ src,
+                                                        const cplx<Ts>* __restrict__ src,
                                                         int64_t srs, int64_t sks, int64_t sstride,
                                                         cplx<double>* __restrict__ dst, int64_t drs,
                                                         int64_t dps, int64_t n, int slot_fast) {
@@ -286,8 +287,8 @@ __global__ void __launch_bounds__(256) k_cluster_gather(const int32_t* __restric
   if (p < mb[1]) {
     const int32_t* sl = mb + 2 + 3 * kClusterCount + 2 * p;
     const int64_t i = sl[0], j = sl[1] >> 1;
-    const cplx<float>* S = src + b * sstride + r * srs;
-    const cplx<float> xi = S[i * sks], xj = S[j * sks];
+    const cplx<Ts>* S = src + b * sstride + r * srs;
+    const cplx<Ts> xi = S[i * sks], xj = S[j * sks];
     if (sl[1] & 1) u = cplx<double>{(double)xi.x - (double)xj.y, (double)xi.y + (double)xj.x};
     else u = cplx<double>{(double)xi.x + (double)xj.x, (double)xi.y + (double)xj.y};
   }
@@ -426,15 +427,23 @@ void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int32_t* me
   PPH_CHECK_LAUNCH();
 }
 
-void launch_cluster_gather(const int32_t* meta, const cplx<float>* src, int64_t srs, int64_t sks,
+template <typename Ts>
+void launch_cluster_gather(const int32_t* meta, const cplx<Ts>* src, int64_t srs, int64_t sks,
                            int64_t sstride, cplx<double>* dst, int64_t drs, int64_t dps, int64_t n,
                            int64_t B, hipStream_t s) {
   if (B <= 0 || n <= 0) return;
   const int slot_fast = dps == 1 ? 1 : 0;
-  hipLaunchKernelGGL(k_cluster_gather, dim3((unsigned)ceil_div(n * kClusterSlots, 256), (unsigned)B),
+  hipLaunchKernelGGL(k_cluster_gather<Ts>, dim3((unsigned)ceil_div(n * kClusterSlots, 256), (unsigned)B),
                      dim3(256), 0, s, meta, src, srs, sks, sstride, dst, drs, dps, n, slot_fast);
   PPH_CHECK_LAUNCH();
 }
+
+template void launch_cluster_gather<float>(const int32_t*, const cplx<float>*, int64_t, int64_t,
+                                           int64_t, cplx<double>*, int64_t, int64_t, int64_t,
+                                           int64_t, hipStream_t);
+template void launch_cluster_gather<double>(const int32_t*, const cplx<double>*, int64_t, int64_t,
+                                            int64_t, cplx<double>*, int64_t, int64_t, int64_t,
+                                            int64_t, hipStream_t);
 
 void launch_cluster_rr(const int32_t* meta, cplx<double>* Z64, cplx<float>* Z32, int64_t n,
                        int64_t mcap, double* lam, double* den, const double* num,

@@ -44,7 +44,7 @@ EXPORTED = ("pph_create", "pph_run", "pph_run_device", "pph_assemble", "pph_expe
             "pph_last_error",
             "pph_version", "pph_debug_zgemm", "pph_profile", "pph_debug_blocking",
             "pph_debug_sparse", "pph_debug_reduction", "pph_debug_graphs", "pph_debug_gemm_tile",
-            "pph_debug_cluster_rr")
+            "pph_debug_cluster_rr", "pph_debug_vector_refinement")
 
 # GEMM tile shapes of pph_debug_gemm_tile (0: the process default)
 GEMM_TILES = {"64x64": 1, "128x64": 2, "64x128": 3, "128x128": 4}
@@ -92,6 +92,8 @@ def load(path: str = LIB_PATH):
     lib.pph_debug_gemm_tile.restype = C.c_int
     lib.pph_debug_cluster_rr.argtypes = [C.c_void_p, C.c_int32]
     lib.pph_debug_cluster_rr.restype = C.c_int
+    lib.pph_debug_vector_refinement.argtypes = [C.c_void_p, C.c_int32]
+    lib.pph_debug_vector_refinement.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_int64, C.c_int32, _dp]
@@ -292,6 +294,11 @@ class Handle:
         eigenvalues (on by default; range A)."""
         _check(load().pph_debug_cluster_rr(self._h, 1 if on else 0), self._h)
 
+    def vector_refinement(self, steps=2):
+        """FP64 residual-correction steps of the FP32 path's eigenvectors
+        (0, 1 or 2; 2 by default; range A)."""
+        _check(load().pph_debug_vector_refinement(self._h, int(steps)), self._h)
+
     def graphs(self, mode=-1):
         """Batch graphs (range A): 1 on (the default), 0 off, -1 unchanged.
         Returns (captures, replays) summed over the handle's lanes."""
@@ -336,6 +343,12 @@ def set_default_cluster_rr(on=True):
     """Process default of the FP32 path's cluster Rayleigh-Ritz step (also
     the temporary handles of heevx)."""
     _check(load().pph_debug_cluster_rr(None, 1 if on else 0))
+
+
+def set_default_vector_refinement(steps=2):
+    """Process default of the FP32 path's vector-correction steps (also the
+    temporary handles of heevx)."""
+    _check(load().pph_debug_vector_refinement(None, int(steps)))
 
 
 def set_default_reduction(mode=-1, band=0):

@@ -1,8 +1,12 @@
-"""The FP32 path's Rayleigh-Ritz step over clusters of close eigenvalues
-(marked gpu).  FP32 eigenvectors of eigenvalues closer than the FP32 error of
-the reduction mix inside their cluster, and the FP64 Rayleigh quotients of the
-mixtures fall anywhere between the true eigenvalues; the FP64 step over each
-cluster's span restores the eigenvalues, vectors and velocity forms.
+"""The FP32 path's FP64 refinement of its eigenpairs (marked gpu): the
+Rayleigh-Ritz step over clusters of close eigenvalues and the residual
+correction of the vectors.  FP32 eigenvectors of eigenvalues closer than the
+FP32 error of the reduction mix inside their cluster, and the FP64 Rayleigh
+quotients of the mixtures fall anywhere between the true eigenvalues; the
+FP64 step over each cluster's span restores the eigenvalues, vectors and
+velocity forms.  Every other vector carries a first-order mixture with its
+neighbours, which the velocities inherit; each residual-correction step
+squares it.
 
 * A Hermitian matrix of known spectrum, A = Q diag(lambda) Q^H with Q from the
   QR factorisation of a seeded complex Gaussian matrix, whose spectrum holds
@@ -52,17 +56,30 @@ def reduction_default(gpu_available):
     lib = _lib()
     yield lib
     lib.set_default_reduction(-1, 32)
+    lib.set_default_vector_refinement(2)
 
 
+# per correction steps: bounds on 1 - |q^H z| of the separated modes, on the
+# residual |A z - w z| / lmax and on |Z^H Z - I| (measured at n = 384, both
+# reductions: 2.3e-7 / 1.3e-6 / 4.6e-7 without a step, 1e-15 / 3.4e-11 /
+# 6.5e-10 after one full step; the second step works on a band of 16 modes
+# either side, which settles close pairs (the velocities' error) but leaves
+# the far pairs' second-order residue of a random spectrum: 1.4e-15 /
+# 1.0e-11 / 5.8e-12)
+STEP_BOUNDS = {0: (1e-6, 1e-5, 1e-5), 1: (1e-12, 1e-10, 1e-8), 2: (1e-12, 1e-10, 1e-8)}
+
+
+@pytest.mark.parametrize("steps", [0, 1, 2])
 @pytest.mark.parametrize("reduction", [0, 1])
 @pytest.mark.parametrize("n", [384])
-def test_known_spectrum_with_close_pairs(reduction_default, n, reduction):
+def test_known_spectrum_with_close_pairs(reduction_default, n, reduction, steps):
     """One-stage and two-stage FP32 reductions (the two-stage one returned
     the exactly degenerate triple inside the low modes' Rayleigh-Ritz
     problem as an exactly diagonal block, on which the device Jacobi solver
-    used to lose unitarity)."""
+    used to lose unitarity), with 0, 1 and 2 correction steps."""
     lib = reduction_default
     lib.set_default_reduction(reduction, 32)
+    lib.set_default_vector_refinement(steps)
     rng = np.random.default_rng(2024)
     lmax = 10.0
     lam, designs = _designed_spectrum(n, lmax, rng)
@@ -88,57 +105,67 @@ def test_known_spectrum_with_close_pairs(reduction_default, n, reduction):
             if min(abs(offs[j] - o) for i, o in enumerate(offs) if i != j) < 1e-8:
                 continue              # 1e-9 split: below what FP64 D resolves to 1e-3
             ov = abs(np.vdot(Q[:, k + j], Z[:, k + j]))
-            _log(f"  mode {k + j}: |q^H z| = {ov:.12f}")
+            _log(f"  mode {k + j}: |q^H z| = {ov:.15f}")
             assert ov >= 1.0 - 1e-6, (k, j, ov)
     # the degenerate pair and the 1e-9 pair: the Ritz vectors span the pair
     for k, m in ((5, 3), (200, 2), (300, 2)):
         s = np.linalg.svd(Q[:, k:k + m].conj().T @ Z[:, k:k + m], compute_uv=False)
         assert s.min() >= 1.0 - 1e-6, (k, s)
-    # orthonormal to the FP32 path's level
-    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() <= 1e-5
+    # every separated mode's vector, the residuals and the orthogonality
+    ovb, resb, orthb = STEP_BOUNDS[steps]
+    clustered = {k + j for k, offs in designs.items() for j in range(len(offs))}
+    sep = [k for k in range(n) if k not in clustered]
+    ov = np.abs(np.einsum("ij,ij->j", Q[:, sep].conj(), Z[:, sep]))
+    res = np.linalg.norm(A @ Z - Z * w, axis=0).max() / lmax
+    orth = np.abs(Z.conj().T @ Z - np.eye(n)).max()
+    _log(f"  steps {steps}: 1 - min |q^H z| = {1 - ov.min():.1e}, residual {res:.1e}, "
+         f"orthogonality {orth:.1e}")
+    assert 1.0 - ov.min() <= ovb
+    assert res <= resb
+    assert orth <= orthb
 
 
 def test_tbg_11_12_fp32_against_fp64(gpu_available):
     lib = _lib()
     from parphom_amd import synth
     st, fc, qs = synth.workload("tbg_11_12")
-    n = 3 * st.natom
     q = qs[[0, 13, 64, 127]]
     res = {}
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision="D",
                     batch=len(q), streams=1) as h:
         res["D"] = h.run(q, evec=False)
+    runs = (("plain", False, 0), ("clusters", True, 0), ("one step", True, 1),
+            ("corrected", True, 2))
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision="S",
                     batch=len(q), streams=1) as h:
-        for tag, on in (("S_off", False), ("S_on", True)):
+        for tag, on, steps in runs:
             h.cluster_rr(on)
+            h.vector_refinement(steps)
             res[tag] = h.run(q, evec=False)
-    dw = {}
-    for tag in ("S_off", "S_on"):
+    dw, dv = {}, {}
+    for tag, _, _ in runs:
         dw[tag] = max(np.abs(res[tag]["eval_cm"][i] - res["D"]["eval_cm"][i]).max()
                       for i in range(len(q)))
-        _log(f"(11,12) {tag}: max |d omega| against FP64 = {dw[tag]:.2e} cm^-1")
-    # measured: 3.9e-4 cm^-1 without the step (a pair 6.8e-4 cm^-1 apart near
-    # 151 cm^-1, fully mixed), 4.2e-6 with it
-    assert dw["S_off"] <= 1e-3
-    assert dw["S_on"] <= 2e-5
-    # velocities of the modes the FP64 path resolves (its eigenvalues more
-    # than 1e-8 max|lambda| from their neighbours): FP32 against FP64
-    # relative to the largest velocity
-    dv = {"S_off": 0.0, "S_on": 0.0}
-    for i in range(len(q)):
-        lam = res["D"]["eval_cm"][i]
-        l2 = np.sign(lam) * lam * lam
-        gap = np.minimum(np.r_[np.inf, np.diff(l2)], np.r_[np.diff(l2), np.inf])
-        ok = gap > 1e-8 * np.abs(l2).max()
-        vD = res["D"]["vel"][i]
-        scale = np.abs(vD).max()
-        for tag in dv:
-            d = np.abs(res[tag]["vel"][i] - vD)[ok].max() / scale
+        # velocities of the modes the FP64 path resolves (its eigenvalues
+        # more than 1e-8 max|lambda| from their neighbours), relative to the
+        # largest velocity
+        dv[tag] = 0.0
+        for i in range(len(q)):
+            lam = res["D"]["eval_cm"][i]
+            l2 = np.sign(lam) * lam * lam
+            gap = np.minimum(np.r_[np.inf, np.diff(l2)], np.r_[np.diff(l2), np.inf])
+            ok = gap > 1e-8 * np.abs(l2).max()
+            vD = res["D"]["vel"][i]
+            d = np.abs(res[tag]["vel"][i] - vD)[ok].max() / np.abs(vD).max()
             dv[tag] = max(dv[tag], d)
-    for tag in dv:
-        _log(f"(11,12) {tag}: max |d v| / max |v| over resolved modes = {dv[tag]:.2e}")
-    # measured: 0.40 without the step, 1.1e-2 with it (what remains is the
-    # first-order mixing of FP32 vectors with modes just outside the cluster
-    # tolerance, 1e-5 max|lambda| away)
-    assert dv["S_on"] <= 2e-2
+        _log(f"(11,12) {tag}: max |d omega| against FP64 = {dw[tag]:.2e} cm^-1, "
+             f"max |d v| / max |v| = {dv[tag]:.2e}")
+    # measured: 3.9e-4 cm^-1 and 0.40 with neither step (a pair 6.8e-4 cm^-1
+    # apart near 151 cm^-1, fully mixed); 4.2e-6 and 1.1e-2 with the cluster
+    # step alone (the first-order mixing of FP32 vectors with modes just
+    # outside the cluster tolerance, 1e-5 max|lambda| away); 6.4e-5 of max|v|
+    # after one correction step and 3.8e-8 after two
+    assert dw["plain"] <= 1e-3
+    assert dw["clusters"] <= 2e-5 and dv["clusters"] <= 2e-2
+    assert dw["one step"] <= 2e-5 and dv["one step"] <= 2e-4
+    assert dw["corrected"] <= 2e-5 and dv["corrected"] <= 1e-6
