@@ -547,6 +547,8 @@ def main():
                                     "small-matrix kernel" if small else "one-stage"),
                       "gemm_tile": a.tile,
                       "vector_correction_steps": a.vec_refine if a.vec_refine >= 0 else 2,
+                      "vector_correction_first_step": "BF16 matrix cores for pairs more than 16 "
+                                                      "modes apart, FP32 for the band",
                       "batch_graphs_replayed_in_timed_steps": r32["graph_replays"]}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}

@@ -14,6 +14,9 @@
  *   mode 5: C = A B for Hermitian A (m = k, opA = 'N') from the lower
  *           triangle of A alone (the two triangular products of the band
  *           reduction's panel update, with four real products per complex one)
+ *   mode 6: C = op(A) op(B) on the BF16 matrix cores (prec 'S' only: the
+ *           complex64 inputs rounded to BF16, FP32 accumulation; the first
+ *           step of the vector correction of the FP32 path)
  * prec 'D' complex128, 'S' complex64; column-major; returns 0 or < 0.
  */
 #ifndef PARPHOM_HIP_DEBUG_H
@@ -91,12 +94,14 @@ int pph_debug_cluster_rr(void* h, int32_t mode);
  * with eigenvectors): 0, 1 or 2 (the default).  A step computes the
  * residual R = D X - X Theta in FP64 (block-sparse or dense D), M = X^H R and
  * X <- X + X E with E_ij = M_ij / (theta_j - theta_i) for pairs farther apart
- * than the cluster tolerance and E_ii = (1 - |x_i|^2) / 2, the two products
- * on the FP32 matrix cores; the group velocities then come from the
- * corrected vectors.  Each step squares the vectors' error.  With h == NULL
- * the setting becomes the process default.  No counterpart in the reference
+ * than the cluster tolerance and E_ii = (1 - |x_i|^2) / 2; the group
+ * velocities then come from the corrected vectors.  Each step squares the
+ * vectors' error.  The first step's two products are full, on the BF16
+ * (bf16 = 1, the default) or FP32 (bf16 = 0) matrix cores; the second works
+ * on a band of 16 modes either side of the diagonal, in FP32.  With h == NULL
+ * the settings become the process default.  No counterpart in the reference
  * (pzheevx returns FP64 vectors). */
-int pph_debug_vector_refinement(void* h, int32_t steps);
+int pph_debug_vector_refinement(void* h, int32_t steps, int32_t bf16);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <type_traits>
 #include <atomic>
 #include <complex>
 #include <cmath>
@@ -42,6 +43,8 @@ static std::atomic<uint64_t> g_alloc_gen{0};
 static std::atomic<int> g_cluster_rr_default{1};
 // process default of Handle::vec_refine (pph_debug_vector_refinement with no handle)
 static std::atomic<int> g_vec_refine_default{2};
+// process default of Handle::vec_bf16
+static std::atomic<int> g_vec_bf16_default{1};
 
 struct DevBuf {
   void* p = nullptr;
@@ -189,7 +192,8 @@ struct Lane {
   DevBuf Z, Z64, Tm, VT, Wt, vacc, num, den, lam, om, WT, C12, CG;
   DevBuf bv, Zt;           // block phase sums of D / dD, mode-contiguous vectors (block-sparse forms)
   DevBuf cmeta, cvec, cform;  // cluster Rayleigh-Ritz: cluster lists, combination vectors, their forms
-  DevBuf vxt, vtol;        // vector correction: FP64 mode-contiguous vectors of a chunk, pair tolerances
+  DevBuf vxt, vtol, vband; // vector correction: FP64 mode-contiguous vectors of a chunk, pair
+                           // tolerances, the band of E (split first step)
   DevBuf gws, sws;         // partials of the GEMM dot epilogue / split-K and of the sparse forms
   DevBuf meta;             // per-matrix offsets and counts of the batch's outputs
   // two-stage reduction: band, bulge-chasing reflectors and their block
@@ -247,6 +251,8 @@ struct Handle : Lane {
   bool cluster_rr = g_cluster_rr_default.load() != 0;
   // FP32 path with eigenvectors: FP64 residual correction steps of the vectors (range A)
   int vec_refine = g_vec_refine_default.load();
+  // the first correction step's two full products on the BF16 matrix cores
+  bool vec_bf16 = g_vec_bf16_default.load() != 0;
   int gemm_tile = 0;       // GEMM tile shape of the eigensolver (kGemmTile*; 0: process default)
   int nb_panel = 32;       // reflectors per reduction panel
   int nb_back = 128;       // reflectors per back-transformation block (<= kVpad)
@@ -836,19 +842,36 @@ static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D
                               bc, s);
       }
     }
-    // after the first step only pairs of close modes carry a noticeable
-    // error: a band of kVecRefBand modes on either side of the diagonal
-    if (step > 0 && mmax > 4 * kVecRefBand) {
-      const int64_t smb = mcap * (2 * kVecRefBand + 1);
+    const int64_t smb = mcap * (2 * kVecRefBand + 1);
+    const bool band_ok = mmax > 4 * kVecRefBand;
+    // the band of kVecRefBand modes on either side of the diagonal in FP32:
+    // Eb = the band of E in L.vband
+    auto band_part = [&](cplx<float>* Eb) {
       if (forms)
-        launch_band_gram(X32, n, sz, R32, mcap, 1, sz, true, n, mmax, Em, smb, B, s);
+        launch_band_gram(X32, n, sz, R32, mcap, 1, sz, true, n, mmax, Eb, smb, B, s);
       else
-        launch_band_gram(X32, n, sz, R32, 1, n, sz, false, n, mmax, Em, smb, B, s);
-      launch_band_coeffs(Em, smb, mmax, theta, den, mcap, L.vtol.as<double>(), B, s);
-      launch_band_apply(X32, n, sz, Em, smb, n, mmax, R32, sz, B, s);
+        launch_band_gram(X32, n, sz, R32, 1, n, sz, false, n, mmax, Eb, smb, B, s);
+      launch_band_coeffs(Eb, smb, mmax, theta, den, mcap, L.vtol.as<double>(), B, s);
+    };
+    // after the first step only pairs of close modes carry a noticeable
+    // error: the band alone
+    if (step > 0 && band_ok) {
+      band_part(Em);
+      launch_band_apply(X32, n, sz, Em, smb, n, mmax, R32, sz, false, B, s);
       launch_vecref_update(X64, X32, R32, n * mmax, sz, B, s);
       launch_colnorm2<double>(X64, n, mmax, n, sz, B, den, mcap, s);
       continue;
+    }
+    // the first step: full products.  With BF16 products, the band (whose
+    // pairs of close modes carry the large coefficients) comes from the FP32
+    // banded kernels and the BF16 products supply only the far pairs, whose
+    // coefficients need a few digits
+    const bool split = h.vec_bf16 && band_ok;
+    cplx<float>* Eband = nullptr;
+    if (split) {
+      L.vband.ensure((size_t)B * smb * sizeof(cplx<float>));
+      Eband = L.vband.as<cplx<float>>();
+      band_part(Eband);
     }
     // M = X^H R on the matrix cores (R stored as R^H, mode-contiguous, on
     // the block-sparse route)
@@ -861,9 +884,11 @@ static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D
       g.B = R32; g.ldb = forms ? mcap : n; g.strideB = sz;
       g.C = Em; g.ldc = mcap; g.strideC = mcap * mcap;
       g.batch = B;
-      launch_gemm<float>(g, s);
+      if (split) launch_cgemm_bf16(g, s);
+      else launch_gemm<float>(g, s);
     }
-    launch_vecref_coeffs(Em, mcap, mcap * mcap, mmax, theta, den, mcap, L.vtol.as<double>(), B, s);
+    launch_vecref_coeffs(Em, mcap, mcap * mcap, mmax, theta, den, mcap, L.vtol.as<double>(),
+                         split ? kVecRefBand : -1, B, s);
     // W = X E (into R32), then X += W and the FP32 copy
     {
       GemmParams<float> g{};
@@ -874,8 +899,10 @@ static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D
       g.B = Em; g.ldb = mcap; g.strideB = mcap * mcap;
       g.C = R32; g.ldc = n; g.strideC = sz;
       g.batch = B;
-      launch_gemm<float>(g, s);
+      if (split) launch_cgemm_bf16(g, s);
+      else launch_gemm<float>(g, s);
     }
+    if (split) launch_band_apply(X32, n, sz, Eband, smb, n, mmax, R32, sz, true, B, s);
     launch_vecref_update(X64, X32, R32, n * mmax, sz, B, s);
     launch_colnorm2<double>(X64, n, mmax, n, sz, B, den, mcap, s);
   }
@@ -1373,7 +1400,8 @@ static void run_batch(Handle& h, Lane& L, int64_t B, const double* q_host_rows /
   L.qdev.ensure((size_t)B * 3 * sizeof(double));
   PPH_HIP(hipMemcpyAsync(L.qdev.p, q_host_rows, B * 3 * sizeof(double), hipMemcpyHostToDevice, s));
   const bool eligible = h.graphs && !L.graph_failed && h.opts.range == 'A' && !L.tm.on;
-  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, (h.use_sparse() ? 1 : 0) | (h.cluster_rr ? 2 : 0) | (h.vec_refine << 2),
+  const std::array<int64_t, 8> key{B, h.nb_panel, h.nb_back, (h.use_sparse() ? 1 : 0) | (h.cluster_rr ? 2 : 0) | (h.vec_refine << 2) |
+                                       (h.vec_bf16 ? 16 : 0),
                                    h.reduction_code(), h.band_width(), h.n,
                                    h.gemm_tile ? h.gemm_tile : gemm_current_tile()};
   if (eligible && L.graph.exec && L.graph.key == key && L.graph.gen == g_alloc_gen.load() &&
@@ -2286,15 +2314,19 @@ extern "C" int pph_debug_cluster_rr(void* hh, int32_t mode) {
   return PPH_OK;
 }
 
-extern "C" int pph_debug_vector_refinement(void* hh, int32_t steps) {
+extern "C" int pph_debug_vector_refinement(void* hh, int32_t steps, int32_t bf16) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (steps < 0 || steps > 2)
     return fail(h, "pph_debug_vector_refinement: steps must be 0, 1 or 2", PPH_ERR_ARG);
+  if (bf16 < 0 || bf16 > 1)
+    return fail(h, "pph_debug_vector_refinement: bf16 must be 0 or 1", PPH_ERR_ARG);
   if (!h) {
     g_vec_refine_default.store(steps);
+    g_vec_bf16_default.store(bf16);
     return PPH_OK;
   }
   h->vec_refine = steps;
+  h->vec_bf16 = bf16 == 1;
   return PPH_OK;
 }
 
@@ -2391,8 +2423,15 @@ extern "C" int pph_debug_zgemm(char prec, char opA, char opB, int64_t m, int64_t
     }
     dW.ensure((size_t)gemm_ws_bytes(g));
     g.ws = dW.p;
+    if constexpr (std::is_same_v<T, float>) {
+      if (mode == 6) {                   // the BF16 matrix-core path (complex64 in and out)
+        launch_cgemm_bf16(g, s);
+        return;
+      }
+    }
     launch_gemm<T>(g, s);
   };
+  if (mode == 6 && prec != 'S') return fail(nullptr, "pph_debug_zgemm: mode 6 is complex64 only", PPH_ERR_ARG);
   if (prec == 'S') run(float{}); else run(double{});
   PPH_HIP(hipMemcpyAsync(C, dC.p, es * ldc * n, hipMemcpyDeviceToHost, s));
   if (dot) PPH_HIP(hipMemcpyAsync(dot, dX.p, 8 * n, hipMemcpyDeviceToHost, s));

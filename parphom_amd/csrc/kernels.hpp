@@ -107,10 +107,11 @@ void launch_dense_residual(const cplx<double>* Y, const cplx<double>* X, const d
                            int64_t ldth, int64_t n, int64_t M, int64_t stride, cplx<float>* R,
                            int64_t B, hipStream_t s);
 // E from M = X^H R in place (M x M, ld ldm): M_ij / (theta_j - theta_i) for
-// pairs farther apart than tol[b], else 0; (1 - nrm2_i) / 2 on the diagonal
+// pairs farther apart than tol[b], else 0; (1 - nrm2_i) / 2 on the diagonal;
+// with band >= 0, 0 for |i - j| <= band
 void launch_vecref_coeffs(cplx<float>* Mx, int64_t ldm, int64_t strideM, int64_t M,
                           const double* theta, const double* nrm2, int64_t ldth, const double* tol,
-                          int64_t B, hipStream_t s);
+                          int64_t band, int64_t B, hipStream_t s);
 // X64 += W, X32 = X64 rounded (count elements per matrix at stride)
 void launch_vecref_update(cplx<double>* X64, cplx<float>* X32, const cplx<float>* W, int64_t count,
                           int64_t stride, int64_t B, hipStream_t s);
@@ -122,10 +123,11 @@ void launch_band_gram(const cplx<float>* X, int64_t ldx, int64_t sx, const cplx<
                       cplx<float>* Mb, int64_t smb, int64_t B, hipStream_t s);
 void launch_band_coeffs(cplx<float>* Mb, int64_t smb, int64_t M, const double* theta,
                         const double* nrm2, int64_t ldth, const double* tol, int64_t B, hipStream_t s);
-// Wo = X E over the band (n x M per matrix, column-major, ld n, stride sw)
+// Wo = X E over the band (n x M per matrix, column-major, ld n, stride sw),
+// or Wo += X E with accumulate
 void launch_band_apply(const cplx<float>* X, int64_t ldx, int64_t sx, const cplx<float>* Eb,
-                       int64_t smb, int64_t n, int64_t M, cplx<float>* Wo, int64_t sw, int64_t B,
-                       hipStream_t s);
+                       int64_t smb, int64_t n, int64_t M, cplx<float>* Wo, int64_t sw, bool accumulate,
+                       int64_t B, hipStream_t s);
 // tol[b] = scale max_k |lam(b, k)|
 void launch_lam_tol(const double* lam, int64_t M, int64_t ld, double scale, double* tol, int64_t B,
                     hipStream_t s);
@@ -215,6 +217,10 @@ struct GemmParams {
 template <typename T> int64_t gemm_ws_bytes(const GemmParams<T>& p);
 
 template <typename T> void launch_gemm(const GemmParams<T>& p, hipStream_t s);
+// C = op(A) op(B) on the BF16 matrix cores (gemm_bf16.hip): complex64 in and
+// out, inputs rounded to BF16, FP32 accumulation; opA / opB 'N' or 'C', alpha
+// 1, beta 0, one plain product (no epilogues, split-K or inner batches)
+void launch_cgemm_bf16(const GemmParams<float>& g, hipStream_t s);
 // GEMM tile shapes (rows x columns of C per 256-thread workgroup); products
 // with the dot epilogue and outputs at most 32 columns wide keep their own
 // (64 x 64 and 64 x 32)

@@ -101,17 +101,22 @@ __global__ void __launch_bounds__(256) k_dense_residual(const cplx<double>* __re
 
 // E from M in place (M x M per matrix, column-major, ld ldm): E_ij =
 // M_ij / (theta_j - theta_i) for pairs farther apart than tol[b], 0 for closer
-// ones, E_ii = (1 - nrm2_i) / 2
+// ones, E_ii = (1 - nrm2_i) / 2; with band >= 0 the entries with |i - j| <=
+// band (the diagonal included) are 0 instead (a banded part supplies them)
 __global__ void __launch_bounds__(256) k_vecref_coeffs(cplx<float>* __restrict__ Mx, int64_t ldm,
                                                        int64_t strideM, int64_t M,
                                                        const double* __restrict__ theta,
                                                        const double* __restrict__ nrm2, int64_t ldth,
-                                                       const double* __restrict__ tol) {
+                                                       const double* __restrict__ tol, int64_t band) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.y;
   if (t >= M * M) return;
   const int64_t i = t % M, j = t / M;
   cplx<float>* e = Mx + b * strideM + j * ldm + i;
+  if (band >= 0 && i - j <= band && j - i <= band) {
+    *e = cplx<float>{0.0f, 0.0f};
+    return;
+  }
   if (i == j) {
     *e = cplx<float>{(float)(0.5 * (1.0 - nrm2[b * ldth + i])), 0.0f};
     return;
@@ -239,7 +244,8 @@ template <int W>
 __global__ void __launch_bounds__(256) k_band_apply(const cplx<float>* __restrict__ X, int64_t ldx,
                                                     int64_t sx, const cplx<float>* __restrict__ Eb,
                                                     int64_t smb, int64_t n, int64_t M,
-                                                    cplx<float>* __restrict__ Wo, int64_t sw) {
+                                                    cplx<float>* __restrict__ Wo, int64_t sw,
+                                                    int accumulate) {
   constexpr int TJ = 64, RC = 32, WI = TJ + 2 * W, ND = 2 * W + 1;
   __shared__ cplx<float> xs[RC][WI + 1];
   __shared__ cplx<float> ws[RC][TJ + 1];
@@ -269,7 +275,10 @@ __global__ void __launch_bounds__(256) k_band_apply(const cplx<float>* __restric
     for (int t = tid; t < RC * TJ; t += 256) {
       const int rr = t % RC, c = t / RC;
       const int64_t r = r0 + rr, jj = j0 + c;
-      if (r < n && jj < M) Wo[b * sw + jj * n + r] = ws[rr][c];
+      if (r < n && jj < M) {
+        cplx<float>* o = Wo + b * sw + jj * n + r;
+        *o = accumulate ? cadd(*o, ws[rr][c]) : ws[rr][c];
+      }
     }
     __syncthreads();
   }
@@ -310,10 +319,10 @@ void launch_dense_residual(const cplx<double>* Y, const cplx<double>* X, const d
 
 void launch_vecref_coeffs(cplx<float>* Mx, int64_t ldm, int64_t strideM, int64_t M,
                           const double* theta, const double* nrm2, int64_t ldth, const double* tol,
-                          int64_t B, hipStream_t s) {
+                          int64_t band, int64_t B, hipStream_t s) {
   if (M <= 0 || B <= 0) return;
   hipLaunchKernelGGL(k_vecref_coeffs, dim3((unsigned)ceil_div(M * M, 256), (unsigned)B), dim3(256), 0,
-                     s, Mx, ldm, strideM, M, theta, nrm2, ldth, tol);
+                     s, Mx, ldm, strideM, M, theta, nrm2, ldth, tol, band);
   PPH_CHECK_LAUNCH();
 }
 
@@ -345,11 +354,11 @@ void launch_band_coeffs(cplx<float>* Mb, int64_t smb, int64_t M, const double* t
 }
 
 void launch_band_apply(const cplx<float>* X, int64_t ldx, int64_t sx, const cplx<float>* Eb,
-                       int64_t smb, int64_t n, int64_t M, cplx<float>* Wo, int64_t sw, int64_t B,
-                       hipStream_t s) {
+                       int64_t smb, int64_t n, int64_t M, cplx<float>* Wo, int64_t sw, bool accumulate,
+                       int64_t B, hipStream_t s) {
   if (n <= 0 || M <= 0 || B <= 0) return;
   hipLaunchKernelGGL((k_band_apply<kVecRefBand>), dim3((unsigned)ceil_div(M, 64), (unsigned)B),
-                     dim3(256), 0, s, X, ldx, sx, Eb, smb, n, M, Wo, sw);
+                     dim3(256), 0, s, X, ldx, sx, Eb, smb, n, M, Wo, sw, accumulate ? 1 : 0);
   PPH_CHECK_LAUNCH();
 }
 

@@ -92,7 +92,7 @@ def load(path: str = LIB_PATH):
     lib.pph_debug_gemm_tile.restype = C.c_int
     lib.pph_debug_cluster_rr.argtypes = [C.c_void_p, C.c_int32]
     lib.pph_debug_cluster_rr.restype = C.c_int
-    lib.pph_debug_vector_refinement.argtypes = [C.c_void_p, C.c_int32]
+    lib.pph_debug_vector_refinement.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.pph_debug_vector_refinement.restype = C.c_int
     lib.pph_debug_zgemm.argtypes = [C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
@@ -294,10 +294,11 @@ class Handle:
         eigenvalues (on by default; range A)."""
         _check(load().pph_debug_cluster_rr(self._h, 1 if on else 0), self._h)
 
-    def vector_refinement(self, steps=2):
+    def vector_refinement(self, steps=2, bf16=True):
         """FP64 residual-correction steps of the FP32 path's eigenvectors
-        (0, 1 or 2; 2 by default; range A)."""
-        _check(load().pph_debug_vector_refinement(self._h, int(steps)), self._h)
+        (0, 1 or 2; 2 by default; range A); bf16: the first step's products
+        on the BF16 matrix cores (the default) or the FP32 ones."""
+        _check(load().pph_debug_vector_refinement(self._h, int(steps), 1 if bf16 else 0), self._h)
 
     def graphs(self, mode=-1):
         """Batch graphs (range A): 1 on (the default), 0 off, -1 unchanged.
@@ -345,10 +346,11 @@ def set_default_cluster_rr(on=True):
     _check(load().pph_debug_cluster_rr(None, 1 if on else 0))
 
 
-def set_default_vector_refinement(steps=2):
-    """Process default of the FP32 path's vector-correction steps (also the
-    temporary handles of heevx)."""
-    _check(load().pph_debug_vector_refinement(None, int(steps)))
+def set_default_vector_refinement(steps=2, bf16=True):
+    """Process default of the FP32 path's vector-correction steps and of
+    the precision of the first step's products (also the temporary handles of
+    heevx)."""
+    _check(load().pph_debug_vector_refinement(None, int(steps), 1 if bf16 else 0))
 
 
 def set_default_reduction(mode=-1, band=0):

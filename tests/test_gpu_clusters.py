@@ -56,30 +56,34 @@ def reduction_default(gpu_available):
     lib = _lib()
     yield lib
     lib.set_default_reduction(-1, 32)
-    lib.set_default_vector_refinement(2)
+    lib.set_default_vector_refinement(2, True)
 
 
-# per correction steps: bounds on 1 - |q^H z| of the separated modes, on the
-# residual |A z - w z| / lmax and on |Z^H Z - I| (measured at n = 384, both
-# reductions: 2.3e-7 / 1.3e-6 / 4.6e-7 without a step, 1e-15 / 3.4e-11 /
-# 6.5e-10 after one full step; the second step works on a band of 16 modes
-# either side, which settles close pairs (the velocities' error) but leaves
-# the far pairs' second-order residue of a random spectrum: 1.4e-15 /
-# 1.0e-11 / 5.8e-12)
-STEP_BOUNDS = {0: (1e-6, 1e-5, 1e-5), 1: (1e-12, 1e-10, 1e-8), 2: (1e-12, 1e-10, 1e-8)}
+# per (correction steps, BF16 first step): bounds on 1 - |q^H z| of the
+# separated modes, on the residual |A z - w z| / lmax and on |Z^H Z - I|
+# (measured at n = 384, both reductions: 2.3e-7 / 1.3e-6 / 4.6e-7 without a
+# step; 1e-15 / 3.4e-11 / 6.5e-10 after one FP32 step, 7.4e-10 / 7.7e-9 /
+# 1.1e-8 after one split BF16 / FP32-band step; the second step works on a
+# band of 16 modes either side, which settles close pairs (the velocities'
+# error) but leaves the far pairs' residue of a random spectrum: 1.4e-15 /
+# 3.4e-11 / 3.1e-11 after FP32, 1.6e-15 / 7.7e-9 / 1.1e-8 after BF16)
+STEP_BOUNDS = {(0, True): (1e-6, 1e-5, 1e-5), (1, False): (1e-12, 1e-10, 1e-8),
+               (1, True): (1e-8, 1e-7, 1e-7), (2, False): (1e-12, 1e-10, 1e-9),
+               (2, True): (1e-12, 1e-7, 1e-7)}
 
 
-@pytest.mark.parametrize("steps", [0, 1, 2])
+@pytest.mark.parametrize("steps,bf16", list(STEP_BOUNDS))
 @pytest.mark.parametrize("reduction", [0, 1])
 @pytest.mark.parametrize("n", [384])
-def test_known_spectrum_with_close_pairs(reduction_default, n, reduction, steps):
+def test_known_spectrum_with_close_pairs(reduction_default, n, reduction, steps, bf16):
     """One-stage and two-stage FP32 reductions (the two-stage one returned
     the exactly degenerate triple inside the low modes' Rayleigh-Ritz
     problem as an exactly diagonal block, on which the device Jacobi solver
-    used to lose unitarity), with 0, 1 and 2 correction steps."""
+    used to lose unitarity), with 0, 1 and 2 correction steps, the first
+    with FP32 or split BF16 / FP32-band products."""
     lib = reduction_default
     lib.set_default_reduction(reduction, 32)
-    lib.set_default_vector_refinement(steps)
+    lib.set_default_vector_refinement(steps, bf16)
     rng = np.random.default_rng(2024)
     lmax = 10.0
     lam, designs = _designed_spectrum(n, lmax, rng)
@@ -112,7 +116,7 @@ def test_known_spectrum_with_close_pairs(reduction_default, n, reduction, steps)
         s = np.linalg.svd(Q[:, k:k + m].conj().T @ Z[:, k:k + m], compute_uv=False)
         assert s.min() >= 1.0 - 1e-6, (k, s)
     # every separated mode's vector, the residuals and the orthogonality
-    ovb, resb, orthb = STEP_BOUNDS[steps]
+    ovb, resb, orthb = STEP_BOUNDS[(steps, bf16)]
     clustered = {k + j for k, offs in designs.items() for j in range(len(offs))}
     sep = [k for k in range(n) if k not in clustered]
     ov = np.abs(np.einsum("ij,ij->j", Q[:, sep].conj(), Z[:, sep]))
@@ -134,16 +138,16 @@ def test_tbg_11_12_fp32_against_fp64(gpu_available):
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision="D",
                     batch=len(q), streams=1) as h:
         res["D"] = h.run(q, evec=False)
-    runs = (("plain", False, 0), ("clusters", True, 0), ("one step", True, 1),
-            ("corrected", True, 2))
+    runs = (("plain", False, 0, True), ("clusters", True, 0, True), ("one step", True, 1, True),
+            ("corrected FP32", True, 2, False), ("corrected", True, 2, True))
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True, precision="S",
                     batch=len(q), streams=1) as h:
-        for tag, on, steps in runs:
+        for tag, on, steps, bf16 in runs:
             h.cluster_rr(on)
-            h.vector_refinement(steps)
+            h.vector_refinement(steps, bf16)
             res[tag] = h.run(q, evec=False)
     dw, dv = {}, {}
-    for tag, _, _ in runs:
+    for tag, _, _, _ in runs:
         dw[tag] = max(np.abs(res[tag]["eval_cm"][i] - res["D"]["eval_cm"][i]).max()
                       for i in range(len(q)))
         # velocities of the modes the FP64 path resolves (its eigenvalues
@@ -164,8 +168,32 @@ def test_tbg_11_12_fp32_against_fp64(gpu_available):
     # apart near 151 cm^-1, fully mixed); 4.2e-6 and 1.1e-2 with the cluster
     # step alone (the first-order mixing of FP32 vectors with modes just
     # outside the cluster tolerance, 1e-5 max|lambda| away); 6.4e-5 of max|v|
-    # after one correction step and 3.8e-8 after two
+    # after one correction step; after two, 6.8e-8 with FP32 products and
+    # 9.2e-8 with the split BF16 / FP32-band first step (the default)
     assert dw["plain"] <= 1e-3
     assert dw["clusters"] <= 2e-5 and dv["clusters"] <= 2e-2
     assert dw["one step"] <= 2e-5 and dv["one step"] <= 2e-4
+    assert dw["corrected FP32"] <= 2e-5 and dv["corrected FP32"] <= 1e-6
     assert dw["corrected"] <= 2e-5 and dv["corrected"] <= 1e-6
+
+
+@pytest.mark.parametrize("mnk", [(77, 130, 45), (300, 261, 97), (129, 65, 33), (256, 256, 512)])
+@pytest.mark.parametrize("ops", ["NN", "NC", "CN", "CC"])
+def test_bf16_complex_gemm(gpu_available, mnk, ops):
+    """The BF16 matrix-core product of the first correction step (complex64
+    inputs rounded to BF16, FP32 accumulation) against numpy, on ragged
+    shapes: every entry within 4e-3 of the sum of the terms' magnitudes
+    (BF16 keeps 8 significant bits; measured at most 2.9e-3)."""
+    lib = _lib()
+    m, n, k = mnk
+    opA, opB = ops
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = rng.standard_normal((m, k) if opA == "N" else (k, m)) + \
+        1j * rng.standard_normal((m, k) if opA == "N" else (k, m))
+    B = rng.standard_normal((k, n) if opB == "N" else (n, k)) + \
+        1j * rng.standard_normal((k, n) if opB == "N" else (n, k))
+    oA = A if opA == "N" else A.conj().T
+    oB = B if opB == "N" else B.conj().T
+    C, _ = lib.debug_zgemm(A, B, opA, opB, "S", mode=6)
+    scale = np.abs(oA) @ np.abs(oB)
+    assert np.max(np.abs(C - oA @ oB) / scale) <= 4e-3
