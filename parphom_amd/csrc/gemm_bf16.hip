@@ -18,6 +18,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace pph {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -25,6 +27,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int BT = 128;        // tile rows and columns
 constexpr int BK = 32;         // K per step (one MFMA)
 constexpr int BKP = BK + 8;    // LDS row length in BF16 (16 bytes of padding)
+constexpr int kBf16TileN = 64; // tile columns (64: two waves per SIMD)
 
 struct Bf16Gemm {
   // op(A)(m, k) = A[m * sam + k * sak], conjugated when ca; op(B)(k, n) =
@@ -37,74 +40,123 @@ struct Bf16Gemm {
   int ca, cb;
 };
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+// KA / KB: the operand is contiguous along k (op 'C' of A, op 'N' of B, with
+// 16-byte aligned pairs): it is loaded two elements (16 bytes) per lane, 16
+// lanes per 256-byte row segment of the tile; otherwise contiguous along its
+// rows (columns): one element per lane, consecutive lanes on consecutive rows
+// TN: tile columns, 128 (each wave 64 x 64, one wave per SIMD) or 64 (each
+// wave 64 x 32: fewer registers, two waves per SIMD hide the global loads)
+template <bool KA, bool KB, int TN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TN == 128 ? 1 : 2)))
 k_cgemm_bf16(Bf16Gemm p) {
+  constexpr int NJ = TN / 32;          // 16-column MFMA blocks per wave
   // three planes (re, im, re + im) of the A and B tiles, row-major in k
   __shared__ __align__(16) __bf16 As[3][BT][BKP];
-  __shared__ __align__(16) __bf16 Bs[3][BT][BKP];
+  __shared__ __align__(16) __bf16 Bs[3][TN][BKP];
   const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
   const int64_t b = blockIdx.z;
-  const int64_t m0 = (int64_t)blockIdx.x * BT, n0 = (int64_t)blockIdx.y * BT;
+  const int64_t m0 = (int64_t)blockIdx.x * BT, n0 = (int64_t)blockIdx.y * TN;
   const cplx<float>* A = p.A + b * p.strideA;
   const cplx<float>* B = p.B + b * p.strideB;
-  // staging: work item w = tid + 256 h (h = 0, 1) is row (m or n) w % 128,
-  // k group w / 128 (8 consecutive k)
-  cplx<float> ra[2][8], rb[2][8];
+  // 16 elements of each operand per thread and K step: k-contiguous, pair e
+  // = tid + 256 h (h < 8) is row e / 16, k = 2 (e % 16) (+1); otherwise item
+  // w = tid + 256 h (h < 2) is row w % 128, k = 8 (w / 128) + t
+  cplx<float> ra[16], rb[TN / 8];
+  // RT rows of the operand's tile, RT / 8 elements per thread: NH = RT / 16
+  // pairs (k-contiguous) or NH / 4 items of 8 consecutive k
+  auto load_op = [&](cplx<float>* reg, const cplx<float>* X, int64_t s_row, int64_t s_k, int64_t r0,
+                     int64_t R, int64_t k0, bool kcont, auto rt) {
+    constexpr int RT = decltype(rt)::value, NH = RT / 16;
+    if (kcont) {
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int e = tid + 256 * h, r = e / 16, kp = e % 16;
+        const int64_t row = r0 + r, k = k0 + 2 * kp;
+        if (row < R && k + 1 < p.K) {
+          const float4 v = *reinterpret_cast<const float4*>(X + row * s_row + k);
+          reg[2 * h] = cplx<float>{v.x, v.y};
+          reg[2 * h + 1] = cplx<float>{v.z, v.w};
+        } else {
+          reg[2 * h] = (row < R && k < p.K) ? X[row * s_row + k] : cplx<float>{0.0f, 0.0f};
+          reg[2 * h + 1] = cplx<float>{0.0f, 0.0f};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int h = 0; h < NH / 4; ++h) {
+        const int w = tid + 256 * h, r = w % RT, kg = w / RT;
+        const int64_t row = r0 + r;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int64_t k = k0 + 8 * kg + t;
+          reg[8 * h + t] = (row < R && k < p.K) ? X[row * s_row + k * s_k] : cplx<float>{0.0f, 0.0f};
+        }
+      }
+    }
+  };
   auto load = [&](int64_t k0) {
+    load_op(ra, A, p.sam, p.sak, m0, p.M, k0, KA, std::integral_constant<int, BT>{});
+    load_op(rb, B, p.sbn, p.sbk, n0, p.N, k0, KB, std::integral_constant<int, TN>{});
+  };
+  auto store_op = [&](const cplx<float>* reg, __bf16* S, int conj, bool kcont, auto rt) {
+    constexpr int RT = decltype(rt)::value, NH = RT / 16;
+    auto at = [&](int pl, int r, int k) { return S + ((size_t)pl * RT + r) * BKP + k; };
+    if (kcont) {
+      typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int w = tid + 256 * h, r = w % BT, kg = w / BT;
-      const int64_t m = m0 + r, nn = n0 + r;
+      for (int h = 0; h < NH; ++h) {
+        const int e = tid + 256 * h, r = e / 16, kp = e % 16;
+        bf16x2 vr, vi, vs;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int64_t k = k0 + 8 * kg + t;
-        const bool oka = m < p.M && k < p.K, okb = nn < p.N && k < p.K;
-        ra[h][t] = oka ? A[m * p.sam + k * p.sak] : cplx<float>{0.0f, 0.0f};
-        rb[h][t] = okb ? B[k * p.sbk + nn * p.sbn] : cplx<float>{0.0f, 0.0f};
+        for (int t = 0; t < 2; ++t) {
+          const float xr = reg[2 * h + t].x, xi = conj ? -reg[2 * h + t].y : reg[2 * h + t].y;
+          vr[t] = (__bf16)xr;
+          vi[t] = (__bf16)xi;
+          vs[t] = (__bf16)(xr + xi);
+        }
+        *reinterpret_cast<bf16x2*>(at(0, r, 2 * kp)) = vr;
+        *reinterpret_cast<bf16x2*>(at(1, r, 2 * kp)) = vi;
+        *reinterpret_cast<bf16x2*>(at(2, r, 2 * kp)) = vs;
+      }
+    } else {
+#pragma unroll
+      for (int h = 0; h < NH / 4; ++h) {
+        const int w = tid + 256 * h, r = w % RT, kg = w / RT;
+        bf16x8 vr, vi, vs;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const float xr = reg[8 * h + t].x, xi = conj ? -reg[8 * h + t].y : reg[8 * h + t].y;
+          vr[t] = (__bf16)xr;
+          vi[t] = (__bf16)xi;
+          vs[t] = (__bf16)(xr + xi);
+        }
+        *reinterpret_cast<bf16x8*>(at(0, r, 8 * kg)) = vr;
+        *reinterpret_cast<bf16x8*>(at(1, r, 8 * kg)) = vi;
+        *reinterpret_cast<bf16x8*>(at(2, r, 8 * kg)) = vs;
       }
     }
   };
   auto store = [&]() {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int w = tid + 256 * h, r = w % BT, kg = w / BT;
-      bf16x8 ar, ai, as, br, bi, bs;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const float axr = ra[h][t].x, axi = p.ca ? -ra[h][t].y : ra[h][t].y;
-        const float bxr = rb[h][t].x, bxi = p.cb ? -rb[h][t].y : rb[h][t].y;
-        ar[t] = (__bf16)axr;
-        ai[t] = (__bf16)axi;
-        as[t] = (__bf16)(axr + axi);
-        br[t] = (__bf16)bxr;
-        bi[t] = (__bf16)bxi;
-        bs[t] = (__bf16)(bxr + bxi);
-      }
-      *reinterpret_cast<bf16x8*>(&As[0][r][8 * kg]) = ar;
-      *reinterpret_cast<bf16x8*>(&As[1][r][8 * kg]) = ai;
-      *reinterpret_cast<bf16x8*>(&As[2][r][8 * kg]) = as;
-      *reinterpret_cast<bf16x8*>(&Bs[0][r][8 * kg]) = br;
-      *reinterpret_cast<bf16x8*>(&Bs[1][r][8 * kg]) = bi;
-      *reinterpret_cast<bf16x8*>(&Bs[2][r][8 * kg]) = bs;
-    }
+    store_op(ra, &As[0][0][0], p.ca, KA, std::integral_constant<int, BT>{});
+    store_op(rb, &Bs[0][0][0], p.cb, KB, std::integral_constant<int, TN>{});
   };
-  f4 acc[3][4][4];
+  f4 acc[3][4][NJ];
 #pragma unroll
   for (int s = 0; s < 3; ++s)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[s][i][j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  const int wr = (wave / 2) * 64, wc = (wave % 2) * 64;
+      for (int j = 0; j < NJ; ++j) acc[s][i][j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int wr = (wave / 2) * 64, wc = (wave % 2) * (TN / 2);
   const int lr = lane % 16, lk = 8 * (lane / 16);
   load(0);
   for (int64_t k0 = 0; k0 < p.K; k0 += BK) {
     store();
     __syncthreads();
     if (k0 + BK < p.K) load(k0 + BK);
-    bf16x8 bo[3][4];
+    bf16x8 bo[3][NJ];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int s = 0; s < 3; ++s) bo[s][j] = *reinterpret_cast<const bf16x8*>(&Bs[s][wc + 16 * j + lr][lk]);
 #pragma unroll
@@ -113,7 +165,7 @@ k_cgemm_bf16(Bf16Gemm p) {
 #pragma unroll
       for (int s = 0; s < 3; ++s) ao[s] = *reinterpret_cast<const bf16x8*>(&As[s][wr + 16 * i + lr][lk]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int s = 0; s < 3; ++s)
           acc[s][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ao[s], bo[s][j], acc[s][i][j], 0, 0, 0);
@@ -124,7 +176,7 @@ k_cgemm_bf16(Bf16Gemm p) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NJ; ++j) {
       const int64_t col = n0 + wc + 16 * j + lr;
       if (col >= p.N) continue;
 #pragma unroll
@@ -154,8 +206,19 @@ void launch_cgemm_bf16(const GemmParams<float>& g, hipStream_t s) {
   else throw std::runtime_error("launch_cgemm_bf16: opB must be 'N' or 'C'");
   p.strideA = g.strideA; p.strideB = g.strideB; p.ldc = g.ldc; p.strideC = g.strideC;
   p.M = g.m; p.N = g.n; p.K = g.k;
-  const dim3 grid((unsigned)ceil_div(g.m, BT), (unsigned)ceil_div(g.n, BT), (unsigned)g.batch);
-  hipLaunchKernelGGL(k_cgemm_bf16, grid, dim3(256), 0, s, p);
+  // the 16-byte pair loads need a unit k stride, even leading dimensions and
+  // strides and 16-byte aligned bases
+  auto pairs_ok = [](const void* base, int64_t ld, int64_t stride, int64_t sk) {
+    return sk == 1 && ld % 2 == 0 && stride % 2 == 0 && (reinterpret_cast<uintptr_t>(base) & 15) == 0;
+  };
+  const bool ka = pairs_ok(p.A, p.sam, p.strideA, p.sak);
+  const bool kb = pairs_ok(p.B, p.sbn, p.strideB, p.sbk);
+  constexpr int TN = kBf16TileN;
+  const dim3 grid((unsigned)ceil_div(g.m, BT), (unsigned)ceil_div(g.n, TN), (unsigned)g.batch);
+  if (ka && kb) hipLaunchKernelGGL((k_cgemm_bf16<true, true, TN>), grid, dim3(256), 0, s, p);
+  else if (ka) hipLaunchKernelGGL((k_cgemm_bf16<true, false, TN>), grid, dim3(256), 0, s, p);
+  else if (kb) hipLaunchKernelGGL((k_cgemm_bf16<false, true, TN>), grid, dim3(256), 0, s, p);
+  else hipLaunchKernelGGL((k_cgemm_bf16<false, false, TN>), grid, dim3(256), 0, s, p);
   PPH_CHECK_LAUNCH();
 }
 
