@@ -757,8 +757,9 @@ static void cluster_rayleigh_ritz(Handle& h, Lane& L, int64_t B, const cplx<doub
                                   size_t scratch_bytes, hipStream_t s) {
   const int64_t n = h.n;
   const bool forms = h.use_sparse();
-  constexpr int64_t P = kClusterSlots;
-  const size_t cv_bytes = (size_t)B * n * P * sizeof(cplx<double>);
+  constexpr int64_t P = kClusterSlots;      // per-matrix stride of the slot forms
+  const int64_t S = cluster_slots(n);          // slots in use
+  const size_t cv_bytes = (size_t)B * n * S * sizeof(cplx<double>);
   if (scratch_bytes < cv_bytes) L.cvec.ensure(cv_bytes);
   L.cform.ensure((size_t)B * 4 * P * sizeof(double));
   const int32_t* meta = L.cmeta.as<int32_t>();
@@ -769,24 +770,24 @@ static void cluster_rayleigh_ritz(Handle& h, Lane& L, int64_t B, const cplx<doub
   v = v && forms;
   launch_fill_zero(numc, B * 4 * P * (int64_t)sizeof(double), s);
   if (forms) {
-    launch_cluster_gather<Ts>(meta, src, srs, sks, n * mcap, cv, P, 1, n, B, s);
-    L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, P, B, v ? 4 : 1));
+    launch_cluster_gather<Ts>(meta, src, srs, sks, n * mcap, cv, S, 1, n, S, B, s);
+    L.sws.ensure((size_t)sparse_dots_ws_bytes(h.N, S, B, v ? 4 : 1));
     launch_sparse_dots<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(), h.sp_nblk,
-                               h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1, cv, P,
-                               n * P, h.N, P, B, numc, P, v ? vc : nullptr, P, 3 * P,
+                               h.sp_phit.as<double>(), L.bv.as<cplx<double>>(), 0, v ? 4 : 1, cv, S,
+                               n * S, h.N, S, B, numc, P, v ? vc : nullptr, P, 3 * P,
                                L.sws.as<double>(), s);
   } else {
-    launch_cluster_gather<Ts>(meta, src, srs, sks, n * mcap, cv, 1, n, n, B, s);
+    launch_cluster_gather<Ts>(meta, src, srs, sks, n * mcap, cv, 1, n, n, S, B, s);
     GemmParams<double> g{};
     g.opA = 'N'; g.opB = 'N';
-    g.m = n; g.n = P; g.k = n;
+    g.m = n; g.n = S; g.k = n;
     g.alpha = cplx<double>{1, 0};
     g.A = D64; g.lda = h.ldA; g.strideA = h.ldA * n;
-    g.B = cv; g.ldb = n; g.strideB = n * P;
+    g.B = cv; g.ldb = n; g.strideB = n * S;
     g.batch = B;
     g.dot_out = numc; g.dot_stride = P;
     g.herm_dot = 1;
-    g.dotX = cv; g.lddx = n; g.strideX = n * P;
+    g.dotX = cv; g.lddx = n; g.strideX = n * S;
     L.gws.ensure((size_t)gemm_ws_bytes(g));
     g.ws = L.gws.p;
     launch_gemm<double>(g, s);
@@ -802,11 +803,16 @@ static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D
   const int64_t n = h.n;
   const bool forms = h.use_sparse();
   if (!forms && !D64) throw std::runtime_error("correct_vectors: no FP64 matrices");
-  const int64_t Bc = std::min<int64_t>(B, kVecRefChunk);
-  // the FP64 mode-contiguous copy of a chunk: in L.Zt (the FP32
-  // mode-contiguous vectors, read only before this step) when it is large
-  // enough, else in a buffer of its own
-  const size_t xt_bytes = (size_t)Bc * n * mcap * sizeof(cplx<double>);
+  // the FP64 copy of a chunk of matrices (mode-contiguous vectors, or D X
+  // for dense D): in L.Zt (the FP32 mode-contiguous vectors, read only
+  // before this step; room for half the batch) when it holds the chunk,
+  // else in a buffer of its own.  A chunk is as many matrices as L.Zt or
+  // 1 GiB holds, at least kVecRefChunk, so small matrices run in one or two
+  // chunks instead of many launches of a few matrices each.
+  const size_t per_mat = (size_t)n * mcap * sizeof(cplx<double>);
+  const size_t room = std::max<size_t>(L.Zt.bytes, (size_t)1 << 30);
+  const int64_t Bc = std::min<int64_t>(B, std::max<int64_t>(kVecRefChunk, (int64_t)(room / per_mat)));
+  const size_t xt_bytes = (size_t)Bc * per_mat;
   if (L.Zt.bytes < xt_bytes) L.vxt.ensure(xt_bytes);
   L.vtol.ensure((size_t)B * sizeof(double));
   cplx<double>* X64 = L.Z64.as<cplx<double>>();
@@ -1216,7 +1222,7 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
       const bool crr = all_modes && h.cluster_rr && mmax > 1;
       if (crr) {
         L.cmeta.ensure((size_t)B * kClusterMeta * sizeof(int32_t));
-        launch_cluster_find(te.w, mmax, mcap, L.cmeta.as<int32_t>(), B, s);
+        launch_cluster_find(te.w, mmax, mcap, cluster_slots(n), L.cmeta.as<int32_t>(), B, s);
       }
       // the reduced matrices A are dead here: scratch for the steps below
       const size_t a_bytes = (size_t)B * h.ldA * n * sizeof(cplx<T>);

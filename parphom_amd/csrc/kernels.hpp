@@ -2,6 +2,8 @@
 #pragma once
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace pph {
 
 // ---------------------------------------------------------------- assembly (K1, K9)
@@ -71,18 +73,23 @@ constexpr double kClusterTol = 1e-5;
 // per-matrix cluster list: [0] clusters, [1] slots used, then per cluster
 // (first mode, size, first slot), then per slot (mode i, 2 mode j + kind)
 constexpr int64_t kClusterMeta = 2 + 3 * kClusterCount + 2 * kClusterSlots;
-// meta (B x kClusterMeta) from lam (B x mcap, modes 0..M-1)
-void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int32_t* meta, int64_t B,
-                         hipStream_t s);
+// the slots a matrix of order n uses (its clusters are fewer than n / 2
+// pairs in practice): up to kClusterSlots
+inline int64_t cluster_slots(int64_t n) {
+  return std::min<int64_t>(kClusterSlots, std::max<int64_t>(64, (2 * n + 63) / 64 * 64));
+}
+// meta (B x kClusterMeta) from lam (B x mcap, modes 0..M-1), at most `slots`
+// slots per matrix
+void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int64_t slots, int32_t* meta,
+                         int64_t B, hipStream_t s);
 // the combination vectors x_i + x_j (kind 0) and x_i + i x_j (kind 1) of every
-// slot in FP64, zero for unused slots: vector p of matrix b, row r at
-// dst[b * n * kClusterSlots + r * drs + p * dps]; the modes from the vectors
-// (FP32 or FP64), element (r, k) of matrix b at src[b * sstride + r * srs +
-// k * sks]
+// slot p < S in FP64, zero for unused slots: vector p of matrix b, row r at
+// dst[b * n * S + r * drs + p * dps]; the modes from the vectors (FP32 or
+// FP64), element (r, k) of matrix b at src[b * sstride + r * srs + k * sks]
 template <typename Ts>
 void launch_cluster_gather(const int32_t* meta, const cplx<Ts>* src, int64_t srs, int64_t sks,
                            int64_t sstride, cplx<double>* dst, int64_t drs, int64_t dps, int64_t n,
-                           int64_t B, hipStream_t s);
+                           int64_t S, int64_t B, hipStream_t s);
 // The k x k problems of the clusters: H and the velocity forms from the
 // diagonal forms (num, vacc: raw forms (b * 3 + a) * mcap + k) and the slot
 // forms (numc: b * kClusterSlots + p; vc: (b * 3 + a) * kClusterSlots + p),

@@ -234,7 +234,8 @@ void launch_low_rr(double* lam, const double* nrm, const cplx<double>* C1, const
 // longer than kClusterMax is cut into pieces; a cluster that would overflow
 // the slots is skipped (its modes keep their quotients).
 __global__ void __launch_bounds__(256) k_cluster_find(const double* __restrict__ lam, int64_t M,
-                                                      int64_t mcap, int32_t* __restrict__ meta) {
+                                                      int64_t mcap, int slots,
+                                                      int32_t* __restrict__ meta) {
   __shared__ double sh[8];
   const int64_t b = blockIdx.x;
   const double* lb = lam + b * mcap;
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(256) k_cluster_find(const double* __restrict__
     int64_t e = k + 1;
     while (e < M && e - k < kClusterMax && fabs(lb[e] - lb[e - 1]) <= tol) ++e;
     const int sz = (int)(e - k);
-    if (sz > 1 && ns + sz * (sz - 1) <= kClusterSlots) {
+    if (sz > 1 && ns + sz * (sz - 1) <= slots) {
       cl[3 * ncl] = (int32_t)k;
       cl[3 * ncl + 1] = sz;
       cl[3 * ncl + 2] = ns;
@@ -276,12 +277,13 @@ __global__ void __launch_bounds__(256) k_cluster_gather(const int32_t* __restric
                                                         const cplx<Ts>* __restrict__ src,
                                                         int64_t srs, int64_t sks, int64_t sstride,
                                                         cplx<double>* __restrict__ dst, int64_t drs,
-                                                        int64_t dps, int64_t n, int slot_fast) {
+                                                        int64_t dps, int64_t n, int64_t S,
+                                                        int slot_fast) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.y;
-  if (t >= n * kClusterSlots) return;
-  const int64_t r = slot_fast ? t / kClusterSlots : t % n;
-  const int64_t p = slot_fast ? t % kClusterSlots : t / n;
+  if (t >= n * S) return;
+  const int64_t r = slot_fast ? t / S : t % n;
+  const int64_t p = slot_fast ? t % S : t / n;
   const int32_t* mb = meta + b * kClusterMeta;
   cplx<double> u{0.0, 0.0};
   if (p < mb[1]) {
@@ -292,7 +294,7 @@ __global__ void __launch_bounds__(256) k_cluster_gather(const int32_t* __restric
     if (sl[1] & 1) u = cplx<double>{(double)xi.x - (double)xj.y, (double)xi.y + (double)xj.x};
     else u = cplx<double>{(double)xi.x + (double)xj.x, (double)xi.y + (double)xj.y};
   }
-  dst[b * n * kClusterSlots + r * drs + p * dps] = u;
+  dst[b * n * S + r * drs + p * dps] = u;
 }
 
 // index of the entry (i, j), i <= j, in the packed upper triangle of a
@@ -419,31 +421,33 @@ __global__ void __launch_bounds__(256) k_cluster_rr(const int32_t* __restrict__ 
   }
 }
 
-void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int32_t* meta, int64_t B,
-                         hipStream_t s) {
+void launch_cluster_find(const double* lam, int64_t M, int64_t mcap, int64_t slots, int32_t* meta,
+                         int64_t B, hipStream_t s) {
   if (B <= 0 || M <= 0) return;
   if (M > INT32_MAX / 2) throw std::runtime_error("launch_cluster_find: too many modes");
-  hipLaunchKernelGGL(k_cluster_find, dim3((unsigned)B), dim3(256), 0, s, lam, M, mcap, meta);
+  if (slots < 2 || slots > kClusterSlots) throw std::runtime_error("launch_cluster_find: slots out of range");
+  hipLaunchKernelGGL(k_cluster_find, dim3((unsigned)B), dim3(256), 0, s, lam, M, mcap, (int)slots, meta);
   PPH_CHECK_LAUNCH();
 }
 
 template <typename Ts>
 void launch_cluster_gather(const int32_t* meta, const cplx<Ts>* src, int64_t srs, int64_t sks,
                            int64_t sstride, cplx<double>* dst, int64_t drs, int64_t dps, int64_t n,
-                           int64_t B, hipStream_t s) {
+                           int64_t S, int64_t B, hipStream_t s) {
   if (B <= 0 || n <= 0) return;
+  if (S < 2 || S > kClusterSlots) throw std::runtime_error("launch_cluster_gather: slots out of range");
   const int slot_fast = dps == 1 ? 1 : 0;
-  hipLaunchKernelGGL(k_cluster_gather<Ts>, dim3((unsigned)ceil_div(n * kClusterSlots, 256), (unsigned)B),
-                     dim3(256), 0, s, meta, src, srs, sks, sstride, dst, drs, dps, n, slot_fast);
+  hipLaunchKernelGGL(k_cluster_gather<Ts>, dim3((unsigned)ceil_div(n * S, 256), (unsigned)B),
+                     dim3(256), 0, s, meta, src, srs, sks, sstride, dst, drs, dps, n, S, slot_fast);
   PPH_CHECK_LAUNCH();
 }
 
 template void launch_cluster_gather<float>(const int32_t*, const cplx<float>*, int64_t, int64_t,
                                            int64_t, cplx<double>*, int64_t, int64_t, int64_t,
-                                           int64_t, hipStream_t);
+                                           int64_t, int64_t, hipStream_t);
 template void launch_cluster_gather<double>(const int32_t*, const cplx<double>*, int64_t, int64_t,
                                             int64_t, cplx<double>*, int64_t, int64_t, int64_t,
-                                            int64_t, hipStream_t);
+                                            int64_t, int64_t, hipStream_t);
 
 void launch_cluster_rr(const int32_t* meta, cplx<double>* Z64, cplx<float>* Z32, int64_t n,
                        int64_t mcap, double* lam, double* den, const double* num,
