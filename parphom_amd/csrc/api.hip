@@ -732,7 +732,8 @@ static void complement_correction(Handle& h, Lane& L, int64_t B, const cplx<doub
 // h.vec_refine steps of X <- X + X E with E from M = X^H (D X - X Theta)
 // (the first with full products, later ones over a band of close modes),
 // on the FP64 vectors Z64 and their FP32 copy L.Z (both updated), values
-// theta (the refined quotients, kept) and |x|^2 in L.den (refreshed).  R32
+// theta (the refined quotients, kept) and |x|^2 in L.den (refreshed);
+// fp32_exact: X64 still equals the FP32 vectors of L.Z and L.Zt.  R32
 // and Em are scratch of B n mcap complex64 each, the caller's dead buffers
 // (the reduced matrices and the tridiagonal vectors).  The residuals and the
 // velocity forms run over chunks of kVecRefChunk matrices (one FP64
@@ -799,7 +800,7 @@ static void cluster_rayleigh_ritz(Handle& h, Lane& L, int64_t B, const cplx<doub
 
 static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D64,
                             cplx<float>* R32, cplx<float>* Em, double* theta, int64_t mmax,
-                            int64_t mcap, bool want_num, hipStream_t s) {
+                            int64_t mcap, bool want_num, bool fp32_exact, hipStream_t s) {
   const int64_t n = h.n;
   const bool forms = h.use_sparse();
   if (!forms && !D64) throw std::runtime_error("correct_vectors: no FP64 matrices");
@@ -822,12 +823,20 @@ static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D
   const int64_t sz = n * mcap;              // per-matrix stride of the n x mcap arrays
   launch_lam_tol(theta, mmax, mcap, kClusterTol, L.vtol.as<double>(), B, s);
   for (int step = 0; step < h.vec_refine; ++step) {
-    // R = D X - X Theta in FP64, rounded to FP32
-    for (int64_t b0 = 0; b0 < B; b0 += Bc) {
+    // R = D X - X Theta in FP64, rounded to FP32.  Before the first step the
+    // vectors are still the FP32 ones, whose mode-contiguous copy L.Zt holds
+    // (the block-sparse Rayleigh numerators read it): the whole batch at once
+    if (step == 0 && fp32_exact && forms) {
+      launch_sparse_residual<float>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(),
+                                    h.sp_tptr.as<int64_t>(), h.sp_trow.as<int32_t>(),
+                                    h.sp_tblk.as<int64_t>(), h.sp_nblk, h.sp_phit.as<double>(),
+                                    L.bv.as<cplx<double>>(), L.Zt.as<cplx<float>>(), mcap, sz, theta,
+                                    mcap, h.N, mmax, R32, mcap, sz, B, s);
+    } else for (int64_t b0 = 0; b0 < B; b0 += Bc) {
       const int64_t bc = std::min(Bc, B - b0);
       if (forms) {
         launch_transpose<double>(X64 + b0 * sz, n, mmax, n, sz, Xt, mcap, sz, bc, s);
-        launch_sparse_residual(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(),
+        launch_sparse_residual<double>(h.sp_rowptr.as<int64_t>(), h.sp_col.as<int32_t>(),
                                h.sp_tptr.as<int64_t>(), h.sp_trow.as<int32_t>(),
                                h.sp_tblk.as<int64_t>(), h.sp_nblk, h.sp_phit.as<double>(),
                                L.bv.as<cplx<double>>() + b0 * h.sp_nblk * 4, Xt, mcap, sz,
@@ -914,7 +923,10 @@ static void correct_vectors(Handle& h, Lane& L, int64_t B, const cplx<double>* D
   }
   // the forms of the corrected vectors: with velocities and block-sparse
   // forms, the velocity forms; with want_num, the Rayleigh numerators (the
-  // diagonal of X^H D X for a cluster step that follows)
+  // diagonal of X^H D X for a cluster step that follows).  In FP64 on the
+  // FP64 vectors: the cluster step combines them with forms of FP64
+  // combinations of the same vectors, whose differences must not carry the
+  // FP32 rounding of either
   const bool v = forms && h.opts.vel != 0;
   if (want_num) launch_fill_zero(L.num.p, B * mcap * (int64_t)sizeof(double), s);
   if (forms && (v || want_num)) {
@@ -1235,8 +1247,9 @@ static void eig_batch(Handle& h, Lane& L, int64_t B, cplx<T>* A, const cplx<doub
                                        A, a_bytes, s);
       }
       if (vref) {
+        // the vectors are still the FP32 ones (no cluster step ran on them)
         correct_vectors(h, L, B, D64, reinterpret_cast<cplx<float>*>(A),
-                        reinterpret_cast<cplx<float>*>(te.zt), te.w, mmax, mcap, true, s);
+                        reinterpret_cast<cplx<float>*>(te.zt), te.w, mmax, mcap, true, true, s);
         // the Rayleigh quotients of the corrected vectors
         launch_rq_finish(L.num.as<double>(), L.den.as<double>(), te.w, B, mmax, mcap, s);
         if (crr)

@@ -102,10 +102,12 @@ void launch_cluster_rr(const int32_t* meta, cplx<double>* Z64, cplx<float>* Z32,
                        hipStream_t s);
 // FP64 residual correction of the vectors (vecref.hip).  Rc = R^H
 // mode-contiguous in FP32 (row r of matrix b at b * strideR + r * ldr) with R
-// = D X - X Theta from the listed blocks; Xt the FP64 vectors mode-contiguous.
+// = D X - X Theta from the listed blocks; Xt the vectors (FP32 or FP64)
+// mode-contiguous.
+template <typename Tx>
 void launch_sparse_residual(const int64_t* rowptr, const int32_t* bcol, const int64_t* tptr,
                             const int32_t* trow, const int64_t* tblk, int64_t nblk,
-                            const double* phit, const cplx<double>* bph, const cplx<double>* Xt,
+                            const double* phit, const cplx<double>* bph, const cplx<Tx>* Xt,
                             int64_t ldt, int64_t strideT, const double* theta, int64_t ldth,
                             int64_t N, int64_t M, cplx<float>* Rc, int64_t ldr, int64_t strideR,
                             int64_t B, hipStream_t s);

@@ -30,29 +30,34 @@
 namespace pph {
 
 // Rc = conj(R)^T, mode-contiguous, in FP32: Rc[b * strideR + r * ldr + i] =
-// conj((D x_i)(r) - theta_i x_i(r)) for modes i < M.  Xt: the vectors in FP64,
-// mode-contiguous (row r of matrix b at b * strideT + r * ldt).  Thread = mode
+// conj((D x_i)(r) - theta_i x_i(r)) for modes i < M.  Xt: the vectors (FP32
+// or FP64, widened to FP64), mode-contiguous (row r of matrix b at b * strideT + r * ldt).  Thread = mode
 // i of the chunk blockIdx.y, grid.x = atom row a (varying fastest, so the
 // workgroups in flight share one chunk of modes and its rows stay in the
 // caches): row a of D is its listed blocks (a, c), c <= a, and the Hermitian
 // partners of the blocks (c', a), c' > a (transposed lists).
+template <typename Tx>
 __global__ void __launch_bounds__(256) k_sparse_residual(
     const int64_t* __restrict__ rowptr, const int32_t* __restrict__ bcol,
     const int64_t* __restrict__ tptr, const int32_t* __restrict__ trow,
     const int64_t* __restrict__ tblk, int64_t nblk, const double* __restrict__ phit,
-    const cplx<double>* __restrict__ bph, const cplx<double>* __restrict__ Xt, int64_t ldt,
+    const cplx<double>* __restrict__ bph, const cplx<Tx>* __restrict__ Xt, int64_t ldt,
     int64_t strideT, const double* __restrict__ theta, int64_t ldth, int64_t M,
     cplx<float>* __restrict__ Rc, int64_t ldr, int64_t strideR) {
   const int64_t i = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
   const int64_t a = blockIdx.x, b = blockIdx.z;
   if (i >= M) return;
-  const cplx<double>* X = Xt + b * strideT + i;
+  const cplx<Tx>* X = Xt + b * strideT + i;
   const cplx<double>* P = bph + b * nblk * 4;
   double yr[3] = {0, 0, 0}, yi[3] = {0, 0, 0};
+  auto ld = [&](int64_t r) {
+    const cplx<Tx> x = X[r * ldt];
+    return cplx<double>{(double)x.x, (double)x.y};
+  };
   auto add = [&](const double* F, cplx<double> p, int64_t j, bool tr) {
     cplx<double> v[3];
 #pragma unroll
-    for (int be = 0; be < 3; ++be) v[be] = X[(3 * j + be) * ldt];
+    for (int be = 0; be < 3; ++be) v[be] = ld(3 * j + be);
 #pragma unroll
     for (int al = 0; al < 3; ++al) {
       double ur = 0.0, ui = 0.0;
@@ -78,7 +83,7 @@ __global__ void __launch_bounds__(256) k_sparse_residual(
   cplx<float>* out = Rc + b * strideR + i;
 #pragma unroll
   for (int al = 0; al < 3; ++al) {
-    const cplx<double> x = X[(3 * a + al) * ldt];
+    const cplx<double> x = ld(3 * a + al);
     out[(3 * a + al) * ldr] = cplx<float>{(float)(yr[al] - th * x.x), (float)(-(yi[al] - th * x.y))};
   }
 }
@@ -295,18 +300,30 @@ __global__ void __launch_bounds__(256) k_lam_tol(const double* __restrict__ lam,
   if (threadIdx.x == 0) tol[b] = scale * mx;
 }
 
+template <typename Tx>
 void launch_sparse_residual(const int64_t* rowptr, const int32_t* bcol, const int64_t* tptr,
                             const int32_t* trow, const int64_t* tblk, int64_t nblk,
-                            const double* phit, const cplx<double>* bph, const cplx<double>* Xt,
+                            const double* phit, const cplx<double>* bph, const cplx<Tx>* Xt,
                             int64_t ldt, int64_t strideT, const double* theta, int64_t ldth,
                             int64_t N, int64_t M, cplx<float>* Rc, int64_t ldr, int64_t strideR,
                             int64_t B, hipStream_t s) {
   if (N <= 0 || M <= 0 || B <= 0) return;
-  hipLaunchKernelGGL(k_sparse_residual, dim3((unsigned)N, (unsigned)ceil_div(M, 256), (unsigned)B),
+  hipLaunchKernelGGL(k_sparse_residual<Tx>, dim3((unsigned)N, (unsigned)ceil_div(M, 256), (unsigned)B),
                      dim3(256), 0, s, rowptr, bcol, tptr, trow, tblk, nblk, phit, bph, Xt, ldt,
                      strideT, theta, ldth, M, Rc, ldr, strideR);
   PPH_CHECK_LAUNCH();
 }
+
+template void launch_sparse_residual<float>(const int64_t*, const int32_t*, const int64_t*,
+                                            const int32_t*, const int64_t*, int64_t, const double*,
+                                            const cplx<double>*, const cplx<float>*, int64_t, int64_t,
+                                            const double*, int64_t, int64_t, int64_t, cplx<float>*,
+                                            int64_t, int64_t, int64_t, hipStream_t);
+template void launch_sparse_residual<double>(const int64_t*, const int32_t*, const int64_t*,
+                                             const int32_t*, const int64_t*, int64_t, const double*,
+                                             const cplx<double>*, const cplx<double>*, int64_t,
+                                             int64_t, const double*, int64_t, int64_t, int64_t,
+                                             cplx<float>*, int64_t, int64_t, int64_t, hipStream_t);
 
 void launch_dense_residual(const cplx<double>* Y, const cplx<double>* X, const double* theta,
                            int64_t ldth, int64_t n, int64_t M, int64_t stride, cplx<float>* R,
