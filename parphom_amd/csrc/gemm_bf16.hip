@@ -11,10 +11,17 @@
 // the three planes Ar, Ai, Ar + Ai (and the same of B) are formed in FP32
 // and rounded once.  v_mfma_f32_16x16x32_bf16: lane l supplies row l % 16 of
 // A and column l % 16 of B at k = 8 (l / 16) .. + 7, and holds rows
-// 4 (l / 16) + r of column l % 16 of the result.  Tile 128 x 128 per
-// workgroup of 4 waves (2 x 2, each 64 x 64 = 4 x 4 MFMA blocks), K in steps
+// 4 (l / 16) + r of column l % 16 of the result.  Tile 128 x 64 per
+// workgroup of 4 waves (2 x 2, each 64 x 32 = 4 x 2 MFMA blocks), K in steps
 // of 32; the next step's global loads are in flight while the current one
 // is multiplied.
+//
+// Tile order: the dispatcher hands workgroup i to XCD i mod 8, and each XCD
+// has its own L2.  The kernel renumbers its workgroups so that XCD x takes a
+// contiguous eighth of the (matrix, tile) list, walked in groups of 8 row
+// tiles by all column tiles: the workgroups in flight on one XCD then share
+// 8 row panels of A and a few column panels of B in that XCD's L2, instead
+// of every XCD streaming every panel.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -28,6 +35,8 @@ constexpr int BT = 128;        // tile rows and columns
 constexpr int BK = 32;         // K per step (one MFMA)
 constexpr int BKP = BK + 8;    // LDS row length in BF16 (16 bytes of padding)
 constexpr int kBf16TileN = 64; // tile columns (64: two waves per SIMD)
+constexpr int kXcds = 8;       // XCDs of an MI355X (round-robin workgroup dispatch)
+constexpr int kGroupM = 8;     // row tiles per group of the tile order
 
 struct Bf16Gemm {
   // op(A)(m, k) = A[m * sam + k * sak], conjugated when ca; op(B)(k, n) =
@@ -36,7 +45,7 @@ struct Bf16Gemm {
   const cplx<float>* B;
   cplx<float>* C;
   int64_t sam, sak, sbk, sbn, strideA, strideB, ldc, strideC;
-  int64_t M, N, K;
+  int64_t M, N, K, batch;
   int ca, cb;
 };
 
@@ -54,8 +63,15 @@ k_cgemm_bf16(Bf16Gemm p) {
   __shared__ __align__(16) __bf16 As[3][BT][BKP];
   __shared__ __align__(16) __bf16 Bs[3][TN][BKP];
   const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-  const int64_t b = blockIdx.z;
-  const int64_t m0 = (int64_t)blockIdx.x * BT, n0 = (int64_t)blockIdx.y * TN;
+  // (matrix, row tile, column tile) of this workgroup (tile order above)
+  const int64_t gm = (p.M + BT - 1) / BT, gn = (p.N + TN - 1) / TN, per = gm * gn;
+  const int64_t total = per * p.batch, whole = total / kXcds * kXcds;
+  int64_t lid = blockIdx.x;
+  if (lid < whole) lid = (lid % kXcds) * (total / kXcds) + lid / kXcds;
+  const int64_t b = lid / per, rt = lid % per;
+  const int64_t first = rt / (kGroupM * gn) * kGroupM, rows = min(gm - first, (int64_t)kGroupM);
+  const int64_t m0 = (first + rt % (kGroupM * gn) % rows) * BT;
+  const int64_t n0 = rt % (kGroupM * gn) / rows * TN;
   const cplx<float>* A = p.A + b * p.strideA;
   const cplx<float>* B = p.B + b * p.strideB;
   // 16 elements of each operand per thread and K step: k-contiguous, pair e
@@ -205,7 +221,7 @@ void launch_cgemm_bf16(const GemmParams<float>& g, hipStream_t s) {
   else if (g.opB == 'C') { p.sbk = g.ldb; p.sbn = 1; p.cb = 1; }
   else throw std::runtime_error("launch_cgemm_bf16: opB must be 'N' or 'C'");
   p.strideA = g.strideA; p.strideB = g.strideB; p.ldc = g.ldc; p.strideC = g.strideC;
-  p.M = g.m; p.N = g.n; p.K = g.k;
+  p.M = g.m; p.N = g.n; p.K = g.k; p.batch = g.batch;
   // the 16-byte pair loads need a unit k stride, even leading dimensions and
   // strides and 16-byte aligned bases
   auto pairs_ok = [](const void* base, int64_t ld, int64_t stride, int64_t sk) {
@@ -214,7 +230,9 @@ void launch_cgemm_bf16(const GemmParams<float>& g, hipStream_t s) {
   const bool ka = pairs_ok(p.A, p.sam, p.strideA, p.sak);
   const bool kb = pairs_ok(p.B, p.sbn, p.strideB, p.sbk);
   constexpr int TN = kBf16TileN;
-  const dim3 grid((unsigned)ceil_div(g.m, BT), (unsigned)ceil_div(g.n, TN), (unsigned)g.batch);
+  const int64_t tiles = ceil_div(g.m, BT) * ceil_div(g.n, TN) * g.batch;
+  if (tiles > INT32_MAX) throw std::runtime_error("launch_cgemm_bf16: too many tiles");
+  const dim3 grid((unsigned)tiles);
   if (ka && kb) hipLaunchKernelGGL((k_cgemm_bf16<true, true, TN>), grid, dim3(256), 0, s, p);
   else if (ka) hipLaunchKernelGGL((k_cgemm_bf16<true, false, TN>), grid, dim3(256), 0, s, p);
   else if (kb) hipLaunchKernelGGL((k_cgemm_bf16<false, true, TN>), grid, dim3(256), 0, s, p);

@@ -776,9 +776,13 @@ __device__ __forceinline__ void q2_store(cplx<double> v, __amdgpu_buffer_rsrc_t 
 // own columns only (no cross-wave dependency):
 //   W = V^H Zw   (bw x 16, K = 2bw; reflectors i0.. meet rows i0 .. i0 + bw + 15)
 //   Zw -= VT W   (2bw x 16, K = bw; VT(rr, i) = 0 for i <= rr - bw)
-// with four real MFMAs per complex step (the three-product form's cancellation
-// in the imaginary part costs the FP32 eigenvectors a factor of accuracy).
-// V and VT of a block are shared by the waves: two barriers per block.
+// with three real MFMAs per complex step in FP32 (P1 = Re a Re b, P2 = Im a
+// Im b, P3 = (Re a +- Im a)(Re b + Im b), the operand sums formed in
+// registers) and four in FP64.  The three-product form's imaginary part
+// carries a cancellation that costs the FP32 vectors a small factor of
+// accuracy; on the FP32 path the FP64 correction of the vectors that follows
+// removes it.  V and VT of a block are shared by the waves: two barriers per
+// block.
 // The rows leave and enter the window while the matrix cores work: the next
 // block's V and VT and the rows of its new half are loaded into registers
 // during a block; at the start of the next block the half its predecessor
@@ -789,6 +793,7 @@ __device__ __forceinline__ void q2_store(cplx<double> v, __amdgpu_buffer_rsrc_t 
 // unit's range check (see q2_load).
 template <typename T, int BW, int NWV, bool ROWMAJOR>
 __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
+  constexpr bool M3 = sizeof(T) == 4;               // three real products per complex step
   constexpr int NT = WAVE * NWV;
   constexpr int SW = 16 * NWV;
   constexpr int VE = (2 * BW * BW + NT - 1) / NT;   // V (and VT) elements per thread
@@ -912,9 +917,9 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
           return Zw[kr < BW ? s0 : s1][kr % BW][cw + lane % 16];
         };
         auto ld_v = [&](int kk, int t) { return Vs[kk + lane / 16][16 * t + lane % 16]; };
-        acc_t p1[NRT], p2[NRT];
+        acc_t p1[NRT], p2[NRT], p3[NRT];
 #pragma unroll
-        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; }
+        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; p3[t] = p1[t]; }
         cplx<T> bz = ld_z(0), v[NRT];
 #pragma unroll
         for (int t = 0; t < NRT; ++t) v[t] = live(t, 0) ? ld_v(0, t) : cplx<T>{0, 0};
@@ -930,15 +935,27 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
               if (live(t, kk + 4)) vn[t] = ld_v(kk + 4, t);
           }
           __builtin_amdgcn_sched_barrier(0);
+          if constexpr (M3) {
+            // conj(v) z = P1 + P2 + i (P3 - P1 + P2), P1 = Re v Re z, P2 =
+            // Im v Im z, P3 = (Re v - Im v)(Re z + Im z)
+            const T zs = bz.x + bz.y;
 #pragma unroll
-          for (int t = 0; t < NRT; ++t) {
-            // conj(v) z with four real products (p1 = Re, p2 = Im): the
-            // three-product form loses digits of Im to cancellation
-            if (!live(t, kk)) continue;
-            p1[t] = mfma<T>::op(v[t].x, bz.x, p1[t]);
-            p1[t] = mfma<T>::op(v[t].y, bz.y, p1[t]);
-            p2[t] = mfma<T>::op(v[t].x, bz.y, p2[t]);
-            p2[t] = mfma<T>::op(-v[t].y, bz.x, p2[t]);
+            for (int t = 0; t < NRT; ++t) {
+              if (!live(t, kk)) continue;
+              p1[t] = mfma<T>::op(v[t].x, bz.x, p1[t]);
+              p2[t] = mfma<T>::op(v[t].y, bz.y, p2[t]);
+              p3[t] = mfma<T>::op(v[t].x - v[t].y, zs, p3[t]);
+            }
+          } else {
+#pragma unroll
+            for (int t = 0; t < NRT; ++t) {
+              // conj(v) z with four real products (p1 = Re, p2 = Im)
+              if (!live(t, kk)) continue;
+              p1[t] = mfma<T>::op(v[t].x, bz.x, p1[t]);
+              p1[t] = mfma<T>::op(v[t].y, bz.y, p1[t]);
+              p2[t] = mfma<T>::op(v[t].x, bz.y, p2[t]);
+              p2[t] = mfma<T>::op(-v[t].y, bz.x, p2[t]);
+            }
           }
           if (kk < 4 * NMS) {
 #pragma unroll
@@ -958,7 +975,9 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
         for (int t = 0; t < NRT; ++t)
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            Ws[16 * t + mfma<T>::row(lane, r)][cw + lane % 16] = cplx<T>{p1[t][r], p2[t][r]};
+            Ws[16 * t + mfma<T>::row(lane, r)][cw + lane % 16] =
+                M3 ? cplx<T>{p1[t][r] + p2[t][r], p3[t][r] - p1[t][r] + p2[t][r]}
+                   : cplx<T>{p1[t][r], p2[t][r]};
       }
       wave_sync();
       // Zw -= VT W on the wave's columns: the row tiles share the W operand;
@@ -970,9 +989,9 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
         auto live = [](int t, int kk) { return kk + 3 >= 16 * t - BW + 1; };
         auto ld_w = [&](int kk) { return Ws[kk + lane / 16][cw + lane % 16]; };
         auto ld_a = [&](int kk, int t) { return VTs[kk + lane / 16][16 * t + lane % 16]; };
-        acc_t p1[NRT], p2[NRT];
+        acc_t p1[NRT], p2[NRT], p3[NRT];
 #pragma unroll
-        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; }
+        for (int t = 0; t < NRT; ++t) { p1[t] = acc_t{0, 0, 0, 0}; p2[t] = p1[t]; p3[t] = p1[t]; }
         cplx<T> wv = ld_w(0), a[NRT];
 #pragma unroll
         for (int t = 0; t < NRT; ++t) a[t] = live(t, 0) ? ld_a(0, t) : cplx<T>{0, 0};
@@ -988,13 +1007,25 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
               if (live(t, kk + 4)) an[t] = ld_a(kk + 4, t);
           }
           __builtin_amdgcn_sched_barrier(0);
+          if constexpr (M3) {
+            // a w = P1 - P2 + i (P3 - P1 - P2)
+            const T ws = wv.x + wv.y;
 #pragma unroll
-          for (int t = 0; t < NRT; ++t) {
-            if (!live(t, kk)) continue;
-            p1[t] = mfma<T>::op(a[t].x, wv.x, p1[t]);
-            p1[t] = mfma<T>::op(-a[t].y, wv.y, p1[t]);
-            p2[t] = mfma<T>::op(a[t].x, wv.y, p2[t]);
-            p2[t] = mfma<T>::op(a[t].y, wv.x, p2[t]);
+            for (int t = 0; t < NRT; ++t) {
+              if (!live(t, kk)) continue;
+              p1[t] = mfma<T>::op(a[t].x, wv.x, p1[t]);
+              p2[t] = mfma<T>::op(a[t].y, wv.y, p2[t]);
+              p3[t] = mfma<T>::op(a[t].x + a[t].y, ws, p3[t]);
+            }
+          } else {
+#pragma unroll
+            for (int t = 0; t < NRT; ++t) {
+              if (!live(t, kk)) continue;
+              p1[t] = mfma<T>::op(a[t].x, wv.x, p1[t]);
+              p1[t] = mfma<T>::op(-a[t].y, wv.y, p1[t]);
+              p2[t] = mfma<T>::op(a[t].x, wv.y, p2[t]);
+              p2[t] = mfma<T>::op(a[t].y, wv.x, p2[t]);
+            }
           }
           __builtin_amdgcn_sched_barrier(0);
           wv = wn;
@@ -1007,7 +1038,8 @@ __global__ void __launch_bounds__(WAVE * NWV) k_q2_apply(Q2Work<T> w) {
           for (int r = 0; r < 4; ++r) {
             const int rr = 16 * t + mfma<T>::row(lane, r);
             cplx<T>& zz = Zw[rr < BW ? s0 : s1][rr % BW][cw + lane % 16];
-            zz = csub(zz, cplx<T>{p1[t][r], p2[t][r]});
+            zz = csub(zz, M3 ? cplx<T>{p1[t][r] - p2[t][r], p3[t][r] - p1[t][r] - p2[t][r]}
+                             : cplx<T>{p1[t][r], p2[t][r]});
           }
       }
       wave_sync();
