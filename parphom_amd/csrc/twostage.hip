@@ -8,9 +8,9 @@
 //          update of the trailing matrix
 //              VT = V T,  X = A VT,  Mx = VT^H X,  Y = X - V Mx / 2,
 //              A <- A - Y V^H - V Y^H
-//          as GEMMs on the matrix cores (four real products per complex one,
-//          for the FP32 path's accuracy), on the lower triangle only (X from
-//          two triangular products).  The trailing matrix is read about three
+//          as GEMMs on the matrix cores (three real products per complex one
+//          in FP32, four in FP64), on the lower triangle only (X from two
+//          triangular products).  The trailing matrix is read about three
 //          times per panel of bw columns instead of once per column.
 //   hb2st  band -> real tridiagonal by bulge chasing (k_hb2st): sweep j
 //          annihilates column j below the subdiagonal with a reflector of
@@ -27,6 +27,15 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "../../include/parphom_hip.h"
+
+namespace pph {
+// he2hb GEMMs with four real products per complex one (FP64) or three (FP32:
+// the three-product form's cancellation in the imaginary part loosens the
+// FP32 reduction by a small factor, which the FP64 refinement of the
+// eigenpairs that follows removes)
+template <typename T>
+constexpr int kHe2hbFourM = sizeof(T) == 8 ? 1 : 0;
+}  // namespace pph
 
 namespace pph {
 
@@ -295,7 +304,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = w.Tp; g.ldb = bw; g.strideB = w.strideTp;
       g.C = w.VT + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
-      g.four_m = 1;
+      g.four_m = kHe2hbFourM<T>;
       launch_gemm<T>(g, s);
     }
     // X = A22 VT   (m x bw), from the lower triangle of A22: its lower part
@@ -309,7 +318,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = w.VT + r0; g.ldb = n; g.strideB = w.strideXV;
       g.C = w.X + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
-      g.four_m = 1;
+      g.four_m = kHe2hbFourM<T>;
       g.tri = 1;
       launch_gemm<T>(g, s);
       g.opA = 'C'; g.tri = 2; g.beta = cplx<T>{1, 0};
@@ -331,7 +340,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B = Mx; g.ldb = bw; g.strideB = w.strideTp;
       g.C = w.X + r0; g.ldc = n; g.strideC = w.strideXV;
       g.batch = B;
-      g.four_m = 1;
+      g.four_m = kHe2hbFourM<T>;
       launch_gemm<T>(g, s);
     }
     // A22 -= Y V^H + V Y^H   (the lower triangle: tiles wholly above the
@@ -347,7 +356,7 @@ void he2hb(const TwoStageWork<T>& w, hipStream_t s) {
       g.B2 = w.X + r0; g.strideB2 = w.strideXV;
       g.C = w.A + r0 + r0 * w.lda; g.ldc = w.lda; g.strideC = w.strideA;
       g.batch = B;
-      g.four_m = 1;
+      g.four_m = kHe2hbFourM<T>;
       g.lower = 1;
       launch_gemm<T>(g, s);
     }
