@@ -46,7 +46,7 @@ def device_outputs(torch, B, n):
 
 
 def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction=-1,
-              tile="64x64", prof_steps=3, vec_refine=-1):
+              tile="64x64", prof_steps=3, vec_refine=-1, vec_bf16=-1):
     """Each step is one pph_run_device call over streams x B q-points: the
     library keeps `streams` batches of B in flight (one stream and host thread
     each).  Returns a dict: the timed seconds, the phase profile of the timed
@@ -61,8 +61,9 @@ def time_path(lib, torch, st, fc, qs, prec, B, steps, warmup, streams, reduction
     h.reduction(reduction)           # -1: the library's choice
     if not (prec == "D" and tile == "128x128"):     # the 128 x 128 tile is FP32 only
         h.gemm_tile(tile)
-    if vec_refine >= 0:              # FP32 path: FP64 correction steps of the vectors
-        h.vector_refinement(vec_refine)
+    if vec_refine >= 0 or vec_bf16 >= 0:     # FP32 path: FP64 correction steps of the vectors
+        h.vector_refinement(vec_refine if vec_refine >= 0 else 2,
+                            bf16=bool(vec_bf16) if vec_bf16 >= 0 else True)
     per_step = B * streams
     out = device_outputs(torch, per_step, n)
     nq = len(qs)
@@ -482,6 +483,9 @@ def main():
     ap.add_argument("--vec-refine", type=int, default=-1, choices=(-1, 0, 1, 2),
                     help="FP64 residual-correction steps of the FP32 path's vectors "
                          "(-1: the library default, 2)")
+    ap.add_argument("--vec-bf16", type=int, default=-1, choices=(-1, 0, 1),
+                    help="products of the first correction step for pairs more than 16 modes "
+                         "apart: 1 BF16 matrix cores, 0 FP32 (-1: the library default, 1)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -529,7 +533,7 @@ def main():
 
     B, S = a.batch, a.streams
     r32 = time_path(lib, torch, st, fc, qs, "S", B, a.steps, a.warmup, S, a.reduction, a.tile,
-                    vec_refine=a.vec_refine)
+                    vec_refine=a.vec_refine, vec_bf16=a.vec_bf16)
     dt32, prof32, iso32, ev32, z32, nblk = (r32["dt"], r32["prof"], r32["iso"], r32["ev"],
                                             r32["z0"], r32["nblk"])
     two, small = r32["two_stage"], r32["small"]
@@ -547,8 +551,9 @@ def main():
                                     "small-matrix kernel" if small else "one-stage"),
                       "gemm_tile": a.tile,
                       "vector_correction_steps": a.vec_refine if a.vec_refine >= 0 else 2,
-                      "vector_correction_first_step": "BF16 matrix cores for pairs more than 16 "
-                                                      "modes apart, FP32 for the band",
+                      "vector_correction_first_step": (
+                          "FP32 products" if a.vec_bf16 == 0 else
+                          "BF16 matrix cores for pairs more than 16 modes apart, FP32 for the band"),
                       "batch_graphs_replayed_in_timed_steps": r32["graph_replays"]}}
     phases, hemv_launch_ms = split_profile(prof32, n)
     res["phase_ms_per_batch"] = {k: round(v, 3) for k, v in phases.items()}

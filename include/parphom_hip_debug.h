@@ -49,8 +49,10 @@ int pph_debug_blocking(void* h, int32_t nb_panel, int32_t nb_back);
 /* Products with D(q) and dD/dq in the Rayleigh quotients and velocities:
  * mode 0 dense (matrix-core GEMMs with the Hermitian-dot epilogue), 1
  * automatic (block-sparse when at most a quarter of the force-constant
- * blocks are nonzero; the default), 2 block-sparse whenever the handle built
- * block lists (N <= 2048 or sparse).  nblk (may be NULL) receives the number
+ * blocks are nonzero, and on precision 'S' with range A, eigenvectors and the
+ * FP64 correction of the vectors, whose velocity forms must be FP64; the
+ * default), 2 block-sparse whenever the handle built block lists (N <= 2048
+ * or sparse).  nblk (may be NULL) receives the number
  * of lower blocks the products now use, 0 for dense.  No counterpart in the
  * reference, whose velocity products are dense (write_output.F90:415-421). */
 int pph_debug_sparse(void* h, int32_t mode, int64_t* nblk);

@@ -267,7 +267,17 @@ struct Handle : Lane {
   int64_t sp_nblk = 0;
   bool sp_ok = false;
   int sp_mode = 1;         // 0 dense products, 1 automatic, 2 block lists whenever built
-  bool use_sparse() const { return sp_nblk > 0 && (sp_mode == 2 || (sp_mode == 1 && sp_ok)); }
+  // FP32 path with all eigenpairs and the FP64 correction of the vectors:
+  // its velocity forms must be FP64 forms of the corrected FP64 vectors, which
+  // the block lists give and the dense FP32 products of their FP32 rounding
+  // do not (1e-6 of the largest velocity off at the acoustic modes of a small
+  // cell, where omega divides the form)
+  bool fp64_forms() const {
+    return opts.precision == 'S' && opts.range == 'A' && opts.jobz == 'V' && vec_refine > 0;
+  }
+  bool use_sparse() const {
+    return sp_nblk > 0 && (sp_mode == 2 || (sp_mode == 1 && (sp_ok || fp64_forms())));
+  }
   // lanes 1.. (created on demand by pph_run)
   std::vector<std::unique_ptr<Lane>> more;
   Lane& lane(size_t i) { return i == 0 ? static_cast<Lane&>(*this) : *more[i - 1]; }

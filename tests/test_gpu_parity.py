@@ -166,11 +166,12 @@ def test_heevx_range_v_fp32_bounds(gpu_available, n):
 
 
 @pytest.mark.parametrize("prec", ["D", "S"])
-@pytest.mark.parametrize("sparse", [1, 2])
+@pytest.mark.parametrize("sparse", [0, 1, 2])
 def test_run_tbg_3_4_path(gpu_available, prec, sparse):
     """Config-1 structure: eigenvalues (<= 1e-3 cm^-1) and velocities, with the
-    dense products (automatic: 30% of its blocks are nonzero) and with the
-    block-sparse quadratic forms forced."""
+    dense products (forced, or automatic on the FP64 path: 30% of its blocks
+    are nonzero) and with the block-sparse quadratic forms (forced, or
+    automatic on the FP32 path, whose corrected vectors need FP64 forms)."""
     lib = _lib()
     st, fc = U.structure(3, 4, 1003)
     S = U.oracle_system(3, 4, 1003)
@@ -178,7 +179,7 @@ def test_run_tbg_3_4_path(gpu_available, prec, sparse):
     qs = synth.q_path_gkmg()[[0, 5, 26, 30, 39, 50]]
     with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
                     precision=prec) as h:
-        assert (h.sparse(sparse) > 0) == (sparse == 2)
+        assert (h.sparse(sparse) > 0) == (sparse == 2 or (sparse == 1 and prec == "S"))
         res = h.run(qs)
     for iq, q in enumerate(qs):
         ref = S.solve(q, vel=True)
