@@ -142,6 +142,41 @@ def test_small_run_tbg_1_2(gpu_available, prec):
                 assert np.all(np.abs(a - b) <= 1e-5 * vmax * len(g) + noise[g].sum())
 
 
+def test_small_fp32_velocities_match_fp64(gpu_available):
+    """The FP32 path with its FP64 correction of the vectors (the default)
+    against the FP64 path on the (1,2) cell, acoustic modes near Gamma
+    included: velocities per mode (sums only over modes closer than 1e-9
+    max|lambda|, degenerate to FP64 precision) within 1e-7 of the largest.
+    omega divides the velocity form, so the lowest modes magnify any error
+    of the form: dense FP32 forms of the corrected vectors' FP32 rounding
+    missed this by an order of magnitude (1.1e-6 of the largest velocity);
+    the automatic choice takes the FP64 block sums on this path."""
+    lib = _lib()
+    st, fc = U.structure(1, 2, 11)
+    qs = np.vstack([[[0, 0.9844, 0], [0.0156, 0, 0], [0, 0.0312, 0], [0.0156, 0.0156, 0],
+                     [0.0312, 0, 0], [0.0625, 0.0312, 0], [0.5, 0, 0], [1 / 3, 1 / 3, 0]],
+                    np.random.default_rng(12).uniform(0, 1, (8, 3)) * [1, 1, 0]])
+    res = {}
+    for prec in ("D", "S"):
+        with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                        precision=prec, batch=16) as h:
+            nblk = h.sparse(1)
+            assert (nblk > 0) == (prec == "S")
+            res[prec] = h.run(qs)
+    vmax = max(np.abs(res["D"]["vel"][iq]).max() for iq in range(len(qs)))
+    for iq in range(len(qs)):
+        assert res["S"]["info"][iq] == 0 and res["D"]["info"][iq] == 0
+        om = res["D"]["eval_cm"][iq]
+        assert np.abs(res["S"]["eval_cm"][iq] - om).max() <= 1e-6
+        lam = np.sign(om) * om ** 2
+        tol = 1e-9 * np.abs(lam).max()
+        cuts = np.flatnonzero(np.diff(lam) > tol) + 1
+        for g in np.split(np.arange(len(lam)), cuts):
+            a = res["S"]["vel"][iq][g].sum(axis=0)
+            b = res["D"]["vel"][iq][g].sum(axis=0)
+            assert np.abs(a - b).max() <= 1e-7 * vmax, (iq, g, a, b)
+
+
 @pytest.mark.parametrize("prec", ["D", "S"])
 def test_small_bit_identical(gpu_available, prec):
     """Two runs of the same batch through the small-matrix kernel return the
