@@ -708,18 +708,24 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
     Tm[a][c] = cplx<double>{0, 0};
   }
   __syncthreads();
-  for (int i = 0; i < BW; ++i) {
-    const cplx<double> ti{(double)ts[i].x, (double)ts[i].y};
-    cplx<double> v{0, 0};
-    if (tid < i) {
-      for (int l = tid; l < i; ++l) v = cadd(v, cmul(Tm[tid][l], G[l][i]));
-      v = cmul(cplx<double>{-ti.x, -ti.y}, v);
+  // T column by column (T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i)): only the
+  // first bw lanes work, so the first wave runs the recurrence alone and
+  // synchronises at wave level; column i is read only after it is written
+  if (tid < WAVE) {
+    for (int i = 0; i < BW; ++i) {
+      const cplx<double> ti{(double)ts[i].x, (double)ts[i].y};
+      if (tid < i) {
+        cplx<double> v{0, 0};
+        for (int l = tid; l < i; ++l) v = cadd(v, cmul(Tm[tid][l], G[l][i]));
+        Tm[tid][i] = cmul(cplx<double>{-ti.x, -ti.y}, v);
+      }
+      if (tid == i) Tm[i][i] = ti;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __syncthreads();
-    if (tid < i) Tm[tid][i] = v;
-    if (tid == i) Tm[i][i] = ti;
-    __syncthreads();
   }
+  __syncthreads();
   // VT(rr, c) = sum_{l <= c} V(rr, l) T(l, c)
   cplx<T>* out = w.VTb + b * w.strideVTb + blk;
   for (int e = tid; e < 2 * BW * BW; e += 256) {
