@@ -684,7 +684,10 @@ template <typename T, int BW>
 __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
   __shared__ cplx<T> Vs[2 * BW][BW + 1];
   __shared__ cplx<T> ts[BW];
-  __shared__ cplx<double> G[BW][BW + 1], Tm[BW][BW + 1];
+  // G = V^H V strictly below the diagonal of M (G(l, i) at M[i][l], l < i),
+  // T on and above it (T(l, i) at M[l][i]): the recurrence for T reads only
+  // G(l, i) with l < i, so one array holds both
+  __shared__ cplx<double> M[BW][BW + 1];
   const int64_t k = blockIdx.x, g = blockIdx.y, b = blockIdx.z;
   const int64_t n = w.n, j0 = g * BW;
   if (j0 >= n - 1 || k >= hb2st_tasks(n, BW, j0)) return;
@@ -694,9 +697,10 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
   const int64_t blk = (g * w.kmax + k) * (int64_t)(2 * BW * BW);
   cplx<T>* vout = w.Vb + b * w.strideVTb + blk;
   for (int e = tid; e < 2 * BW * BW; e += 256) vout[e] = Vs[e / BW][e % BW];
-  // G = V^H V (column i nonzero on rows i .. i + bw - 1)
+  // G = V^H V (column i nonzero on rows i .. i + bw - 1), entries a < c
   for (int e = tid; e < BW * BW; e += 256) {
     const int a = e % BW, c = e / BW;
+    if (a >= c) continue;
     const int lo = max(a, c), hi = min(a, c) + BW;
     double sr = 0.0, si = 0.0;
     for (int rr = lo; rr < hi; ++rr) {
@@ -704,8 +708,7 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
       sr += (double)x.x * y.x + (double)x.y * y.y;
       si += (double)x.x * y.y - (double)x.y * y.x;
     }
-    G[a][c] = cplx<double>{sr, si};
-    Tm[a][c] = cplx<double>{0, 0};
+    M[c][a] = cplx<double>{sr, si};
   }
   __syncthreads();
   // T column by column (T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i)): only the
@@ -716,10 +719,10 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
       const cplx<double> ti{(double)ts[i].x, (double)ts[i].y};
       if (tid < i) {
         cplx<double> v{0, 0};
-        for (int l = tid; l < i; ++l) v = cadd(v, cmul(Tm[tid][l], G[l][i]));
-        Tm[tid][i] = cmul(cplx<double>{-ti.x, -ti.y}, v);
+        for (int l = tid; l < i; ++l) v = cadd(v, cmul(M[tid][l], M[i][l]));
+        M[tid][i] = cmul(cplx<double>{-ti.x, -ti.y}, v);
       }
-      if (tid == i) Tm[i][i] = ti;
+      if (tid == i) M[i][i] = ti;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -733,7 +736,7 @@ __global__ void __launch_bounds__(256) k_q2_prep(Q2Work<T> w) {
     double sr = 0.0, si = 0.0;
     for (int l = max(0, rr - BW + 1); l <= min(c, rr); ++l) {
       const cplx<T> x = Vs[rr][l];
-      const cplx<double> t = Tm[l][c];
+      const cplx<double> t = M[l][c];
       sr += (double)x.x * t.x - (double)x.y * t.y;
       si += (double)x.x * t.y + (double)x.y * t.x;
     }
