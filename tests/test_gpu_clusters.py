@@ -129,6 +129,42 @@ def test_known_spectrum_with_close_pairs(reduction_default, n, reduction, steps,
     assert orth <= orthb
 
 
+def _resolved_dv(r, rD, nq):
+    """Largest velocity difference over the modes the FP64 run resolves (its
+    eigenvalues more than 1e-8 max|lambda| from their neighbours), relative
+    to the largest velocity of the q-point; and the largest frequency
+    difference (cm^-1)."""
+    dv = dw = 0.0
+    for i in range(nq):
+        lam = rD["eval_cm"][i]
+        l2 = np.sign(lam) * lam * lam
+        gap = np.minimum(np.r_[np.inf, np.diff(l2)], np.r_[np.diff(l2), np.inf])
+        ok = gap > 1e-8 * np.abs(l2).max()
+        vD = rD["vel"][i]
+        dv = max(dv, np.abs(r["vel"][i] - vD)[ok].max() / np.abs(vD).max())
+        dw = max(dw, np.abs(r["eval_cm"][i] - lam).max())
+    return dw, dv
+
+
+def test_tbg_8_9_fp32_against_fp64(gpu_available):
+    """The (8,9) cell (n = 2604, two-stage reduction with three real products
+    per complex step) on the default FP32 path: frequencies within 1e-6
+    cm^-1 and velocities of the resolved modes within 1e-6 of the largest,
+    against the FP64 path."""
+    lib = _lib()
+    from parphom_amd import synth
+    st, fc, qs = synth.workload("tbg_8_9")
+    q = qs[[0, 21, 77]]
+    res = {}
+    for prec in ("D", "S"):
+        with lib.Handle(st.lat, st.pos, st.types, st.masses, fc, jobz="V", vel=True,
+                        precision=prec, batch=len(q), streams=1) as h:
+            res[prec] = h.run(q, evec=False)
+    dw, dv = _resolved_dv(res["S"], res["D"], len(q))
+    _log(f"(8,9) default FP32 path: max |d omega| = {dw:.2e} cm^-1, max |d v| / max |v| = {dv:.2e}")
+    assert dw <= 1e-6 and dv <= 1e-6
+
+
 def test_tbg_11_12_fp32_against_fp64(gpu_available):
     lib = _lib()
     from parphom_amd import synth
@@ -148,28 +184,17 @@ def test_tbg_11_12_fp32_against_fp64(gpu_available):
             res[tag] = h.run(q, evec=False)
     dw, dv = {}, {}
     for tag, _, _, _ in runs:
-        dw[tag] = max(np.abs(res[tag]["eval_cm"][i] - res["D"]["eval_cm"][i]).max()
-                      for i in range(len(q)))
-        # velocities of the modes the FP64 path resolves (its eigenvalues
-        # more than 1e-8 max|lambda| from their neighbours), relative to the
-        # largest velocity
-        dv[tag] = 0.0
-        for i in range(len(q)):
-            lam = res["D"]["eval_cm"][i]
-            l2 = np.sign(lam) * lam * lam
-            gap = np.minimum(np.r_[np.inf, np.diff(l2)], np.r_[np.diff(l2), np.inf])
-            ok = gap > 1e-8 * np.abs(l2).max()
-            vD = res["D"]["vel"][i]
-            d = np.abs(res[tag]["vel"][i] - vD)[ok].max() / np.abs(vD).max()
-            dv[tag] = max(dv[tag], d)
+        dw[tag], dv[tag] = _resolved_dv(res[tag], res["D"], len(q))
         _log(f"(11,12) {tag}: max |d omega| against FP64 = {dw[tag]:.2e} cm^-1, "
              f"max |d v| / max |v| = {dv[tag]:.2e}")
-    # measured: 3.9e-4 cm^-1 and 0.40 with neither step (a pair 6.8e-4 cm^-1
-    # apart near 151 cm^-1, fully mixed); 4.2e-6 and 1.1e-2 with the cluster
-    # step alone (the first-order mixing of FP32 vectors with modes just
-    # outside the cluster tolerance, 1e-5 max|lambda| away); 6.4e-5 of max|v|
-    # after one correction step; after two, 6.8e-8 with FP32 products and
-    # 9.2e-8 with the split BF16 / FP32-band first step (the default)
+    # measured (FP32 reduction and Q2 with three real products per complex
+    # step): 4.1e-5 cm^-1 and 9.3e-2 with neither step (pairs closer than the
+    # FP32 resolution come out mixed; 3.9e-4 and 0.40 with four products);
+    # 4.2e-6 and 4.4e-3 with the cluster step alone (the first-order mixing
+    # of FP32 vectors with modes just outside the cluster tolerance, 1e-5
+    # max|lambda| away); 4.7e-5 of max|v| after one correction step; after
+    # two, 1.7e-7 with FP32 products and 1.6e-7 with the split BF16 /
+    # FP32-band first step (the default)
     assert dw["plain"] <= 1e-3
     assert dw["clusters"] <= 2e-5 and dv["clusters"] <= 2e-2
     assert dw["one step"] <= 2e-5 and dv["one step"] <= 2e-4
